@@ -1,0 +1,284 @@
+/*
+ * libdft_mi355x -- MI355X-native Orsay-Trento (OT) hot path of libdft.
+ *
+ * C ABI: plain structs, pointers and sizes.  REAL = double, INT = int64_t
+ * at the API; the device storage precision (FP32 or FP64) is a property of
+ * each grid handle, fixed when the grid is allocated.
+ *
+ * The functional entry points keep the reference names and signatures:
+ *   dft_ot_alloc          src/ot.c:66       dft_ot_free              src/ot.c:268
+ *   dft_ot_potential      src/ot.c:314      dft_ot_potential2        src/ot.c:431
+ *   dft_ot_energy_density src/ot-energy.c:38
+ *   dft_ot_energy_density_kc src/ot-energy.c:123, _bf src/ot-energy.c:202
+ *   dft_ot_backflow_potential src/ot.c:840  dft_ot_init_params       src/ot.c:1002
+ *   dft_ot_bulk_*         src/helium-ot-bulk.c:21-463
+ * and the libgrid subset they and their callers use (libgrid is not part of
+ * the reference tree; call sites are cited per function).
+ *
+ * Semantics (same as the reference): dft_ot_potential ADDS V_OT to
+ * `potential` and sets otf->density; dft_ot_energy_density overwrites the
+ * output and sets otf->density.  Calls on one functional are ordered on the
+ * library's HIP stream; functions returning host scalars synchronise.
+ * Functions are not reentrant for one otf (shared workspaces).
+ */
+#ifndef LIBDFT_MI355X_H
+#define LIBDFT_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double REAL;
+typedef int64_t INT;
+typedef double _Complex REAL_COMPLEX;
+#define FMT_R "%le"
+#define FMT_I "%ld"
+
+/* Device storage precision of a grid. */
+#define GRID_FP32 1
+#define GRID_FP64 2
+
+typedef struct rgrid_struct rgrid;
+typedef struct cgrid_struct cgrid;
+
+/* Real grid handle (libgrid rgrid; fields callers read are kept). */
+struct rgrid_struct {
+  INT nx, ny, nz;      /* row-major [i][j][k], k fastest */
+  REAL step;           /* bohr */
+  REAL x0, y0, z0;     /* origin; x_i = (i - nx/2) step - x0 */
+  void *dev;           /* device storage (float or double), owned */
+  INT dev_elems;       /* elements allocated in dev */
+  int prec;            /* GRID_FP32 or GRID_FP64 */
+  int flag;            /* claim flag (rgrid_claim / rgrid_release) */
+  int spectral;        /* 1: holds a real half spectrum [nx][ny][nz/2+1] */
+  char id[32];
+};
+
+/* Complex grid handle (libgrid cgrid). */
+struct cgrid_struct {
+  INT nx, ny, nz;
+  REAL step;
+  REAL x0, y0, z0;
+  void *dev;           /* device storage (float2 or double2), owned */
+  INT dev_elems;
+  int prec;
+  int flag;
+  REAL kx0, ky0, kz0;  /* momentum frame (cgrid_set_momentum); only 0 is supported */
+  char id[32];
+};
+
+#define WF_PERIODIC_BOUNDARY 1
+#define WF_2ND_ORDER_FFT 2
+
+/* Wave function (libgrid wf). */
+typedef struct wf_struct {
+  cgrid *grid;
+  REAL mass;
+  REAL norm;           /* target norm for grid_wf_normalize */
+  char boundary;
+  char propagator;
+  cgrid *cworkspace;   /* propagator scratch (unused by the FFT propagator) */
+} wf;
+
+/* Boundary "functions" accepted by rgrid_alloc / cgrid_alloc (only periodic). */
+REAL rgrid_value_outside_periodic(rgrid *grid, INT i, INT j, INT k);
+REAL_COMPLEX cgrid_value_outside_periodic(cgrid *grid, INT i, INT j, INT k);
+#define RGRID_PERIODIC_BOUNDARY rgrid_value_outside_periodic
+#define CGRID_PERIODIC_BOUNDARY cgrid_value_outside_periodic
+
+/* ---------------------------------------------------------------- setup -- */
+/* Precision used by the allocations that follow (default GRID_FP64, the
+   reference's REAL = double build). */
+void grid_set_precision(int prec);
+int grid_get_precision(void);
+/* No-ops kept for drop-in source compatibility with reference drivers. */
+void grid_threads_init(INT threads);
+void grid_set_fftw_flags(char flags);
+void grid_fft_read_wisdom(char *file);
+void grid_fft_write_wisdom(char *file);
+void cuda_enable(char flag, int ngpus, int *gpus);
+/* Synchronise the library stream. */
+void grid_synchronize(void);
+/* The HIP stream (hipStream_t) every kernel of the library is launched on. */
+void *grid_get_stream(void);
+
+/* ------------------------------------------------------------ allocation -- */
+rgrid *rgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL (*value_outside)(rgrid *, INT, INT, INT),
+                   void *outside_params_ptr, char *id);
+cgrid *cgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL_COMPLEX (*value_outside)(cgrid *, INT, INT, INT),
+                   void *outside_params_ptr, char *id);
+rgrid *rgrid_clone(rgrid *grid, char *id);
+cgrid *cgrid_clone(cgrid *grid, char *id);
+void rgrid_free(rgrid *grid);
+void cgrid_free(cgrid *grid);
+void rgrid_set_origin(rgrid *grid, REAL x0, REAL y0, REAL z0);
+void cgrid_set_origin(cgrid *grid, REAL x0, REAL y0, REAL z0);
+void rgrid_claim(rgrid *grid);
+void rgrid_release(rgrid *grid);
+wf *grid_wf_alloc(INT nx, INT ny, INT nz, REAL step, REAL mass, char boundary, char propagator, char *id);
+wf *grid_wf_clone(wf *gwf, char *id);
+void grid_wf_free(wf *gwf);
+
+/* ------------------------------------------- host <-> device data transfer -- */
+/* Host buffers are FP64 (REAL / REAL_COMPLEX), nx*ny*nz entries, row-major. */
+void rgrid_copy_from_host(rgrid *grid, const REAL *src);
+void rgrid_copy_to_host(const rgrid *grid, REAL *dst);
+void cgrid_copy_from_host(cgrid *grid, const REAL_COMPLEX *src);
+void cgrid_copy_to_host(const cgrid *grid, REAL_COMPLEX *dst);
+REAL rgrid_value_at_index(rgrid *grid, INT i, INT j, INT k);
+REAL_COMPLEX cgrid_value_at_index(cgrid *grid, INT i, INT j, INT k);
+/* Host callback maps (evaluated on the host, uploaded once). */
+void rgrid_map(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg);
+void cgrid_map(cgrid *grid, REAL_COMPLEX (*func)(void *arg, REAL x, REAL y, REAL z), void *farg);
+void grid_wf_map(wf *gwf, REAL_COMPLEX (*func)(void *arg, REAL x, REAL y, REAL z), void *farg);
+
+/* -------------------------------------------------- grid operations (GPU) -- */
+void rgrid_zero(rgrid *grid);
+void cgrid_zero(cgrid *grid);
+void rgrid_constant(rgrid *grid, REAL c);
+void cgrid_constant(cgrid *grid, REAL_COMPLEX c);
+void grid_wf_constant(wf *gwf, REAL_COMPLEX c);
+void rgrid_copy(rgrid *dst, rgrid *src);
+void cgrid_copy(cgrid *dst, cgrid *src);
+void cgrid_add(cgrid *grid, REAL_COMPLEX c);
+void cgrid_multiply(cgrid *grid, REAL_COMPLEX c);
+void rgrid_add(rgrid *grid, REAL c);
+void rgrid_multiply(rgrid *grid, REAL c);
+/* dst = dst + s * a * b */
+void rgrid_add_scaled_product(rgrid *dst, REAL s, rgrid *a, rgrid *b);
+void grid_real_to_complex_re(cgrid *dst, rgrid *src);
+void grid_add_real_to_complex_re(cgrid *dst, rgrid *src);
+void grid_add_real_to_complex_im(cgrid *dst, rgrid *src);
+void grid_wf_density(wf *gwf, rgrid *density);
+/* v_d = Im(psi* d_d psi) / (M (|psi|^2 + eps)), periodic central differences. */
+void grid_wf_velocity(wf *gwf, rgrid *vx, rgrid *vy, rgrid *vz, REAL eps);
+void rgrid_gradient_x(rgrid *src, rgrid *dst);
+void rgrid_gradient_y(rgrid *src, rgrid *dst);
+void rgrid_gradient_z(rgrid *src, rgrid *dst);
+
+/* Reductions: FP64 accumulation with a compensated final sum. */
+REAL rgrid_integral(rgrid *grid);
+REAL grid_wf_norm(wf *gwf);
+void grid_wf_normalize(wf *gwf);
+/* Kinetic energy h^3/N sum_k (k^2/2M)|psi_k|^2 (the potential argument must be NULL). */
+REAL grid_wf_energy(wf *gwf, cgrid *potential);
+
+/* ------------------------------------------------ split-operator stepping -- */
+/* psi <- e^{-iV dt/2} F^-1 e^{-i k^2 dt/2M} F e^{-iV dt/2} psi, complex V and dt. */
+void grid_wf_propagate(wf *gwf, cgrid *potential, REAL_COMPLEX time);
+/* gwfp = Step(gwf; potential) */
+void grid_wf_propagate_predict(wf *gwf, wf *gwfp, cgrid *potential, REAL_COMPLEX time);
+/* gwf = Step(gwf; potential), potential = (V_n + V_p)/2 formed by the caller */
+void grid_wf_propagate_correct(wf *gwf, cgrid *potential, REAL_COMPLEX time);
+
+/* ------------------------------------------------------------ timers -- */
+typedef struct { double t0; } grid_timer;
+void grid_timer_start(grid_timer *timer);
+REAL grid_timer_wall_clock_time(grid_timer *timer);
+
+/* ============================================================ OT functional */
+
+/* Model flags, src/ot.h:37-60 (PLAIN, KC and BACKFLOW are implemented). */
+#define DFT_OT_PLAIN    0
+#define DFT_OT_KC       1
+#define DFT_OT_HD       2
+#define DFT_OT_HD2      4
+#define DFT_OT_BACKFLOW 8
+#define DFT_OT_T0MK    16
+#define DFT_GP         524288
+#define DFT_DR         1048576
+#define DFT_ZERO       2097152
+#define DFT_GP2        4194304
+
+#define DFT_MIN_SUBSTEPS 4   /* src/ot.h:142 */
+#define DFT_MAX_SUBSTEPS 32  /* src/ot.h:143 */
+#define DFT_EPS 1E-4         /* src/ot.h:149 */
+
+typedef struct {
+  REAL h, sigma, epsilon, cval;   /* src/dft.h:24-29 */
+} dft_common_lj;
+
+typedef struct {
+  REAL g11, g12, g21, g22, a1, a2; /* src/ot.h:67-69 */
+} dft_ot_bf;
+
+struct dft_ot_device;
+
+/* src/ot.h:90-133.  Kernel fields hold real half spectra on the device
+   (the kernels are even, so their spectra are real). */
+typedef struct dft_ot_functional_struct {
+  INT model;
+  REAL b, c2, c2_exp, c3, c3_exp, c4, rho_0s, alpha_s, l_g, mass, rho0, temp;
+  dft_common_lj lj_params;
+  dft_ot_bf bf_params;
+  rgrid *lennard_jones, *spherical_avg, *gaussian_tf, *gaussian_x_tf, *gaussian_y_tf, *gaussian_z_tf, *backflow_pot;
+  REAL beta, rhom, C, mu0, xi, rhobf;
+  REAL div_epsilon, max_veloc, max_bfpot, c_bfpot;
+  rgrid *workspace1, *workspace2, *workspace3, *workspace4, *workspace5, *workspace6, *workspace7, *workspace8,
+      *workspace9;
+  rgrid *density;
+  /* ---- extensions (not in the reference struct) ---- */
+  REAL kc_coef23;                 /* KC terms 2,3 coefficient: 1 (reference) or 1/rho_0s (exact) */
+  struct dft_ot_device *device;   /* device state: plans, workspaces, scratch */
+} dft_ot_functional;
+
+#define DFT_HELIUM_MASS (4.002602 / GRID_AUTOAMU) /* src/ot.h:139 */
+
+dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_substeps);
+void dft_ot_free(dft_ot_functional *otf);
+void dft_ot_init_params(dft_ot_functional *otf, INT model);
+void dft_ot_potential(dft_ot_functional *otf, cgrid *potential, wf *gwf);
+void dft_ot_potential2(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *veloc_x, rgrid *veloc_y,
+                       rgrid *veloc_z);
+void dft_ot_backflow_potential(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *veloc_x,
+                               rgrid *veloc_y, rgrid *veloc_z, rgrid *workspace1, rgrid *workspace2,
+                               rgrid *workspace3, rgrid *workspace4, rgrid *workspace5, rgrid *workspace6);
+void dft_ot_energy_density(dft_ot_functional *otf, rgrid *energy_density, wf *gwf);
+void dft_ot_energy_density_kc(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density);
+void dft_ot_energy_density_bf(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density);
+
+/* Bulk closed forms, src/helium-ot-bulk.c (host FP64). */
+REAL dft_ot_bulk_energy(dft_ot_functional *otf, REAL rho);
+REAL dft_ot_bulk_dEdRho(dft_ot_functional *otf, REAL rho);
+REAL dft_ot_bulk_density(dft_ot_functional *otf);
+REAL dft_ot_bulk_chempot(dft_ot_functional *otf);
+REAL dft_ot_bulk_chempot2(dft_ot_functional *otf);
+REAL dft_ot_bulk_chempot3(dft_ot_functional *otf, REAL rho0);
+REAL dft_ot_bulk_pressure(dft_ot_functional *otf, REAL rho);
+REAL dft_ot_bulk_dPdRho(dft_ot_functional *otf, REAL rho);
+REAL dft_ot_bulk_density_pressurized(dft_ot_functional *otf, REAL pressure);
+REAL dft_ot_bulk_chempot_pressurized(dft_ot_functional *otf, REAL pressure);
+REAL dft_ot_bulk_compressibility(dft_ot_functional *otf, REAL rho);
+REAL dft_ot_bulk_sound_speed(dft_ot_functional *otf, REAL rho);
+REAL dft_ot_bulk_dispersion(dft_ot_functional *otf, REAL *k, REAL rho0);
+REAL dft_ot_bulk_istatic(dft_ot_functional *otf, REAL *k, REAL rho0);
+
+/* Kernel functions, src/common.c:31-66, 131-138, 173-179, 193-200; src/ot-private.h:11-22. */
+REAL dft_common_lj_func(REAL r2, REAL sig, REAL eps);
+REAL dft_common_lennard_jones(void *arg, REAL x, REAL y, REAL z);
+REAL dft_common_spherical_avg(void *arg, REAL x, REAL y, REAL z);
+REAL dft_common_spherical_avg_k(void *arg, REAL kx, REAL ky, REAL kz);
+REAL dft_common_gaussian(void *arg, REAL x, REAL y, REAL z);
+REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z);
+
+/* ---- extensions ---- */
+/* KC potential mode: 0 = reference formula (default), 1 = exact dE/drho. */
+void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact);
+/* Energy of the functional, h^3 sum eps_OT, in FP64 (kinetic part excluded). */
+REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf);
+
+/* Atomic-unit conversions (CODATA 2018; libgrid's au.h). */
+#define GRID_AUTOANG 0.529177210903
+#define GRID_AUTOK 315775.02480407
+#define GRID_AUTOFS 0.024188843265857
+#define GRID_AUTOAMU (1.0 / 1822.888486209)
+#define GRID_AUTOMPS 2187691.26364
+#define GRID_AUTOBAR 2.9421015697E8
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBDFT_MI355X_H */

@@ -1,0 +1,151 @@
+// Host FP64 parts of the API: bulk closed forms (src/helium-ot-bulk.c) and the
+// kernel functions (src/common.c, src/ot-private.h).  T = 0 OT models only
+// (c4 = 0), so the ideal-gas terms of the thermal models do not appear.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "../../include/libdft_mi355x.h"
+
+extern "C" {
+
+// src/helium-ot-bulk.c:21-33
+REAL dft_ot_bulk_energy(dft_ot_functional *otf, REAL rho) {
+  return 0.5 * otf->b * rho * rho + 0.5 * otf->c2 * rho * rho * rho + (1.0 / 3.0) * otf->c3 * rho * rho * rho * rho;
+}
+
+// src/helium-ot-bulk.c:45-57
+REAL dft_ot_bulk_dEdRho(dft_ot_functional *otf, REAL rho) {
+  return otf->b * rho + 1.5 * otf->c2 * rho * rho + (4.0 / 3.0) * otf->c3 * rho * rho * rho;
+}
+
+// src/helium-ot-bulk.c:143-146
+REAL dft_ot_bulk_pressure(dft_ot_functional *otf, REAL rho) {
+  return rho * dft_ot_bulk_dEdRho(otf, rho) - dft_ot_bulk_energy(otf, rho);
+}
+
+// src/helium-ot-bulk.c:157-172 (central difference, step 1e-7)
+REAL dft_ot_bulk_dPdRho(dft_ot_functional *otf, REAL rho) {
+  const double e = 1e-7;
+  return (dft_ot_bulk_pressure(otf, rho + e) - dft_ot_bulk_pressure(otf, rho - e)) / (2.0 * e);
+}
+
+// src/helium-ot-bulk.c:185-211: Newton from rho = 1 until |P(rho) - P| < 1e-12.
+REAL dft_ot_bulk_density_pressurized(dft_ot_functional *otf, REAL pressure) {
+  double rho0 = 1.0, misP = dft_ot_bulk_pressure(otf, rho0) - pressure;
+  for (int i = 0; i < 1000; ++i) {
+    if (std::fabs(misP) < 1e-12) return rho0;
+    rho0 -= misP / dft_ot_bulk_dPdRho(otf, rho0);
+    misP = dft_ot_bulk_pressure(otf, rho0) - pressure;
+  }
+  fprintf(stderr, "libdft: Error in dft_ot_bulk_density_pressurized - Newton-Raphson did not converge.\n");
+  abort();
+}
+
+REAL dft_ot_bulk_density(dft_ot_functional *otf) { return dft_ot_bulk_density_pressurized(otf, 0.0); }
+REAL dft_ot_bulk_chempot(dft_ot_functional *otf) { return dft_ot_bulk_dEdRho(otf, dft_ot_bulk_density(otf)); }
+REAL dft_ot_bulk_chempot2(dft_ot_functional *otf) { return dft_ot_bulk_dEdRho(otf, otf->rho0); }
+REAL dft_ot_bulk_chempot3(dft_ot_functional *otf, REAL rho0) { return dft_ot_bulk_dEdRho(otf, rho0); }
+REAL dft_ot_bulk_chempot_pressurized(dft_ot_functional *otf, REAL pressure) {
+  return dft_ot_bulk_dEdRho(otf, dft_ot_bulk_density_pressurized(otf, pressure));
+}
+REAL dft_ot_bulk_compressibility(dft_ot_functional *otf, REAL rho) {
+  return 1.0 / (rho * dft_ot_bulk_dPdRho(otf, rho));
+}
+REAL dft_ot_bulk_sound_speed(dft_ot_functional *otf, REAL rho) {
+  return 1.0 / std::sqrt(otf->mass * rho * dft_ot_bulk_compressibility(otf, rho));
+}
+
+// Radial transforms used by the semi-analytic response, src/helium-ot-bulk.c:376-409.
+static double ft_lj(const dft_ot_functional *otf, double k) {
+  const double sigma = otf->lj_params.sigma, eps = otf->lj_params.epsilon, h = otf->lj_params.h, ks = k * sigma;
+  double val = 0.0;
+  for (double x = h / sigma; x < 1e3; x += 1e-2) val += std::sin(ks * x) * (std::pow(x, -11.0) - std::pow(x, -5.0));
+  return val * 1e-2 * 16.0 * M_PI * eps * sigma * sigma * sigma / ks;
+}
+static double ft_pi(const dft_ot_functional *otf, double k) {
+  const double kh = k * otf->lj_params.h;
+  return (3.0 / (kh * kh * kh)) * (std::sin(kh) - kh * std::cos(kh));
+}
+static double ft_vj(const dft_ot_functional *otf, double k) {
+  const dft_ot_bf &p = otf->bf_params;
+  const double k2 = k * k, m = std::pow(M_PI, 1.5);
+  const double e1 = std::exp(-k2 / (4.0 * p.a1)), e2 = std::exp(-k2 / (4.0 * p.a2));
+  return (p.g11 * std::pow(M_PI / p.a1, 1.5) + p.g12 * (6.0 * p.a1 - k2) * m / (4.0 * std::pow(p.a1, 3.5))) * e1 +
+         (p.g21 * std::pow(M_PI / p.a2, 1.5) + p.g22 * (6.0 * p.a2 - k2) * m / (4.0 * std::pow(p.a2, 3.5))) * e2;
+}
+
+// src/helium-ot-bulk.c:411-434
+REAL dft_ot_bulk_dispersion(dft_ot_functional *otf, REAL *k, REAL rho0) {
+  if (rho0 < 0.0) return 0.0;
+  const double tk = *k;
+  if (tk < 1e-3) return 0.0;
+  const double lj = ft_lj(otf, tk), pi = ft_pi(otf, tk), tmp = tk * tk / otf->mass;
+  double ikai = tmp / 4.0 + rho0 * lj + otf->c2 * (2.0 * pi + pi * pi) * rho0 * rho0 +
+                2.0 * otf->c3 * (pi + pi * pi) * rho0 * rho0 * rho0;
+  if (otf->model & DFT_OT_KC)
+    ikai += -(tmp / 2.0) * otf->alpha_s * rho0 * (1.0 - rho0 / otf->rho_0s) * (1.0 - rho0 / otf->rho_0s) *
+            std::exp(-tk * tk * otf->l_g * otf->l_g / 4.0);
+  if (otf->model & DFT_OT_BACKFLOW)
+    return std::sqrt(tmp * ikai * (1.0 - rho0 * (ft_vj(otf, 0.0) - ft_vj(otf, tk))));
+  return std::sqrt(tmp * ikai);
+}
+
+// src/helium-ot-bulk.c:445-463
+REAL dft_ot_bulk_istatic(dft_ot_functional *otf, REAL *k, REAL rho0) {
+  if (rho0 < 0.0) return 0.0;
+  const double tk = *k;
+  if (tk < 1e-2) return 1e-2;
+  const double lj = ft_lj(otf, tk), pi = ft_pi(otf, tk);
+  double ikai = tk * tk / (4.0 * otf->mass) + rho0 * lj + otf->c2 * (2.0 * pi + pi * pi) * rho0 * rho0 +
+                2.0 * otf->c3 * (pi + pi * pi) * rho0 * rho0 * rho0;
+  if (otf->model & DFT_OT_KC)
+    ikai += -(1.0 / (2.0 * otf->mass)) * otf->alpha_s * rho0 * (1.0 - rho0 / otf->rho_0s) *
+            (1.0 - rho0 / otf->rho_0s) * std::exp(-tk * tk * otf->l_g * otf->l_g / 4.0);
+  return ikai;
+}
+
+// src/common.c:31-39
+REAL dft_common_lj_func(REAL r2, REAL sig, REAL eps) {
+  r2 = sig * sig / r2;
+  r2 = r2 * r2 * r2;
+  return 4.0 * eps * r2 * (r2 - 1.0);
+}
+
+// src/common.c:53-66
+REAL dft_common_lennard_jones(void *arg, REAL x, REAL y, REAL z) {
+  const dft_common_lj *p = (const dft_common_lj *)arg;
+  const double r2 = x * x + y * y + z * z;
+  if (r2 <= p->h * p->h) return p->cval;
+  return dft_common_lj_func(r2, p->sigma, p->epsilon);
+}
+
+// src/common.c:131-138
+REAL dft_common_spherical_avg(void *arg, REAL x, REAL y, REAL z) {
+  const double h = *(const double *)arg;
+  return (x * x + y * y + z * z <= h * h) ? 3.0 / (4.0 * M_PI * h * h * h) : 0.0;
+}
+
+// src/common.c:173-179
+REAL dft_common_spherical_avg_k(void *arg, REAL kx, REAL ky, REAL kz) {
+  const double hk = *(const double *)arg * std::sqrt(kx * kx + ky * ky + kz * kz);
+  if (hk < 1e-5) return 1.0 - 0.1 * hk * hk;
+  return 3.0 * (std::sin(hk) - hk * std::cos(hk)) / (hk * hk * hk);
+}
+
+// src/common.c:193-200
+REAL dft_common_gaussian(void *arg, REAL x, REAL y, REAL z) {
+  const double iw = *(const double *)arg;
+  double norm = 0.5 * (2.0 / std::sqrt(M_PI)) * iw;
+  norm = norm * norm * norm;
+  return norm * std::exp(-(x * x + y * y + z * z) * iw * iw);
+}
+
+// src/ot-private.h:11-22
+REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z) {
+  const dft_ot_bf *p = (const dft_ot_bf *)arg;
+  const double r2 = x * x + y * y + z * z;
+  return (p->g11 + p->g12 * r2) * std::exp(-p->a1 * r2) + (p->g21 + p->g22 * r2) * std::exp(-p->a2 * r2);
+}
+
+}  // extern "C"

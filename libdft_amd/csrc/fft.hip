@@ -1,0 +1,242 @@
+// Host side of the FFT: plans (FP64-computed twiddles), per-size dispatch and
+// the 3-D transform sequences built from the pass kernels.
+#include <cmath>
+#include <vector>
+
+#include "fft_kernels.hpp"
+#include "internal.hpp"
+
+namespace dft {
+
+bool fft_size_ok(int64_t n) {
+  return n == 4 || n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+}
+
+template <typename T> static cplx<T> *make_twiddles(int n) {
+  std::vector<cplx<T>> h(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    h[m] = cmk<cplx<T>>((T)std::cos(a), (T)std::sin(a));
+  }
+  auto *d = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n);
+  DFT_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(cplx<T>) * n, hipMemcpyHostToDevice));
+  return d;
+}
+
+// sin(2 pi m / n) / h for m in [0, cnt): the symbol of the periodic central difference.
+template <typename T> static T *make_sin(int n, int cnt, double step) {
+  std::vector<T> h(cnt);
+  for (int m = 0; m < cnt; ++m) h[m] = (T)(std::sin(2.0 * M_PI * (double)m / (double)n) / step);
+  auto *d = (T *)dmalloc(sizeof(T) * cnt);
+  DFT_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(T) * cnt, hipMemcpyHostToDevice));
+  return d;
+}
+
+template <typename T> void Plan3D<T>::init(int nx_, int ny_, int nz_, double step_) {
+  nx = nx_;
+  ny = ny_;
+  nz = nz_;
+  step = step_;
+  P = nz / 2 + 1;
+  twx = make_twiddles<T>(nx);
+  twy = make_twiddles<T>(ny);
+  twz = make_twiddles<T>(nz);
+  sinx = make_sin<T>(nx, nx, step);
+  siny = make_sin<T>(ny, ny, step);
+  sinz = make_sin<T>(nz, P, step);
+}
+
+template <typename T> void Plan3D<T>::destroy() {
+  dfree(twx);
+  dfree(twy);
+  dfree(twz);
+  dfree(sinx);
+  dfree(siny);
+  dfree(sinz);
+  twx = twy = twz = nullptr;
+  sinx = siny = sinz = nullptr;
+}
+
+#define DFT_DISPATCH_N(n, CALL)                                   \
+  switch (n) {                                                    \
+    case 4: { constexpr int N_ = 4; CALL; } break;                \
+    case 8: { constexpr int N_ = 8; CALL; } break;                \
+    case 16: { constexpr int N_ = 16; CALL; } break;              \
+    case 32: { constexpr int N_ = 32; CALL; } break;              \
+    case 64: { constexpr int N_ = 64; CALL; } break;              \
+    case 128: { constexpr int N_ = 128; CALL; } break;            \
+    case 256: { constexpr int N_ = 256; CALL; } break;            \
+    case 512: { constexpr int N_ = 512; CALL; } break;            \
+    default:                                                      \
+      fprintf(stderr, "libdft: unsupported FFT length %d\n", n);  \
+      abort();                                                    \
+  }
+
+namespace {
+
+template <typename T, int N>
+void launch_zr2c(const Plan3D<T> &p, const fft::Batch &b, int nb, hipStream_t s) {
+  using TL = fft::Tile<T, N>;
+  const int64_t npairs = (int64_t)p.nx * p.ny / 2;
+  dim3 grid((unsigned)((npairs + TL::TILE - 1) / TL::TILE), nb);
+  hipLaunchKernelGGL((fft::k_zr2c<T, N>), grid, dim3(TL::THREADS), 0, s, b, npairs, p.P, p.twz);
+  DFT_LAUNCH_CHECK();
+}
+
+template <typename T, int N>
+void launch_zc2r(const Plan3D<T> &p, const fft::Batch &b, int nb, T scale, hipStream_t s) {
+  using TL = fft::Tile<T, N, 2>;
+  const int64_t npairs = (int64_t)p.nx * p.ny / 2;
+  dim3 grid((unsigned)((npairs + TL::TILE - 1) / TL::TILE), nb);
+  hipLaunchKernelGGL((fft::k_zc2r<T, N>), grid, dim3(TL::THREADS), 0, s, b, npairs, p.P, p.twz, scale);
+  DFT_LAUNCH_CHECK();
+}
+
+// Strided pass along y (axis 1) or x (axis 0) over columns [0, cols).
+template <typename T, int N, int DIR>
+void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, int cols, T scale, hipStream_t s) {
+  using TL = fft::Tile<T, N>;
+  const int ntile = (cols + TL::TILE - 1) / TL::TILE;
+  int64_t es, os;
+  int outer;
+  if (axis == 1) {
+    es = cols;
+    os = (int64_t)p.ny * cols;
+    outer = p.nx;
+  } else {
+    es = (int64_t)p.ny * cols;
+    os = cols;
+    outer = p.ny;
+  }
+  dim3 grid((unsigned)(outer * ntile), nb);
+  const cplx<T> *tw = axis == 1 ? p.twy : p.twx;
+  hipLaunchKernelGGL((fft::k_strided<T, N, DIR>), grid, dim3(TL::THREADS), 0, s, b, cols, es, os, ntile, tw, scale);
+  DFT_LAUNCH_CHECK();
+}
+
+template <typename T, int N>
+void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::CombineSet &cs, const fft::KernTab &kt,
+                     int nb, hipStream_t s) {
+  using TL = fft::Tile<T, N>;
+  const int ntile = (p.P + TL::TILE - 1) / TL::TILE;
+  dim3 grid((unsigned)(p.ny * ntile), nb);
+  hipLaunchKernelGGL((fft::k_xcombine_inv<T, N>), grid, dim3(TL::THREADS), 0, s, b, cs, kt, p.ny, p.P, ntile,
+                     p.twx);
+  DFT_LAUNCH_CHECK();
+}
+
+template <typename T, int N, int DIR>
+void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cplx<T> *pot, T dtr, T dti, T scale,
+                 double *partial, hipStream_t s) {
+  using TL = fft::Tile<T, N>;
+  const int64_t nlines = (int64_t)p.nx * p.ny;
+  dim3 grid((unsigned)((nlines + TL::TILE - 1) / TL::TILE));
+  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, DIR>), grid, dim3(TL::THREADS), 0, s, src, dst, pot, nlines, p.twz, dtr,
+                     dti, scale, partial);
+  DFT_LAUNCH_CHECK();
+}
+
+template <typename T, int N>
+void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dti, hipStream_t s) {
+  using TL = fft::Tile<T, N>;
+  const int ntile = (p.nz + TL::TILE - 1) / TL::TILE;
+  dim3 grid((unsigned)(p.ny * ntile));
+  hipLaunchKernelGGL((fft::k_xkin<T, N>), grid, dim3(TL::THREADS), 0, s, psi, p.ny, p.nz, ntile, p.twx, cx, cy, cz,
+                     dtr, dti);
+  DFT_LAUNCH_CHECK();
+}
+
+template <typename T> int zc2c_tile(int nz) {
+  int t = 1;
+  DFT_DISPATCH_N(nz, (t = fft::Tile<T, N_>::TILE));
+  return t;
+}
+
+}  // namespace
+
+int64_t strang_partials(int nx, int ny, int nz, int prec) {
+  const int t = prec == GRID_FP32 ? zc2c_tile<float>(nz) : zc2c_tile<double>(nz);
+  return ((int64_t)nx * ny + t - 1) / t;
+}
+
+template <typename T>
+void r2c3d(const Plan3D<T> &p, int nb, const T *const *in, cplx<T> *const *out, hipStream_t s) {
+  for (int b0 = 0; b0 < nb; b0 += fft::kMaxBatch) {
+    const int cnt = nb - b0 < fft::kMaxBatch ? nb - b0 : fft::kMaxBatch;
+    fft::Batch bz{}, by{};
+    for (int i = 0; i < cnt; ++i) {
+      bz.in[i] = in[b0 + i];
+      bz.out[i] = out[b0 + i];
+      by.in[i] = out[b0 + i];
+      by.out[i] = out[b0 + i];
+    }
+    DFT_DISPATCH_N(p.nz, (launch_zr2c<T, N_>(p, bz, cnt, s)));
+    DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, cnt, 1, p.P, (T)1, s)));
+    DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, by, cnt, 0, p.P, (T)1, s)));
+  }
+}
+
+template <typename T>
+void c2r3d(const Plan3D<T> &p, int nin, const cplx<T> *const *in, int nout, const fft::Combine *cmb,
+           const fft::KernTab &kt, cplx<T> *const *tmp, T *const *out, hipStream_t s) {
+  if (nin > fft::kMaxBatch) {
+    fprintf(stderr, "libdft: c2r3d: too many inputs\n");
+    abort();
+  }
+  for (int b0 = 0; b0 < nout; b0 += fft::kMaxBatch) {
+    const int cnt = nout - b0 < fft::kMaxBatch ? nout - b0 : fft::kMaxBatch;
+    fft::Batch bx{}, by{}, bz{};
+    fft::CombineSet cs{};
+    for (int i = 0; i < nin; ++i) bx.in[i] = in[i];
+    for (int i = 0; i < cnt; ++i) {
+      bx.out[i] = tmp[b0 + i];
+      by.in[i] = tmp[b0 + i];
+      by.out[i] = tmp[b0 + i];
+      bz.in[i] = tmp[b0 + i];
+      bz.out[i] = out[b0 + i];
+      cs.c[i] = cmb[b0 + i];
+    }
+    DFT_DISPATCH_N(p.nx, (launch_xcombine<T, N_>(p, bx, cs, kt, cnt, s)));
+    DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, cnt, 1, p.P, (T)1, s)));
+    DFT_DISPATCH_N(p.nz, (launch_zc2r<T, N_>(p, bz, cnt, (T)1, s)));
+  }
+}
+
+template <typename T>
+void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, double dti, double mass,
+            double *partial, hipStream_t s) {
+  fft::Batch by{};
+  by.in[0] = psi;
+  by.out[0] = psi;
+  const double L2x = std::pow(2.0 * M_PI / (p.nx * p.step), 2), L2y = std::pow(2.0 * M_PI / (p.ny * p.step), 2),
+               L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
+  const T cx = (T)(L2x / (2.0 * mass)), cy = (T)(L2y / (2.0 * mass)), cz = (T)(L2z / (2.0 * mass));
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, psi, psi, pot, (T)dtr, (T)dti, (T)1, nullptr, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, (T)dtr, (T)dti, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, (T)1, s)));
+  const T inv = (T)(1.0 / (double)p.n());
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, psi, psi, pot, (T)dtr, (T)dti, inv, partial, s)));
+}
+
+template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s) {
+  fft::Batch b{};
+  b.in[0] = out;
+  b.out[0] = out;
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, in, out, nullptr, (T)0, (T)0, (T)1, nullptr, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, b, 1, 0, p.nz, (T)1, s)));
+}
+
+#define DFT_INST(T)                                                                                          \
+  template struct Plan3D<T>;                                                                                 \
+  template void r2c3d<T>(const Plan3D<T> &, int, const T *const *, cplx<T> *const *, hipStream_t);           \
+  template void c2r3d<T>(const Plan3D<T> &, int, const cplx<T> *const *, int, const fft::Combine *,          \
+                         const fft::KernTab &, cplx<T> *const *, T *const *, hipStream_t);                   \
+  template void strang<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double, double *,   \
+                          hipStream_t);                                                                      \
+  template void c2c_fwd<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);
+DFT_INST(float)
+DFT_INST(double)
+
+}  // namespace dft

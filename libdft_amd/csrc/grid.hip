@@ -1,0 +1,748 @@
+// Grid handles (libgrid-compatible subset), elementwise device kernels,
+// FP64 reductions and the split-operator propagator API.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace dft;
+
+namespace dft {
+
+static hipStream_t g_stream = nullptr;
+static int g_prec = GRID_FP64;
+static std::mutex g_mu;
+
+hipStream_t lib_stream() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_stream) DFT_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return g_stream;
+}
+
+void *dmalloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    fprintf(stderr, "libdft: hipMalloc(%zu) failed: %s\n", bytes, hipGetErrorString(e));
+    return nullptr;
+  }
+  return p;
+}
+
+void dfree(void *p) {
+  if (p) DFT_HIP_CHECK(hipFree(p));
+}
+
+// --------------------------------------------------------------- kernels --
+#define GRID_STRIDE(i, n) for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+template <typename T> __global__ void k_fill(T *a, T v, int64_t n) { GRID_STRIDE(i, n) a[i] = v; }
+template <typename T> __global__ void k_fillc(cplx<T> *a, T re, T im, int64_t n) {
+  GRID_STRIDE(i, n) a[i] = cmk<cplx<T>>(re, im);
+}
+template <typename T> __global__ void k_addc(cplx<T> *a, T re, T im, int64_t n) {
+  GRID_STRIDE(i, n) a[i] = cmk<cplx<T>>(a[i].x + re, a[i].y + im);
+}
+template <typename T> __global__ void k_mulc(cplx<T> *a, T re, T im, int64_t n) {
+  GRID_STRIDE(i, n) a[i] = cmul(a[i], cmk<cplx<T>>(re, im));
+}
+template <typename T> __global__ void k_add(T *a, T c, int64_t n) { GRID_STRIDE(i, n) a[i] += c; }
+template <typename T> __global__ void k_mul(T *a, T c, int64_t n) { GRID_STRIDE(i, n) a[i] *= c; }
+template <typename T> __global__ void k_add_scaled_product(T *d, T s, const T *a, const T *b, int64_t n) {
+  GRID_STRIDE(i, n) d[i] += s * a[i] * b[i];
+}
+template <typename T> __global__ void k_r2c(cplx<T> *d, const T *r, int add, int imag, int64_t n) {
+  GRID_STRIDE(i, n) {
+    cplx<T> v = add ? d[i] : cmk<cplx<T>>(0, 0);
+    if (imag) v.y += r[i];
+    else v.x += r[i];
+    d[i] = v;
+  }
+}
+template <typename T> __global__ void k_density(const cplx<T> *psi, T *rho, int64_t n) {
+  GRID_STRIDE(i, n) rho[i] = psi[i].x * psi[i].x + psi[i].y * psi[i].y;
+}
+template <typename Td, typename Ts> __global__ void k_convert(Td *d, const Ts *s, int64_t n) {
+  GRID_STRIDE(i, n) d[i] = (Td)s[i];
+}
+
+// Periodic central difference along `dir` of a real field.
+template <typename T>
+__global__ void k_gradient(const T *f, T *g, int nx, int ny, int nz, int dir, T inv2h) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
+    int64_t p, m;
+    if (dir == 0) {
+      p = c + (int64_t)(i == nx - 1 ? -(nx - 1) : 1) * ny * nz;
+      m = c + (int64_t)(i == 0 ? (nx - 1) : -1) * ny * nz;
+    } else if (dir == 1) {
+      p = c + (int64_t)(j == ny - 1 ? -(ny - 1) : 1) * nz;
+      m = c + (int64_t)(j == 0 ? (ny - 1) : -1) * nz;
+    } else {
+      p = c + (k == nz - 1 ? -(nz - 1) : 1);
+      m = c + (k == 0 ? (nz - 1) : -1);
+    }
+    g[c] = inv2h * (f[p] - f[m]);
+  }
+}
+
+// v_d = Im(psi* (psi_{+d} - psi_{-d})/2h) / (M (|psi|^2 + eps))
+template <typename T>
+__global__ void k_velocity(const cplx<T> *psi, T *vx, T *vy, T *vz, int nx, int ny, int nz, T inv2h, T invM, T eps) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
+    const cplx<T> s = psi[c];
+    const T den = invM / (s.x * s.x + s.y * s.y + eps);
+    T *out[3] = {vx, vy, vz};
+    for (int d = 0; d < 3; ++d) {
+      if (!out[d]) continue;
+      int64_t p, m;
+      if (d == 0) {
+        p = c + (int64_t)(i == nx - 1 ? -(nx - 1) : 1) * ny * nz;
+        m = c + (int64_t)(i == 0 ? (nx - 1) : -1) * ny * nz;
+      } else if (d == 1) {
+        p = c + (int64_t)(j == ny - 1 ? -(ny - 1) : 1) * nz;
+        m = c + (int64_t)(j == 0 ? (ny - 1) : -1) * nz;
+      } else {
+        p = c + (k == nz - 1 ? -(nz - 1) : 1);
+        m = c + (k == 0 ? (nz - 1) : -1);
+      }
+      const T dx = (psi[p].x - psi[m].x) * inv2h, dy = (psi[p].y - psi[m].y) * inv2h;
+      out[d][c] = (s.x * dy - s.y * dx) * den;
+    }
+  }
+}
+
+// ----------------------------------------------------------- reductions --
+constexpr int kRedBlocks = 1024;
+
+int64_t red_blocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  return b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks;
+}
+
+__device__ inline void block_sum_store(double acc, double *partial) {
+  __shared__ double s[kBlock];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// sum a (b == null) or sum a*b
+template <typename T> __global__ void k_red_sum(const T *a, const T *b, int64_t n, double *partial) {
+  double acc = 0.0;
+  GRID_STRIDE(i, n) acc += b ? (double)a[i] * (double)b[i] : (double)a[i];
+  block_sum_store(acc, partial);
+}
+template <typename T> __global__ void k_red_abs2(const cplx<T> *a, int64_t n, double *partial) {
+  double acc = 0.0;
+  GRID_STRIDE(i, n) acc += (double)a[i].x * (double)a[i].x + (double)a[i].y * (double)a[i].y;
+  block_sum_store(acc, partial);
+}
+// sum_k K(k) |spec(k)|^2 over a full complex spectrum, K = cx mx^2 + cy my^2 + cz mz^2.
+template <typename T>
+__global__ void k_red_kinetic(const cplx<T> *a, int nx, int ny, int nz, double cx, double cy, double cz,
+                              double *partial) {
+  double acc = 0.0;
+  const int64_t n = (int64_t)nx * ny * nz;
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
+    const int mx = i < nx / 2 ? i : i - nx, my = j < ny / 2 ? j : j - ny, mz = k < nz / 2 ? k : k - nz;
+    const double K = cx * (double)mx * mx + cy * (double)my * my + cz * (double)mz * mz;
+    acc += K * ((double)a[c].x * (double)a[c].x + (double)a[c].y * (double)a[c].y);
+  }
+  block_sum_store(acc, partial);
+}
+
+double sum_partials(const double *dpartials, int64_t n, hipStream_t s) {
+  std::vector<double> h(n);
+  DFT_HIP_CHECK(hipMemcpyAsync(h.data(), dpartials, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+  // Neumaier compensated summation
+  double sum = 0.0, c = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double t = sum + h[i];
+    if (std::fabs(sum) >= std::fabs(h[i])) c += (sum - t) + h[i];
+    else c += (h[i] - t) + sum;
+    sum = t;
+  }
+  return sum + c;
+}
+
+template <typename T> void reduce_sum(const T *a, const T *b, int64_t n, double *partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_red_sum<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, a, b, n, partial);
+  DFT_LAUNCH_CHECK();
+}
+template <typename T> void reduce_abs2(const cplx<T> *a, int64_t n, double *partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_red_abs2<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, a, n, partial);
+  DFT_LAUNCH_CHECK();
+}
+template <typename T>
+void reduce_kinetic(const cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz, double *partial,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_red_kinetic<T>, dim3((unsigned)red_blocks((int64_t)nx * ny * nz)), dim3(kBlock), 0, s, spec,
+                     nx, ny, nz, cx, cy, cz, partial);
+  DFT_LAUNCH_CHECK();
+}
+template <typename T> void ew_density(const cplx<T> *psi, T *rho, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_density<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, psi, rho, n);
+  DFT_LAUNCH_CHECK();
+}
+
+template void reduce_sum<float>(const float *, const float *, int64_t, double *, hipStream_t);
+template void reduce_sum<double>(const double *, const double *, int64_t, double *, hipStream_t);
+template void reduce_abs2<float>(const float2 *, int64_t, double *, hipStream_t);
+template void reduce_abs2<double>(const double2 *, int64_t, double *, hipStream_t);
+template void reduce_kinetic<float>(const float2 *, int, int, int, double, double, double, double *, hipStream_t);
+template void reduce_kinetic<double>(const double2 *, int, int, int, double, double, double, double *, hipStream_t);
+template void ew_density<float>(const float2 *, float *, int64_t, hipStream_t);
+template void ew_density<double>(const double2 *, double *, int64_t, hipStream_t);
+
+}  // namespace dft
+
+// Scratch for FP64 partial sums shared by the library (stream-ordered use only).
+static double *g_partials = nullptr;
+static int64_t g_npartials = 0;
+double *dft_scratch_partials(int64_t n) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (n > g_npartials) {
+    if (g_partials) {
+      DFT_HIP_CHECK(hipDeviceSynchronize());
+      dfree(g_partials);
+    }
+    g_npartials = n < 4096 ? 4096 : n;
+    g_partials = (double *)dmalloc(sizeof(double) * g_npartials);
+  }
+  return g_partials;
+}
+
+// ================================================================== C ABI ==
+
+namespace {
+
+int64_t npts(INT nx, INT ny, INT nz) { return (int64_t)nx * ny * nz; }
+int64_t half_elems(INT nx, INT ny, INT nz) { return (int64_t)nx * ny * (nz / 2 + 1); }
+
+bool dims_ok(INT nx, INT ny, INT nz) {
+  return fft_size_ok(nx) && fft_size_ok(ny) && fft_size_ok(nz);
+}
+
+size_t esize(int prec) { return prec == GRID_FP32 ? sizeof(float) : sizeof(double); }
+
+void need_same(const char *fn, int64_t a, int64_t b, int pa, int pb) {
+  if (a != b || pa != pb) {
+    fprintf(stderr, "libdft: %s: grid shape or precision mismatch\n", fn);
+    abort();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+REAL rgrid_value_outside_periodic(rgrid *, INT, INT, INT) { return 0.0; }
+REAL_COMPLEX cgrid_value_outside_periodic(cgrid *, INT, INT, INT) { return 0.0; }
+
+void grid_set_precision(int prec) {
+  if (prec != GRID_FP32 && prec != GRID_FP64) {
+    fprintf(stderr, "libdft: grid_set_precision: unknown precision %d\n", prec);
+    return;
+  }
+  g_prec = prec;
+}
+int grid_get_precision(void) { return g_prec; }
+void grid_threads_init(INT) {}
+void grid_set_fftw_flags(char) {}
+void grid_fft_read_wisdom(char *) {}
+void grid_fft_write_wisdom(char *) {}
+void cuda_enable(char, int, int *) {}
+void grid_synchronize(void) { DFT_HIP_CHECK(hipStreamSynchronize(lib_stream())); }
+void *grid_get_stream(void) { return (void *)lib_stream(); }
+
+rgrid *rgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL (*)(rgrid *, INT, INT, INT), void *, char *id) {
+  if (!dims_ok(nx, ny, nz)) {
+    fprintf(stderr, "libdft: rgrid_alloc: dimensions must be powers of two in [4, 512] (got %ld %ld %ld)\n",
+            (long)nx, (long)ny, (long)nz);
+    return nullptr;
+  }
+  rgrid *g = (rgrid *)calloc(1, sizeof(rgrid));
+  if (!g) return nullptr;
+  g->nx = nx;
+  g->ny = ny;
+  g->nz = nz;
+  g->step = step;
+  g->prec = g_prec;
+  // room for a real field or a real half spectrum
+  g->dev_elems = npts(nx, ny, nz) > half_elems(nx, ny, nz) ? npts(nx, ny, nz) : half_elems(nx, ny, nz);
+  g->dev = dmalloc(esize(g->prec) * g->dev_elems);
+  if (!g->dev) {
+    free(g);
+    return nullptr;
+  }
+  DFT_HIP_CHECK(hipMemsetAsync(g->dev, 0, esize(g->prec) * g->dev_elems, lib_stream()));
+  if (id) strncpy(g->id, id, sizeof(g->id) - 1);
+  return g;
+}
+
+cgrid *cgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL_COMPLEX (*)(cgrid *, INT, INT, INT), void *, char *id) {
+  if (!dims_ok(nx, ny, nz)) {
+    fprintf(stderr, "libdft: cgrid_alloc: dimensions must be powers of two in [4, 512] (got %ld %ld %ld)\n",
+            (long)nx, (long)ny, (long)nz);
+    return nullptr;
+  }
+  cgrid *g = (cgrid *)calloc(1, sizeof(cgrid));
+  if (!g) return nullptr;
+  g->nx = nx;
+  g->ny = ny;
+  g->nz = nz;
+  g->step = step;
+  g->prec = g_prec;
+  g->dev_elems = npts(nx, ny, nz);
+  g->dev = dmalloc(2 * esize(g->prec) * g->dev_elems);
+  if (!g->dev) {
+    free(g);
+    return nullptr;
+  }
+  DFT_HIP_CHECK(hipMemsetAsync(g->dev, 0, 2 * esize(g->prec) * g->dev_elems, lib_stream()));
+  if (id) strncpy(g->id, id, sizeof(g->id) - 1);
+  return g;
+}
+
+rgrid *rgrid_clone(rgrid *src, char *id) {
+  int save = g_prec;
+  g_prec = src->prec;
+  rgrid *g = rgrid_alloc(src->nx, src->ny, src->nz, src->step, RGRID_PERIODIC_BOUNDARY, nullptr, id);
+  g_prec = save;
+  if (g) rgrid_set_origin(g, src->x0, src->y0, src->z0);
+  return g;
+}
+
+cgrid *cgrid_clone(cgrid *src, char *id) {
+  int save = g_prec;
+  g_prec = src->prec;
+  cgrid *g = cgrid_alloc(src->nx, src->ny, src->nz, src->step, CGRID_PERIODIC_BOUNDARY, nullptr, id);
+  g_prec = save;
+  if (g) cgrid_set_origin(g, src->x0, src->y0, src->z0);
+  return g;
+}
+
+void rgrid_free(rgrid *g) {
+  if (!g) return;
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  dfree(g->dev);
+  free(g);
+}
+void cgrid_free(cgrid *g) {
+  if (!g) return;
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  dfree(g->dev);
+  free(g);
+}
+void rgrid_set_origin(rgrid *g, REAL x0, REAL y0, REAL z0) {
+  g->x0 = x0;
+  g->y0 = y0;
+  g->z0 = z0;
+}
+void cgrid_set_origin(cgrid *g, REAL x0, REAL y0, REAL z0) {
+  g->x0 = x0;
+  g->y0 = y0;
+  g->z0 = z0;
+}
+void rgrid_claim(rgrid *g) {
+  if (g->flag) {
+    fprintf(stderr, "libdft: grid %s already claimed\n", g->id);
+    abort();
+  }
+  g->flag = 1;
+}
+void rgrid_release(rgrid *g) {
+  if (!g->flag) {
+    fprintf(stderr, "libdft: grid %s released without claim\n", g->id);
+    abort();
+  }
+  g->flag = 0;
+}
+
+wf *grid_wf_alloc(INT nx, INT ny, INT nz, REAL step, REAL mass, char boundary, char propagator, char *id) {
+  if (boundary != WF_PERIODIC_BOUNDARY || propagator != WF_2ND_ORDER_FFT) {
+    fprintf(stderr, "libdft: grid_wf_alloc: only periodic boundaries with the 2nd order FFT propagator\n");
+    return nullptr;
+  }
+  wf *w = (wf *)calloc(1, sizeof(wf));
+  if (!w) return nullptr;
+  w->grid = cgrid_alloc(nx, ny, nz, step, CGRID_PERIODIC_BOUNDARY, nullptr, id);
+  if (!w->grid) {
+    free(w);
+    return nullptr;
+  }
+  w->mass = mass;
+  w->norm = 1.0;
+  w->boundary = boundary;
+  w->propagator = propagator;
+  return w;
+}
+
+wf *grid_wf_clone(wf *src, char *id) {
+  wf *w = (wf *)calloc(1, sizeof(wf));
+  if (!w) return nullptr;
+  *w = *src;
+  w->cworkspace = nullptr;
+  w->grid = cgrid_clone(src->grid, id);
+  if (!w->grid) {
+    free(w);
+    return nullptr;
+  }
+  return w;
+}
+
+void grid_wf_free(wf *w) {
+  if (!w) return;
+  cgrid_free(w->grid);
+  if (w->cworkspace) cgrid_free(w->cworkspace);
+  free(w);
+}
+
+// ------------------------------------------------------------ transfers --
+void rgrid_copy_from_host(rgrid *g, const REAL *src) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  hipStream_t s = lib_stream();
+  if (g->prec == GRID_FP64) {
+    DFT_HIP_CHECK(hipMemcpyAsync(g->dev, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  } else {
+    std::vector<float> t(n);
+    for (int64_t i = 0; i < n; ++i) t[i] = (float)src[i];
+    DFT_HIP_CHECK(hipMemcpyAsync(g->dev, t.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+  }
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// For a spectral grid (a functional's kernel field) this copies the real half
+// spectrum, nx*ny*(nz/2+1) values.
+void rgrid_copy_to_host(const rgrid *g, REAL *dst) {
+  const int64_t n = g->spectral ? half_elems(g->nx, g->ny, g->nz) : npts(g->nx, g->ny, g->nz);
+  hipStream_t s = lib_stream();
+  if (g->prec == GRID_FP64) {
+    DFT_HIP_CHECK(hipMemcpyAsync(dst, g->dev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    std::vector<float> t(n);
+    DFT_HIP_CHECK(hipMemcpyAsync(t.data(), g->dev, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) dst[i] = t[i];
+  }
+}
+
+void cgrid_copy_from_host(cgrid *g, const REAL_COMPLEX *src) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  hipStream_t s = lib_stream();
+  if (g->prec == GRID_FP64) {
+    DFT_HIP_CHECK(hipMemcpyAsync(g->dev, src, 2 * sizeof(double) * n, hipMemcpyHostToDevice, s));
+  } else {
+    const double *d = (const double *)src;
+    std::vector<float> t(2 * n);
+    for (int64_t i = 0; i < 2 * n; ++i) t[i] = (float)d[i];
+    DFT_HIP_CHECK(hipMemcpyAsync(g->dev, t.data(), 2 * sizeof(float) * n, hipMemcpyHostToDevice, s));
+  }
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void cgrid_copy_to_host(const cgrid *g, REAL_COMPLEX *dst) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  hipStream_t s = lib_stream();
+  if (g->prec == GRID_FP64) {
+    DFT_HIP_CHECK(hipMemcpyAsync(dst, g->dev, 2 * sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    std::vector<float> t(2 * n);
+    DFT_HIP_CHECK(hipMemcpyAsync(t.data(), g->dev, 2 * sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    double *d = (double *)dst;
+    for (int64_t i = 0; i < 2 * n; ++i) d[i] = t[i];
+  }
+}
+
+REAL rgrid_value_at_index(rgrid *g, INT i, INT j, INT k) {
+  const int64_t c = (i * g->ny + j) * g->nz + k;
+  hipStream_t s = lib_stream();
+  double v = 0;
+  if (g->prec == GRID_FP64) {
+    DFT_HIP_CHECK(hipMemcpyAsync(&v, (double *)g->dev + c, sizeof(double), hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    float f = 0;
+    DFT_HIP_CHECK(hipMemcpyAsync(&f, (float *)g->dev + c, sizeof(float), hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    v = f;
+  }
+  return v;
+}
+
+REAL_COMPLEX cgrid_value_at_index(cgrid *g, INT i, INT j, INT k) {
+  const int64_t c = (i * g->ny + j) * g->nz + k;
+  hipStream_t s = lib_stream();
+  double v[2] = {0, 0};
+  if (g->prec == GRID_FP64) {
+    DFT_HIP_CHECK(hipMemcpyAsync(v, (double *)g->dev + 2 * c, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    float f[2] = {0, 0};
+    DFT_HIP_CHECK(hipMemcpyAsync(f, (float *)g->dev + 2 * c, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    v[0] = f[0];
+    v[1] = f[1];
+  }
+  REAL_COMPLEX r;
+  __real__ r = v[0];
+  __imag__ r = v[1];
+  return r;
+}
+
+void rgrid_map(rgrid *g, REAL (*func)(void *, REAL, REAL, REAL), void *farg) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  std::vector<double> h(n);
+  for (INT i = 0; i < g->nx; ++i)
+    for (INT j = 0; j < g->ny; ++j)
+      for (INT k = 0; k < g->nz; ++k)
+        h[(i * g->ny + j) * g->nz + k] = func(farg, (double)(i - g->nx / 2) * g->step - g->x0,
+                                              (double)(j - g->ny / 2) * g->step - g->y0,
+                                              (double)(k - g->nz / 2) * g->step - g->z0);
+  rgrid_copy_from_host(g, h.data());
+}
+
+void cgrid_map(cgrid *g, REAL_COMPLEX (*func)(void *, REAL, REAL, REAL), void *farg) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  std::vector<REAL_COMPLEX> h(n);
+  for (INT i = 0; i < g->nx; ++i)
+    for (INT j = 0; j < g->ny; ++j)
+      for (INT k = 0; k < g->nz; ++k)
+        h[(i * g->ny + j) * g->nz + k] = func(farg, (double)(i - g->nx / 2) * g->step - g->x0,
+                                              (double)(j - g->ny / 2) * g->step - g->y0,
+                                              (double)(k - g->nz / 2) * g->step - g->z0);
+  cgrid_copy_from_host(g, h.data());
+}
+
+void grid_wf_map(wf *w, REAL_COMPLEX (*func)(void *, REAL, REAL, REAL), void *farg) { cgrid_map(w->grid, func, farg); }
+
+// ------------------------------------------------------------ operations --
+void rgrid_zero(rgrid *g) { rgrid_constant(g, 0.0); }
+void cgrid_zero(cgrid *g) { cgrid_constant(g, 0.0); }
+
+#define LAUNCH(kern, n, ...) \
+  do {                                                                                    \
+    hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(kBlock), 0, lib_stream(), __VA_ARGS__); \
+    DFT_LAUNCH_CHECK();                                                                    \
+  } while (0)
+
+void rgrid_constant(rgrid *g, REAL c) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  DFT_BY_PREC(g->prec, LAUNCH(k_fill<T>, n, (T *)g->dev, (T)c, n));
+}
+
+void cgrid_constant(cgrid *g, REAL_COMPLEX c) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  DFT_BY_PREC(g->prec, LAUNCH(k_fillc<T>, n, (cplx<T> *)g->dev, (T)__real__ c, (T)__imag__ c, n));
+}
+
+void grid_wf_constant(wf *w, REAL_COMPLEX c) { cgrid_constant(w->grid, c); }
+
+void rgrid_copy(rgrid *dst, rgrid *src) {
+  need_same("rgrid_copy", npts(dst->nx, dst->ny, dst->nz), npts(src->nx, src->ny, src->nz), dst->prec, src->prec);
+  DFT_HIP_CHECK(hipMemcpyAsync(dst->dev, src->dev, esize(src->prec) * npts(src->nx, src->ny, src->nz),
+                               hipMemcpyDeviceToDevice, lib_stream()));
+}
+
+void cgrid_copy(cgrid *dst, cgrid *src) {
+  need_same("cgrid_copy", npts(dst->nx, dst->ny, dst->nz), npts(src->nx, src->ny, src->nz), dst->prec, src->prec);
+  DFT_HIP_CHECK(hipMemcpyAsync(dst->dev, src->dev, 2 * esize(src->prec) * npts(src->nx, src->ny, src->nz),
+                               hipMemcpyDeviceToDevice, lib_stream()));
+}
+
+void cgrid_add(cgrid *g, REAL_COMPLEX c) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  DFT_BY_PREC(g->prec, LAUNCH(k_addc<T>, n, (cplx<T> *)g->dev, (T)__real__ c, (T)__imag__ c, n));
+}
+
+void cgrid_multiply(cgrid *g, REAL_COMPLEX c) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  DFT_BY_PREC(g->prec, LAUNCH(k_mulc<T>, n, (cplx<T> *)g->dev, (T)__real__ c, (T)__imag__ c, n));
+}
+
+void rgrid_add(rgrid *g, REAL c) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  DFT_BY_PREC(g->prec, LAUNCH(k_add<T>, n, (T *)g->dev, (T)c, n));
+}
+
+void rgrid_multiply(rgrid *g, REAL c) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  DFT_BY_PREC(g->prec, LAUNCH(k_mul<T>, n, (T *)g->dev, (T)c, n));
+}
+
+void rgrid_add_scaled_product(rgrid *d, REAL s, rgrid *a, rgrid *b) {
+  const int64_t n = npts(d->nx, d->ny, d->nz);
+  need_same("rgrid_add_scaled_product", n, npts(a->nx, a->ny, a->nz), d->prec, a->prec);
+  need_same("rgrid_add_scaled_product", n, npts(b->nx, b->ny, b->nz), d->prec, b->prec);
+  DFT_BY_PREC(d->prec, LAUNCH(k_add_scaled_product<T>, n, (T *)d->dev, (T)s, (const T *)a->dev, (const T *)b->dev, n));
+}
+
+static void r2c_op(cgrid *dst, rgrid *src, int add, int imag, const char *fn) {
+  const int64_t n = npts(dst->nx, dst->ny, dst->nz);
+  need_same(fn, n, npts(src->nx, src->ny, src->nz), dst->prec, src->prec);
+  DFT_BY_PREC(dst->prec, LAUNCH(k_r2c<T>, n, (cplx<T> *)dst->dev, (const T *)src->dev, add, imag, n));
+}
+void grid_real_to_complex_re(cgrid *dst, rgrid *src) { r2c_op(dst, src, 0, 0, "grid_real_to_complex_re"); }
+void grid_add_real_to_complex_re(cgrid *dst, rgrid *src) { r2c_op(dst, src, 1, 0, "grid_add_real_to_complex_re"); }
+void grid_add_real_to_complex_im(cgrid *dst, rgrid *src) { r2c_op(dst, src, 1, 1, "grid_add_real_to_complex_im"); }
+
+void grid_wf_density(wf *w, rgrid *rho) {
+  cgrid *g = w->grid;
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  need_same("grid_wf_density", n, npts(rho->nx, rho->ny, rho->nz), g->prec, rho->prec);
+  DFT_BY_PREC(g->prec, ew_density<T>((const cplx<T> *)g->dev, (T *)rho->dev, n, lib_stream()));
+}
+
+void grid_wf_velocity(wf *w, rgrid *vx, rgrid *vy, rgrid *vz, REAL eps) {
+  cgrid *g = w->grid;
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  for (rgrid *v : {vx, vy, vz})
+    if (v) need_same("grid_wf_velocity", n, npts(v->nx, v->ny, v->nz), g->prec, v->prec);
+  DFT_BY_PREC(g->prec, LAUNCH(k_velocity<T>, n, (const cplx<T> *)g->dev, vx ? (T *)vx->dev : nullptr,
+                              vy ? (T *)vy->dev : nullptr, vz ? (T *)vz->dev : nullptr, (int)g->nx, (int)g->ny,
+                              (int)g->nz, (T)(0.5 / g->step), (T)(1.0 / w->mass), (T)eps));
+}
+
+static void gradient_op(rgrid *src, rgrid *dst, int dir) {
+  const int64_t n = npts(src->nx, src->ny, src->nz);
+  need_same("rgrid_gradient", n, npts(dst->nx, dst->ny, dst->nz), src->prec, dst->prec);
+  DFT_BY_PREC(src->prec, LAUNCH(k_gradient<T>, n, (const T *)src->dev, (T *)dst->dev, (int)src->nx, (int)src->ny,
+                                (int)src->nz, dir, (T)(0.5 / src->step)));
+}
+void rgrid_gradient_x(rgrid *src, rgrid *dst) { gradient_op(src, dst, 0); }
+void rgrid_gradient_y(rgrid *src, rgrid *dst) { gradient_op(src, dst, 1); }
+void rgrid_gradient_z(rgrid *src, rgrid *dst) { gradient_op(src, dst, 2); }
+
+// ----------------------------------------------------------- reductions --
+REAL rgrid_integral(rgrid *g) {
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  double *part = dft_scratch_partials(red_blocks(n));
+  DFT_BY_PREC(g->prec, reduce_sum<T>((const T *)g->dev, nullptr, n, part, lib_stream()));
+  return sum_partials(part, red_blocks(n), lib_stream()) * g->step * g->step * g->step;
+}
+
+REAL grid_wf_norm(wf *w) {
+  cgrid *g = w->grid;
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  double *part = dft_scratch_partials(red_blocks(n));
+  DFT_BY_PREC(g->prec, reduce_abs2<T>((const cplx<T> *)g->dev, n, part, lib_stream()));
+  return sum_partials(part, red_blocks(n), lib_stream()) * g->step * g->step * g->step;
+}
+
+void grid_wf_normalize(wf *w) {
+  const double nrm = grid_wf_norm(w);
+  if (!(nrm > 0)) {
+    fprintf(stderr, "libdft: grid_wf_normalize: zero norm\n");
+    return;
+  }
+  REAL_COMPLEX c = std::sqrt(w->norm / nrm);
+  cgrid_multiply(w->grid, c);
+}
+
+}  // extern "C"
+
+// Plans shared by the propagator entry points, one per (shape, step, precision).
+struct PlanCache {
+  int nx = 0, ny = 0, nz = 0, prec = 0;
+  double step = 0;
+  void *plan = nullptr;
+  cgrid *scratch = nullptr;  // spectrum scratch for grid_wf_energy
+};
+static PlanCache g_pc;
+
+template <typename T> static Plan3D<T> *plan_for(cgrid *g) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_pc.plan && (g_pc.nx != g->nx || g_pc.ny != g->ny || g_pc.nz != g->nz || g_pc.prec != g->prec ||
+                    g_pc.step != g->step)) {
+    DFT_HIP_CHECK(hipStreamSynchronize(g_stream));
+    if (g_pc.prec == GRID_FP32) {
+      ((Plan3D<float> *)g_pc.plan)->destroy();
+      delete (Plan3D<float> *)g_pc.plan;
+    } else {
+      ((Plan3D<double> *)g_pc.plan)->destroy();
+      delete (Plan3D<double> *)g_pc.plan;
+    }
+    g_pc.plan = nullptr;
+  }
+  if (!g_pc.plan) {
+    auto *p = new Plan3D<T>();
+    p->init((int)g->nx, (int)g->ny, (int)g->nz, g->step);
+    g_pc.plan = p;
+    g_pc.nx = (int)g->nx;
+    g_pc.ny = (int)g->ny;
+    g_pc.nz = (int)g->nz;
+    g_pc.prec = g->prec;
+    g_pc.step = g->step;
+  }
+  return (Plan3D<T> *)g_pc.plan;
+}
+
+extern "C" {
+
+REAL grid_wf_energy(wf *w, cgrid *potential) {
+  if (potential) {
+    fprintf(stderr, "libdft: grid_wf_energy: only the kinetic energy (potential = NULL) is provided\n");
+    abort();
+  }
+  cgrid *g = w->grid;
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  cgrid *tmp = cgrid_clone(g, (char *)"energy scratch");
+  double *part = dft_scratch_partials(red_blocks(n));
+  const double cx = std::pow(2.0 * M_PI / (g->nx * g->step), 2) / (2.0 * w->mass);
+  const double cy = std::pow(2.0 * M_PI / (g->ny * g->step), 2) / (2.0 * w->mass);
+  const double cz = std::pow(2.0 * M_PI / (g->nz * g->step), 2) / (2.0 * w->mass);
+  DFT_BY_PREC(g->prec, {
+    Plan3D<T> *p = plan_for<T>(g);
+    c2c_fwd<T>(*p, (const cplx<T> *)g->dev, (cplx<T> *)tmp->dev, lib_stream());
+    reduce_kinetic<T>((const cplx<T> *)tmp->dev, (int)g->nx, (int)g->ny, (int)g->nz, cx, cy, cz, part, lib_stream());
+  });
+  const double s = sum_partials(part, red_blocks(n), lib_stream());
+  cgrid_free(tmp);
+  return s * g->step * g->step * g->step / (double)n;
+}
+
+// ----------------------------------------------------------- propagation --
+void grid_wf_propagate(wf *w, cgrid *potential, REAL_COMPLEX time) {
+  cgrid *g = w->grid;
+  need_same("grid_wf_propagate", npts(g->nx, g->ny, g->nz), npts(potential->nx, potential->ny, potential->nz),
+            g->prec, potential->prec);
+  DFT_BY_PREC(g->prec, {
+    Plan3D<T> *p = plan_for<T>(g);
+    strang<T>(*p, (cplx<T> *)g->dev, (const cplx<T> *)potential->dev, __real__ time, __imag__ time, w->mass, nullptr,
+              lib_stream());
+  });
+}
+
+void grid_wf_propagate_predict(wf *w, wf *wp, cgrid *potential, REAL_COMPLEX time) {
+  cgrid_copy(wp->grid, w->grid);
+  grid_wf_propagate(wp, potential, time);
+}
+
+void grid_wf_propagate_correct(wf *w, cgrid *potential, REAL_COMPLEX time) { grid_wf_propagate(w, potential, time); }
+
+// ---------------------------------------------------------------- timers --
+void grid_timer_start(grid_timer *t) {
+  t->t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+REAL grid_timer_wall_clock_time(grid_timer *t) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t->t0;
+}
+
+}  // extern "C"

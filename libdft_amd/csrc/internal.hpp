@@ -1,0 +1,59 @@
+// Internal C++ interfaces shared by the library's translation units.
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+#include "../../include/libdft_mi355x.h"
+
+namespace dft {
+
+hipStream_t lib_stream();
+void *dmalloc(size_t bytes);
+void dfree(void *p);
+
+// ------------------------------------------------------------------ FFT ----
+template <typename T> struct Plan3D {
+  int nx = 0, ny = 0, nz = 0, P = 0;
+  double step = 0;
+  cplx<T> *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  T *sinx = nullptr, *siny = nullptr, *sinz = nullptr;
+  void init(int nx_, int ny_, int nz_, double step_);
+  void destroy();
+  int64_t n() const { return (int64_t)nx * ny * nz; }
+  int64_t nhalf() const { return (int64_t)nx * ny * P; }
+};
+
+bool fft_size_ok(int64_t n);
+
+// Real fields -> half spectra [nx][ny][nz/2+1] (unnormalised forward transform).
+template <typename T>
+void r2c3d(const Plan3D<T> &p, int nb, const T *const *in, cplx<T> *const *out, hipStream_t s);
+// Half spectra -> real fields: out_m = F^-1[ sum_t s_t K_t D_t in_t ], one combine per output.
+// tmp: nout half-spectrum scratch buffers.
+template <typename T>
+void c2r3d(const Plan3D<T> &p, int nin, const cplx<T> *const *in, int nout, const fft::Combine *cmb,
+           const fft::KernTab &kt, cplx<T> *const *tmp, T *const *out, hipStream_t s);
+// One Strang step on psi; partial (may be null) receives per-block sums of |psi|^2.
+template <typename T>
+void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, double dti, double mass,
+            double *partial, hipStream_t s);
+int64_t strang_partials(int nx, int ny, int nz, int prec);
+// Full complex forward transform (out of place).
+template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s);
+
+// ----------------------------------------------------------- reductions ----
+// Neumaier sum of n FP64 partials (device pointer), synchronising.
+double sum_partials(const double *dpartials, int64_t n, hipStream_t s);
+int64_t red_blocks(int64_t n);
+template <typename T> void reduce_sum(const T *a, const T *b, int64_t n, double *partial, hipStream_t s);
+template <typename T> void reduce_abs2(const cplx<T> *a, int64_t n, double *partial, hipStream_t s);
+template <typename T>
+void reduce_kinetic(const cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz, double *partial,
+                    hipStream_t s);
+
+// ------------------------------------------------------------- elementwise --
+template <typename T> void ew_density(const cplx<T> *psi, T *rho, int64_t n, hipStream_t s);
+
+}  // namespace dft
+
+// Library-wide scratch for FP64 reduction partials (grows on demand).
+double *dft_scratch_partials(int64_t n);

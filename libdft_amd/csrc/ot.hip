@@ -1,0 +1,1011 @@
+// Orsay-Trento functional on gfx950: kernel spectra, fused pointwise stages
+// and the FFT schedule of dft_ot_potential / dft_ot_energy_density.
+//
+// Schedule of one full-OT potential (PLAIN|KC|BACKFLOW), src/ot.c:314-991:
+//   P1   psi -> rho, v_d, q_d = rho v_d, q4 = rho v^2           (one stencil pass)
+//   F1   r2c {rho, q_x, q_y, q_z, q4}
+//   I1   c2r {Pi*rho, F*rho, dF_d*rho (3), U*rho, U*q_d (3), U*q4}
+//   P2   local terms, Q = (c2 rhobar + c3 rhobar^2) rho, G_d = d_d rho (1 - F*rho/rho_0s),
+//        BF real part, W_d = v_d A - B_d, (d_d rho/(rho+eps)) W_d
+//   F2   r2c {Q, G_x, G_y, G_z}
+//   I2   c2r {J_d = F*G_d (3), T1 = F*sum_d d_d G_d}
+//   P3   H = sum_d d_d rho J_d, KC terms 1 and 3, BF sum_d d_d W_d
+//   F3   r2c {H}
+//   I3   c2r {V_LJ*rho + Pi*Q + k23 c F*H}  (all linear terms summed in k space)
+//   P4   V += ...
+// i.e. 10 forward + 15 inverse real 3-D transforms, against 34 in the
+// reference.  The derivative kernels dF_d are never stored: their spectra
+// are F^ * i sin(k_d h)/h, the symbol of the periodic central difference
+// the reference applies to F before transforming (src/ot.c:196).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace dft;
+
+namespace dft {
+
+#define GRID_STRIDE(i, n) \
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// ------------------------------------------------------- kernel functions --
+struct KParams {
+  double lj_h, lj_sigma, lj_eps, lj_cval;
+  double sa_radius;
+  double g_inv_width;
+  double g11, g12, g21, g22, a1, a2;
+};
+
+// src/common.c:31-66 (LJ, zero core), 131-138 (spherical average),
+// 193-200 (gaussian F), src/ot-private.h:11-22 (backflow U_J).
+__device__ inline double kfun(const KParams &p, int which, double x, double y, double z) {
+#pragma clang fp contract(off)
+  const double r2 = x * x + y * y + z * z;
+  if (which == 0) {
+    if (r2 <= p.lj_h * p.lj_h) return p.lj_cval;
+    double s = p.lj_sigma * p.lj_sigma / r2;
+    s = s * s * s;
+    return 4.0 * p.lj_eps * s * (s - 1.0);
+  } else if (which == 1) {
+    const double h = p.sa_radius;
+    return (r2 <= h * h) ? 3.0 / (4.0 * M_PI * h * h * h) : 0.0;
+  } else if (which == 2) {
+    const double iw = p.g_inv_width;
+    double nrm = 0.5 * (2.0 / sqrt(M_PI)) * iw;
+    nrm = nrm * nrm * nrm;
+    return nrm * exp(-r2 * iw * iw);
+  }
+  return (p.g11 + p.g12 * r2) * exp(-p.a1 * r2) + (p.g21 + p.g22 * r2) * exp(-p.a2 * r2);
+}
+
+// rgrid_adaptive_map (libgrid): cell average over an ns^3 midpoint sub-grid,
+// ns = min_sub, 2 min_sub, ... max_sub, stopping when two successive averages
+// differ by less than tol.  Sampled about the index origin (wrapped
+// coordinates), which equals the reference's r = 0 at n/2 with the
+// (-1)^(i+j+k) factor of rgrid_fft_convolute.
+__global__ void k_kernel_map(double *out, int nx, int ny, int nz, double h, int which, KParams kp, int min_sub,
+                             int max_sub, double tol) {
+#pragma clang fp contract(off)
+  const int64_t n = (int64_t)nx * ny * nz;
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
+    const double x = (double)(i < nx / 2 ? i : i - nx) * h, y = (double)(j < ny / 2 ? j : j - ny) * h,
+                 z = (double)(k < nz / 2 ? k : k - nz) * h;
+    double prev = 0.0;
+    for (int ns = min_sub; ns <= max_sub; ns *= 2) {
+      double s = 0.0;
+      for (int a = 0; a < ns; ++a) {
+        const double xs = x + (((double)a + 0.5) / (double)ns - 0.5) * h;
+        for (int b = 0; b < ns; ++b) {
+          const double ys = y + (((double)b + 0.5) / (double)ns - 0.5) * h;
+          for (int cc = 0; cc < ns; ++cc) {
+            const double zs = z + (((double)cc + 0.5) / (double)ns - 0.5) * h;
+            s += kfun(kp, which, xs, ys, zs);
+          }
+        }
+      }
+      const double cur = s / ((double)ns * (double)ns * (double)ns);
+      const bool done = (ns > min_sub) && fabs(cur - prev) < tol;
+      prev = cur;
+      if (done) break;
+    }
+    out[c] = prev;
+  }
+}
+
+template <typename T> __global__ void k_spec_real(const double2 *spec, T *out, int64_t nh, double scale) {
+  GRID_STRIDE(i, nh) out[i] = (T)(spec[i].x * scale);
+}
+
+// --------------------------------------------------------- pointwise math --
+template <typename T> struct PW {
+  int nx, ny, nz;
+  T inv2h;                 // 1 / 2h
+  T invM;                  // 1 / wavefunction mass (velocity)
+  T eps_v;                 // density epsilon in the velocity
+  T c2, c3;                // local correlation
+  T inv_rho0s;             // 1 / rho_0s
+  T ckc;                   // alpha_s / (2 mass)
+  T k23;                   // KC terms 2,3 coefficient
+  T mass;                  // functional mass
+  T div_eps;               // backflow density epsilon
+  T cbf;                   // backflow weight
+  T vmax, bfmax;           // clamps
+  int clamp_v, clamp_bf;
+};
+
+struct Nbr {
+  int64_t p[3], m[3];
+};
+__device__ inline Nbr neighbours(int64_t c, int nx, int ny, int nz) {
+  const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
+  const int64_t sx = (int64_t)ny * nz;
+  Nbr r;
+  r.p[0] = c + (i == nx - 1 ? -(int64_t)(nx - 1) * sx : sx);
+  r.m[0] = c + (i == 0 ? (int64_t)(nx - 1) * sx : -sx);
+  r.p[1] = c + (j == ny - 1 ? -(int64_t)(ny - 1) * nz : (int64_t)nz);
+  r.m[1] = c + (j == 0 ? (int64_t)(ny - 1) * nz : -(int64_t)nz);
+  r.p[2] = c + (k == nz - 1 ? -(int64_t)(nz - 1) : 1);
+  r.m[2] = c + (k == 0 ? (int64_t)(nz - 1) : -1);
+  return r;
+}
+
+template <typename T> __device__ inline T clampv(T v, T lim) {
+  if (v > lim) v = lim;
+  if (v < -lim) v = -lim;
+  return v;
+}
+
+// P1: psi -> rho and, for backflow, v_d (grid_wf_velocity), q_d = v_d rho,
+// q4 = |v|^2 rho (src/ot.c:385-396, 852-891).
+template <typename T, bool BF>
+__global__ void k_prep(const cplx<T> *__restrict__ psi, T *__restrict__ rho, T *__restrict__ vx, T *__restrict__ vy,
+                       T *__restrict__ vz, T *__restrict__ qx, T *__restrict__ qy, T *__restrict__ qz,
+                       T *__restrict__ q4, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const cplx<T> s = psi[c];
+    const T r = s.x * s.x + s.y * s.y;
+    rho[c] = r;
+    if constexpr (BF) {
+      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+      const T den = pw.invM / (r + pw.eps_v);
+      T v[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const cplx<T> a = psi[nb.p[d]], b = psi[nb.m[d]];
+        const T dx = (a.x - b.x) * pw.inv2h, dy = (a.y - b.y) * pw.inv2h;
+        v[d] = (s.x * dy - s.y * dx) * den;
+        if (pw.clamp_v) v[d] = clampv(v[d], pw.vmax);
+      }
+      vx[c] = v[0];
+      vy[c] = v[1];
+      vz[c] = v[2];
+      qx[c] = v[0] * r;
+      qy[c] = v[1] * r;
+      qz[c] = v[2] * r;
+      q4[c] = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * r;
+    }
+  }
+}
+
+// P1 for given density and velocities (dft_ot_potential2, dft_ot_backflow_potential).
+template <typename T>
+__global__ void k_prep_v(const T *__restrict__ rho, T *vx, T *vy, T *vz, T *__restrict__ qx, T *__restrict__ qy,
+                         T *__restrict__ qz, T *__restrict__ q4, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const T r = rho[c];
+    T v[3] = {vx[c], vy[c], vz[c]};
+    if (pw.clamp_v) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = clampv(v[d], pw.vmax);
+      vx[c] = v[0];
+      vy[c] = v[1];
+      vz[c] = v[2];
+    }
+    qx[c] = v[0] * r;
+    qy[c] = v[1] * r;
+    qz[c] = v[2] * r;
+    q4[c] = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * r;
+  }
+}
+
+struct StageBIn {
+  const void *rho, *rhobar, *rhot, *vx, *vy, *vz, *A, *Bx, *By, *Bz, *C;
+};
+struct StageBOut {
+  void *Q, *Gx, *Gy, *Gz, *Vre, *Vim, *Wx, *Wy, *Wz;
+};
+
+// P2: src/ot.c:547-586 (local terms), 604-656 (G_d), 893-961 (BF real part,
+// W_d and the (d rho/(rho+eps)) W_d half of the imaginary part).
+template <typename T, bool LOC, bool KC, bool BF>
+__global__ void k_stageB(StageBIn in, StageBOut out, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *rho = (const T *)in.rho;
+  GRID_STRIDE(c, n) {
+    const T r = rho[c];
+    T vre = 0;
+    if constexpr (LOC) {
+      const T rb = ((const T *)in.rhobar)[c];
+      vre = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
+      ((T *)out.Q)[c] = (rb * pw.c2 + rb * rb * pw.c3) * r;
+    }
+    T g[3] = {0, 0, 0};
+    if constexpr (KC || BF) {
+      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) g[d] = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+    }
+    if constexpr (KC) {
+      const T rst = ((const T *)in.rhot)[c] * (-pw.inv_rho0s) + (T)1;
+      ((T *)out.Gx)[c] = g[0] * rst;
+      ((T *)out.Gy)[c] = g[1] * rst;
+      ((T *)out.Gz)[c] = g[2] * rst;
+    }
+    if constexpr (BF) {
+      const T vx = ((const T *)in.vx)[c], vy = ((const T *)in.vy)[c], vz = ((const T *)in.vz)[c];
+      const T A = ((const T *)in.A)[c], Bx = ((const T *)in.Bx)[c], By = ((const T *)in.By)[c],
+              Bz = ((const T *)in.Bz)[c], Cc = ((const T *)in.C)[c];
+      T t = (vz * vz + vy * vy + vx * vx) * A;
+      t += (T)-2 * vx * Bx;
+      t += (T)-2 * vy * By;
+      t += (T)-2 * vz * Bz;
+      t = t + Cc;
+      T re = t * ((T)-0.5 * pw.mass);
+      if (pw.clamp_bf) re = clampv(re, pw.bfmax);
+      vre += re * pw.cbf;
+      const T Wx = vx * A - Bx, Wy = vy * A - By, Wz = vz * A - Bz;
+      ((T *)out.Wx)[c] = Wx;
+      ((T *)out.Wy)[c] = Wy;
+      ((T *)out.Wz)[c] = Wz;
+      const T inv = (T)1 / (r + pw.div_eps);
+      T im = (T)0.5 * (g[0] * inv) * Wx;
+      im += (T)0.5 * (g[1] * inv) * Wy;
+      im += (T)0.5 * (g[2] * inv) * Wz;
+      ((T *)out.Vim)[c] = im;
+    }
+    ((T *)out.Vre)[c] = vre;
+  }
+}
+
+struct StageCIn {
+  const void *rho, *rhot, *T1, *Jx, *Jy, *Jz, *Px, *Py, *Pz, *Wx, *Wy, *Wz;
+};
+struct StageCOut {
+  void *H, *Vre, *Vim;
+};
+
+// P3: src/ot.c:640-675 (KC terms 1 and 3, H_d = d_d rho J_d summed over d
+// because F*(sum_d H_d) = sum_d F*H_d), 963-990 (BF d_d W_d half of Im V).
+template <typename T, bool KC, bool BF>
+__global__ void k_stageC(StageCIn in, StageCOut out, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    if constexpr (KC) {
+      const T *rho = (const T *)in.rho;
+      const T rst = ((const T *)in.rhot)[c] * (-pw.inv_rho0s) + (T)1;
+      const T J[3] = {((const T *)in.Jx)[c], ((const T *)in.Jy)[c], ((const T *)in.Jz)[c]};
+      const T Pd[3] = {((const T *)in.Px)[c], ((const T *)in.Py)[c], ((const T *)in.Pz)[c]};
+      T H = 0, t3 = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+        H += g * J[d];
+        t3 += Pd[d] * J[d];
+      }
+      ((T *)out.H)[c] = H;
+      ((T *)out.Vre)[c] += ((const T *)in.T1)[c] * rst * pw.ckc - t3 * (pw.ckc * pw.k23);
+    }
+    if constexpr (BF) {
+      const T *W[3] = {(const T *)in.Wx, (const T *)in.Wy, (const T *)in.Wz};
+      T im = ((const T *)out.Vim)[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) im += (T)0.5 * ((W[d][nb.p[d]] - W[d][nb.m[d]]) * pw.inv2h);
+      if (pw.clamp_bf) im = clampv(im, pw.bfmax);
+      ((T *)out.Vim)[c] = im * pw.cbf;
+    }
+  }
+}
+
+// P4: V += (Vre + lin) + i Vim
+template <typename T>
+__global__ void k_final(cplx<T> *V, const T *__restrict__ vre, const T *__restrict__ lin, const T *__restrict__ vim,
+                        int64_t n) {
+  GRID_STRIDE(c, n) {
+    cplx<T> v = V[c];
+    T add = vre ? vre[c] : (T)0;
+    if (lin) add += lin[c];
+    v.x += add;
+    if (vim) v.y += vim[c];
+    V[c] = v;
+  }
+}
+
+// Energy density, src/ot-energy.c:64-93 (plain), 202-288 (BF), and the
+// G_d of 144-161 (KC).  acc: add to eps instead of overwriting it.
+struct EnergyIn {
+  const void *rho, *ljr, *rhobar, *rhot, *vx, *vy, *vz, *A, *Bx, *By, *Bz, *C;
+};
+template <typename T, bool LOC, bool KC, bool BF>
+__global__ void k_energyB(EnergyIn in, T *eps, T *Gx, T *Gy, T *Gz, PW<T> pw, int acc) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *rho = (const T *)in.rho;
+  GRID_STRIDE(c, n) {
+    const T r = rho[c];
+    T e = acc ? eps[c] : (T)0;
+    if constexpr (LOC) {
+      const T rb = ((const T *)in.rhobar)[c];
+      e += (T)0.5 * r * ((const T *)in.ljr)[c];
+      e += (pw.c2 * (T)0.5) * (rb * rb * r);
+      e += (pw.c3 / (T)3) * (rb * rb * rb * r);
+    }
+    if constexpr (KC) {
+      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+      const T rst = ((const T *)in.rhot)[c] * (-pw.inv_rho0s) + (T)1;
+      Gx[c] = (rho[nb.p[0]] - rho[nb.m[0]]) * pw.inv2h * rst;
+      Gy[c] = (rho[nb.p[1]] - rho[nb.m[1]]) * pw.inv2h * rst;
+      Gz[c] = (rho[nb.p[2]] - rho[nb.m[2]]) * pw.inv2h * rst;
+    }
+    if constexpr (BF) {
+      const T vx = ((const T *)in.vx)[c], vy = ((const T *)in.vy)[c], vz = ((const T *)in.vz)[c];
+      const T v2 = (vx * vx + vy * vy) + vz * vz;
+      const T mq = pw.cbf * pw.mass * (T)0.25, mh = pw.cbf * pw.mass * (T)0.5;
+      e += -mq * (((const T *)in.A)[c] * v2 * r);
+      e += mh * (((const T *)in.Bx)[c] * r * vx);
+      e += mh * (((const T *)in.By)[c] * r * vy);
+      e += mh * (((const T *)in.Bz)[c] * r * vz);
+      e += -mq * (((const T *)in.C)[c] * r);
+    }
+    eps[c] = e;
+  }
+}
+
+// KC energy: eps += -(alpha_s / 4M) sum_d J_d G_d   (src/ot-energy.c:163-190)
+template <typename T>
+__global__ void k_energyC(T *eps, const T *__restrict__ Jx, const T *__restrict__ Jy, const T *__restrict__ Jz,
+                          const T *__restrict__ Gx, const T *__restrict__ Gy, const T *__restrict__ Gz, T s,
+                          int64_t n) {
+  GRID_STRIDE(c, n) {
+    T e = eps[c];
+    e += s * (Jx[c] * Gx[c]);
+    e += s * (Jy[c] * Gy[c]);
+    e += s * (Jz[c] * Gz[c]);
+    eps[c] = e;
+  }
+}
+
+}  // namespace dft
+
+// ============================================================ device state ==
+
+enum Kern { K_LJ = 0, K_SA = 1, K_F = 2, K_U = 3 };
+
+// Slot roles (each slot holds a real field or a real-field half spectrum).
+enum Slot {
+  SP0, SP1, SP2, SP3, SP4,                  // spectra
+  VX, VY, VZ, QX, QY, QZ, Q4,               // velocities, products
+  RB, RT, PX, PY, PZ, SA_, SBX, SBY, SBZ, SC,  // first inverse batch
+  TMP0, TMP1, TMP2, TMP3, TMP4, TMP5, TMP6, TMP7, TMP8, TMP9,
+  VRE, VIM, WX, WY, WZ, SH,
+  NSLOT
+};
+
+struct dft_ot_device {
+  int prec;
+  int nx, ny, nz, P;
+  int64_t n, slot_elems;
+  double step;
+  void *plan;                // Plan3D<float> or Plan3D<double>
+  void *slot[NSLOT];
+  void *kern[4];             // real half spectra (LJ, SA, F, U)
+};
+
+namespace {
+
+template <typename T> Plan3D<T> &plan_of(dft_ot_device *d) { return *(Plan3D<T> *)d->plan; }
+template <typename T> T *S(dft_ot_device *d, int i) { return (T *)d->slot[i]; }
+template <typename T> cplx<T> *CS(dft_ot_device *d, int i) { return (cplx<T> *)d->slot[i]; }
+
+fft::Combine cmb(int nterms, const fft::Term *t) {
+  fft::Combine c{};
+  c.nterms = nterms;
+  for (int i = 0; i < nterms; ++i) c.t[i] = t[i];
+  return c;
+}
+fft::Combine cmb1(int in, int kern, int deriv, double scale) {
+  fft::Term t{in, kern, deriv, scale};
+  return cmb(1, &t);
+}
+
+template <typename T> fft::KernTab kerntab(dft_ot_device *d) {
+  fft::KernTab kt{};
+  for (int i = 0; i < 4; ++i) kt.k[i] = d->kern[i];
+  Plan3D<T> &p = plan_of<T>(d);
+  kt.sinx = p.sinx;
+  kt.siny = p.siny;
+  kt.sinz = p.sinz;
+  return kt;
+}
+
+template <typename T> PW<T> make_pw(dft_ot_functional *otf, double wf_mass, double eps_v, bool clamp_v) {
+  dft_ot_device *d = otf->device;
+  PW<T> pw{};
+  pw.nx = d->nx;
+  pw.ny = d->ny;
+  pw.nz = d->nz;
+  pw.inv2h = (T)(0.5 / d->step);
+  pw.invM = (T)(1.0 / wf_mass);
+  pw.eps_v = (T)eps_v;
+  pw.c2 = (T)(otf->c2 * otf->c2_exp / 2.0);
+  pw.c3 = (T)(otf->c3 * otf->c3_exp / 3.0);
+  pw.inv_rho0s = (T)(1.0 / otf->rho_0s);
+  pw.ckc = (T)(otf->alpha_s / (2.0 * otf->mass));
+  pw.k23 = (T)otf->kc_coef23;
+  pw.mass = (T)otf->mass;
+  pw.div_eps = (T)otf->div_epsilon;
+  pw.cbf = (T)otf->c_bfpot;
+  pw.vmax = (T)otf->max_veloc;
+  pw.bfmax = (T)otf->max_bfpot;
+  pw.clamp_v = clamp_v && otf->max_veloc < (999.0 / GRID_AUTOMPS);
+  pw.clamp_bf = otf->max_bfpot < (10000.0 / GRID_AUTOK);
+  return pw;
+}
+
+#define LAUNCH_PW(kern, ...)                                                            \
+  do {                                                                                  \
+    hipLaunchKernelGGL(kern, dim3(grid_for(d->n)), dim3(kBlock), 0, lib_stream(), __VA_ARGS__); \
+    DFT_LAUNCH_CHECK();                                                                  \
+  } while (0)
+
+// Which parts of the potential to evaluate.
+struct PotParts {
+  bool loc, kc, bf;
+};
+
+// Shared body of dft_ot_potential / potential2 / backflow_potential.
+// rho (device) holds the density on entry if psi == nullptr.
+template <typename T>
+void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double wf_mass, T *rho, T *vx, T *vy,
+                    T *vz, PotParts parts, bool write_back_w) {
+  dft_ot_device *d = otf->device;
+  Plan3D<T> &p = plan_of<T>(d);
+  hipStream_t s = lib_stream();
+  const double h3N = d->step * d->step * d->step / (double)d->n;
+  const bool KC = parts.kc, BF = parts.bf, LOC = parts.loc;
+  fft::KernTab kt = kerntab<T>(d);
+  PW<T> pw = make_pw<T>(otf, wf_mass, otf->div_epsilon, true);
+
+  // P1
+  if (psi) {
+    if (BF) {
+      vx = S<T>(d, VX);
+      vy = S<T>(d, VY);
+      vz = S<T>(d, VZ);
+      LAUNCH_PW((k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+    } else {
+      LAUNCH_PW((k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
+    }
+  } else if (BF) {
+    LAUNCH_PW((k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+  }
+  // F1
+  {
+    std::vector<const T *> in{rho};
+    std::vector<cplx<T> *> out{CS<T>(d, SP0)};
+    if (BF) {
+      for (int i : {QX, QY, QZ, Q4}) in.push_back(S<T>(d, i));
+      for (int i : {SP1, SP2, SP3, SP4}) out.push_back(CS<T>(d, i));
+    }
+    r2c3d<T>(p, (int)in.size(), in.data(), out.data(), s);
+  }
+  // I1
+  {
+    const cplx<T> *in[5] = {CS<T>(d, SP0), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3), CS<T>(d, SP4)};
+    std::vector<fft::Combine> c;
+    std::vector<T *> out;
+    if (LOC) {
+      c.push_back(cmb1(0, K_SA, -1, h3N));
+      out.push_back(S<T>(d, RB));
+    }
+    if (KC) {
+      c.push_back(cmb1(0, K_F, -1, h3N));
+      out.push_back(S<T>(d, RT));
+      for (int dd = 0; dd < 3; ++dd) {
+        c.push_back(cmb1(0, K_F, dd, h3N));
+        out.push_back(S<T>(d, PX + dd));
+      }
+    }
+    if (BF) {
+      c.push_back(cmb1(0, K_U, -1, h3N));
+      out.push_back(S<T>(d, SA_));
+      for (int dd = 0; dd < 3; ++dd) {
+        c.push_back(cmb1(1 + dd, K_U, -1, h3N));
+        out.push_back(S<T>(d, SBX + dd));
+      }
+      c.push_back(cmb1(4, K_U, -1, h3N));
+      out.push_back(S<T>(d, SC));
+    }
+    std::vector<cplx<T> *> tmp;
+    for (size_t i = 0; i < out.size(); ++i) tmp.push_back(CS<T>(d, TMP0 + (int)i));
+    if (!out.empty()) c2r3d<T>(p, 5, in, (int)out.size(), c.data(), kt, tmp.data(), out.data(), s);
+  }
+  // P2
+  {
+    StageBIn bi{rho, S<T>(d, RB), S<T>(d, RT), vx, vy, vz, S<T>(d, SA_), S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ),
+                S<T>(d, SC)};
+    StageBOut bo{S<T>(d, Q4), S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, VRE), S<T>(d, VIM),
+                 S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
+#define SB(L, K, B) LAUNCH_PW((k_stageB<T, L, K, B>), bi, bo, pw)
+    if (LOC && KC && BF) SB(true, true, true);
+    else if (LOC && KC) SB(true, true, false);
+    else if (LOC && BF) SB(true, false, true);
+    else if (LOC) SB(true, false, false);
+    else if (KC && BF) SB(false, true, true);
+    else if (KC) SB(false, true, false);
+    else if (BF) SB(false, false, true);
+#undef SB
+  }
+  // F2
+  {
+    std::vector<const T *> in;
+    std::vector<cplx<T> *> out;
+    if (LOC) {
+      in.push_back(S<T>(d, Q4));
+      out.push_back(CS<T>(d, SP4));
+    }
+    if (KC)
+      for (int dd = 0; dd < 3; ++dd) {
+        in.push_back(S<T>(d, QX + dd));
+        out.push_back(CS<T>(d, SP1 + dd));
+      }
+    if (!in.empty()) r2c3d<T>(p, (int)in.size(), in.data(), out.data(), s);
+  }
+  // I2
+  if (KC) {
+    const cplx<T> *in[3] = {CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3)};
+    fft::Combine c[4];
+    T *out[4] = {S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ), S<T>(d, SA_)};
+    for (int dd = 0; dd < 3; ++dd) c[dd] = cmb1(dd, K_F, -1, h3N);
+    fft::Term t[3] = {{0, K_F, 0, h3N}, {1, K_F, 1, h3N}, {2, K_F, 2, h3N}};
+    c[3] = cmb(3, t);
+    cplx<T> *tmp[4] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2), CS<T>(d, TMP3)};
+    c2r3d<T>(p, 3, in, 4, c, kt, tmp, out, s);
+  }
+  // P3
+  if (KC || BF) {
+    StageCIn ci{rho, S<T>(d, RT), S<T>(d, SA_), S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ), S<T>(d, PX),
+                S<T>(d, PY), S<T>(d, PZ), S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
+    StageCOut co{S<T>(d, SC), S<T>(d, VRE), S<T>(d, VIM)};
+    if (KC && BF) LAUNCH_PW((k_stageC<T, true, true>), ci, co, pw);
+    else if (KC) LAUNCH_PW((k_stageC<T, true, false>), ci, co, pw);
+    else LAUNCH_PW((k_stageC<T, false, true>), ci, co, pw);
+  }
+  // F3 + I3
+  if (LOC || KC) {
+    if (KC) {
+      const T *in[1] = {S<T>(d, SC)};
+      cplx<T> *out[1] = {CS<T>(d, SH)};
+      r2c3d<T>(p, 1, in, out, s);
+    }
+    const cplx<T> *in[3] = {CS<T>(d, SP0), CS<T>(d, SP4), CS<T>(d, SH)};
+    fft::Term t[3];
+    int nt = 0;
+    if (LOC) {
+      t[nt++] = {0, K_LJ, -1, h3N};
+      t[nt++] = {1, K_SA, -1, h3N};
+    }
+    if (KC) t[nt++] = {2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass)};
+    fft::Combine c = cmb(nt, t);
+    T *out[1] = {S<T>(d, RB)};
+    cplx<T> *tmp[1] = {CS<T>(d, TMP0)};
+    c2r3d<T>(p, 3, in, 1, &c, kt, tmp, out, s);
+  }
+  // P4
+  LAUNCH_PW((k_final<T>), (cplx<T> *)V->dev, S<T>(d, VRE), (LOC || KC) ? S<T>(d, RB) : (T *)nullptr,
+            BF ? S<T>(d, VIM) : (T *)nullptr, d->n);
+  if (write_back_w && BF) {
+    // the reference leaves v_d A - B_d in the velocity inputs (src/ot.c:923-931)
+    DFT_HIP_CHECK(hipMemcpyAsync(vx, S<T>(d, WX), sizeof(T) * d->n, hipMemcpyDeviceToDevice, s));
+    DFT_HIP_CHECK(hipMemcpyAsync(vy, S<T>(d, WY), sizeof(T) * d->n, hipMemcpyDeviceToDevice, s));
+    DFT_HIP_CHECK(hipMemcpyAsync(vz, S<T>(d, WZ), sizeof(T) * d->n, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+// Energy density: eps = (acc ? eps : 0) + selected parts.  rho holds the
+// density on entry when psi == nullptr (the _kc / _bf entry points).
+template <typename T>
+void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_mass, T *rho, bool rho_given,
+                 PotParts parts, bool acc) {
+  dft_ot_device *d = otf->device;
+  Plan3D<T> &p = plan_of<T>(d);
+  hipStream_t s = lib_stream();
+  const double h3N = d->step * d->step * d->step / (double)d->n;
+  const bool KC = parts.kc, BF = parts.bf, LOC = parts.loc;
+  fft::KernTab kt = kerntab<T>(d);
+  PW<T> pw = make_pw<T>(otf, wf_mass, DFT_EPS, false);
+  T *vx = S<T>(d, VX), *vy = S<T>(d, VY), *vz = S<T>(d, VZ);
+  if (BF && !psi) {
+    fprintf(stderr, "libdft: backflow energy needs the wave function\n");
+    abort();
+  }
+  if (!rho_given) {
+    // density (and velocities) from psi
+    if (BF)
+      LAUNCH_PW((k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+    else
+      LAUNCH_PW((k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
+  } else if (BF) {
+    // velocities from psi (grid_wf_velocity, src/ot-energy.c:231), products with the given density
+    LAUNCH_PW((k_prep<T, true>), psi, S<T>(d, VRE), vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4),
+              pw);
+    LAUNCH_PW((k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+  }
+  {
+    std::vector<const T *> in{rho};
+    std::vector<cplx<T> *> out{CS<T>(d, SP0)};
+    if (BF) {
+      for (int i : {QX, QY, QZ, Q4}) in.push_back(S<T>(d, i));
+      for (int i : {SP1, SP2, SP3, SP4}) out.push_back(CS<T>(d, i));
+    }
+    r2c3d<T>(p, (int)in.size(), in.data(), out.data(), s);
+  }
+  {
+    const cplx<T> *in[5] = {CS<T>(d, SP0), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3), CS<T>(d, SP4)};
+    std::vector<fft::Combine> c;
+    std::vector<T *> out;
+    if (LOC) {
+      c.push_back(cmb1(0, K_LJ, -1, h3N));
+      out.push_back(S<T>(d, PX));
+      c.push_back(cmb1(0, K_SA, -1, h3N));
+      out.push_back(S<T>(d, RB));
+    }
+    if (KC) {
+      c.push_back(cmb1(0, K_F, -1, h3N));
+      out.push_back(S<T>(d, RT));
+    }
+    if (BF) {
+      c.push_back(cmb1(0, K_U, -1, h3N));
+      out.push_back(S<T>(d, SA_));
+      for (int dd = 0; dd < 3; ++dd) {
+        c.push_back(cmb1(1 + dd, K_U, -1, h3N));
+        out.push_back(S<T>(d, SBX + dd));
+      }
+      c.push_back(cmb1(4, K_U, -1, h3N));
+      out.push_back(S<T>(d, SC));
+    }
+    std::vector<cplx<T> *> tmp;
+    for (size_t i = 0; i < out.size(); ++i) tmp.push_back(CS<T>(d, TMP0 + (int)i));
+    c2r3d<T>(p, 5, in, (int)out.size(), c.data(), kt, tmp.data(), out.data(), s);
+  }
+  {
+    EnergyIn ei{rho, S<T>(d, PX), S<T>(d, RB), S<T>(d, RT), vx, vy, vz, S<T>(d, SA_), S<T>(d, SBX), S<T>(d, SBY),
+                S<T>(d, SBZ), S<T>(d, SC)};
+    T *G[3] = {S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ)};
+    const int a = acc ? 1 : 0;
+#define EB(L, K, B) LAUNCH_PW((k_energyB<T, L, K, B>), ei, eps, G[0], G[1], G[2], pw, a)
+    if (LOC && KC && BF) EB(true, true, true);
+    else if (LOC && KC) EB(true, true, false);
+    else if (LOC && BF) EB(true, false, true);
+    else if (LOC) EB(true, false, false);
+    else if (KC && BF) EB(false, true, true);
+    else if (KC) EB(false, true, false);
+    else if (BF) EB(false, false, true);
+#undef EB
+    if (KC) {
+      const T *in3[3] = {S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ)};
+      cplx<T> *sp[3] = {CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3)};
+      r2c3d<T>(p, 3, in3, sp, s);
+      const cplx<T> *cin[3] = {sp[0], sp[1], sp[2]};
+      fft::Combine c[3];
+      for (int dd = 0; dd < 3; ++dd) c[dd] = cmb1(dd, K_F, -1, h3N);
+      T *J[3] = {S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ)};
+      cplx<T> *tmp[3] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2)};
+      c2r3d<T>(p, 3, cin, 3, c, kt, tmp, J, s);
+      LAUNCH_PW((k_energyC<T>), eps, J[0], J[1], J[2], G[0], G[1], G[2], (T)(-otf->alpha_s / (4.0 * otf->mass)),
+                d->n);
+    }
+  }
+}
+
+// Build the kernel spectra (FP64 on the device), src/ot.c:138-217.
+template <typename T> bool build_kernels(dft_ot_functional *otf, int min_sub, int max_sub) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  const int64_t nh = (int64_t)d->nx * d->ny * d->P;
+  Plan3D<double> p64;
+  p64.init(d->nx, d->ny, d->nz, d->step);
+  double *field = (double *)dmalloc(sizeof(double) * d->n);
+  double2 *spec = (double2 *)dmalloc(sizeof(double2) * nh);
+  if (!field || !spec) return false;
+  KParams kp{};
+  kp.lj_h = otf->lj_params.h;
+  kp.lj_sigma = otf->lj_params.sigma;
+  kp.lj_eps = otf->lj_params.epsilon;
+  kp.lj_cval = otf->lj_params.cval;
+  kp.sa_radius = otf->lj_params.h;  // original spherical-average radius (no HD2)
+  kp.g_inv_width = 1.0 / otf->l_g;
+  kp.g11 = otf->bf_params.g11;
+  kp.g12 = otf->bf_params.g12;
+  kp.g21 = otf->bf_params.g21;
+  kp.g22 = otf->bf_params.g22;
+  kp.a1 = otf->bf_params.a1;
+  kp.a2 = otf->bf_params.a2;
+  const double tol = 0.01 / GRID_AUTOK;
+  const double h3 = d->step * d->step * d->step;
+  const bool need[4] = {true, true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
+  for (int w = 0; w < 4; ++w) {
+    if (!need[w]) continue;
+    hipLaunchKernelGGL(k_kernel_map, dim3(grid_for(d->n)), dim3(kBlock), 0, s, field, d->nx, d->ny, d->nz, d->step,
+                       w, kp, min_sub, max_sub, tol);
+    DFT_LAUNCH_CHECK();
+    const double *in[1] = {field};
+    double2 *out[1] = {spec};
+    r2c3d<double>(p64, 1, in, out, s);
+    double dc[2];
+    DFT_HIP_CHECK(hipMemcpyAsync(dc, spec, sizeof(double2), hipMemcpyDeviceToHost, s));
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    double scale = 1.0;
+    if (w == K_LJ) scale = otf->b / (h3 * dc[0]);   // integral of V_LJ := b (src/ot.c:160)
+    if (w == K_SA) scale = 1.0 / (h3 * dc[0]);      // integral of Pi := 1 (src/ot.c:181)
+    hipLaunchKernelGGL(k_spec_real<T>, dim3(grid_for(nh)), dim3(kBlock), 0, s, spec, (T *)d->kern[w], nh, scale);
+    DFT_LAUNCH_CHECK();
+  }
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+  dfree(field);
+  dfree(spec);
+  p64.destroy();
+  return true;
+}
+
+void free_device(dft_ot_device *d) {
+  if (!d) return;
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  for (int i = 0; i < NSLOT; ++i) dfree(d->slot[i]);
+  for (int i = 0; i < 4; ++i) dfree(d->kern[i]);
+  if (d->plan) {
+    if (d->prec == GRID_FP32) {
+      ((Plan3D<float> *)d->plan)->destroy();
+      delete (Plan3D<float> *)d->plan;
+    } else {
+      ((Plan3D<double> *)d->plan)->destroy();
+      delete (Plan3D<double> *)d->plan;
+    }
+  }
+  delete d;
+}
+
+rgrid *spectral_handle(dft_ot_functional *otf, int w, const char *id) {
+  dft_ot_device *d = otf->device;
+  rgrid *g = (rgrid *)calloc(1, sizeof(rgrid));
+  g->nx = d->nx;
+  g->ny = d->ny;
+  g->nz = d->nz;
+  g->step = d->step;
+  g->dev = d->kern[w];
+  g->dev_elems = (int64_t)d->nx * d->ny * d->P;
+  g->prec = d->prec;
+  g->spectral = 1;
+  strncpy(g->id, id, sizeof(g->id) - 1);
+  return g;
+}
+
+bool supported_model(INT model) {
+  return (model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW)) == 0;
+}
+
+void check_wf(dft_ot_functional *otf, wf *gwf, const char *fn) {
+  dft_ot_device *d = otf->device;
+  cgrid *g = gwf->grid;
+  if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
+    fprintf(stderr, "libdft: %s: wave function does not match the functional's grid\n", fn);
+    abort();
+  }
+}
+
+void check_r(dft_ot_functional *otf, rgrid *g, const char *fn) {
+  dft_ot_device *d = otf->device;
+  if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
+    fprintf(stderr, "libdft: %s: grid does not match the functional's grid\n", fn);
+    abort();
+  }
+}
+
+}  // namespace
+
+// ================================================================== C ABI ==
+extern "C" {
+
+// src/ot.c:1002-1271: T = 0 parameter row (1026-1036) and common block (1234-1270).
+void dft_ot_init_params(dft_ot_functional *otf, INT model) {
+  const double a3 = GRID_AUTOANG * GRID_AUTOANG * GRID_AUTOANG;
+  otf->beta = otf->rhom = otf->C = 0.0;
+  otf->b = -718.99;
+  otf->c2 = -2.411857E4;
+  otf->c2_exp = 2;
+  otf->c3 = 1.858496E6;
+  otf->c3_exp = 3;
+  otf->c4 = 0.0;
+  otf->temp = 0.0;
+  otf->rho0 = 0.0218360;
+  otf->lj_params.h = 2.1903;
+  otf->mu0 = 0.0;
+  otf->xi = 1E4 / a3;
+  otf->rhobf = 0.033 * a3;
+  (void)model;
+  otf->b /= GRID_AUTOK * a3;
+  otf->c2 /= GRID_AUTOK * std::pow(GRID_AUTOANG, 3.0 * otf->c2_exp);
+  otf->c3 /= GRID_AUTOK * std::pow(GRID_AUTOANG, 3.0 * otf->c3_exp);
+  otf->rho0 *= a3;
+  otf->lj_params.h /= GRID_AUTOANG;
+  otf->lj_params.sigma = 2.556 / GRID_AUTOANG;
+  otf->lj_params.epsilon = 10.22 / GRID_AUTOK;
+  otf->lj_params.cval = 0.0;
+  otf->rho_0s = 0.04 * a3;
+  otf->alpha_s = 54.31 / a3;
+  otf->l_g = 1.0 / GRID_AUTOANG;
+  otf->mass = 4.0026 / GRID_AUTOAMU;
+  otf->bf_params.g11 = -19.7544;
+  otf->bf_params.g12 = 12.5616 * GRID_AUTOANG * GRID_AUTOANG;
+  otf->bf_params.a1 = 1.023 * GRID_AUTOANG * GRID_AUTOANG;
+  otf->bf_params.g21 = -0.2395;
+  otf->bf_params.g22 = 0.0312 * GRID_AUTOANG * GRID_AUTOANG;
+  otf->bf_params.a2 = 0.14912 * GRID_AUTOANG * GRID_AUTOANG;
+  otf->div_epsilon = DFT_EPS;
+  otf->max_veloc = 999.0 / GRID_AUTOMPS;
+  otf->max_bfpot = 10000.0 / GRID_AUTOK;
+  otf->c_bfpot = 1.0;
+  otf->kc_coef23 = 1.0;
+}
+
+// src/ot.c:66-258
+dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_substeps) {
+  if (!supported_model(model)) {
+    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC and BACKFLOW only)\n", (long)model);
+    return nullptr;
+  }
+  cgrid *g = gwf->grid;
+  if (g->nx < 4 || g->ny < 4 || g->nz < 4) {
+    fprintf(stderr, "libdft: dft_ot_alloc: 1-D and 2-D grids are not supported\n");
+    return nullptr;
+  }
+  if (min_substeps < 1 || max_substeps < min_substeps) {
+    fprintf(stderr, "libdft: dft_ot_alloc: bad substep range\n");
+    return nullptr;
+  }
+  auto *otf = (dft_ot_functional *)calloc(1, sizeof(dft_ot_functional));
+  if (!otf) {
+    fprintf(stderr, "libdft: Error in dft_ot_alloc(): Could not allocate memory for dft_ot_functional.\n");
+    return nullptr;
+  }
+  otf->model = model;
+  fprintf(stderr, "libdft: Grid %ld x %ld x %ld with step %le Bohr (%s).\n", (long)g->nx, (long)g->ny, (long)g->nz,
+          g->step, g->prec == GRID_FP32 ? "FP32" : "FP64");
+  fprintf(stderr, "libdft: Functional: %ld.\n", (long)model);
+  dft_ot_init_params(otf, model);
+
+  auto *d = new dft_ot_device();
+  memset(d, 0, sizeof(*d));
+  otf->device = d;
+  d->prec = g->prec;
+  d->nx = (int)g->nx;
+  d->ny = (int)g->ny;
+  d->nz = (int)g->nz;
+  d->P = d->nz / 2 + 1;
+  d->n = (int64_t)d->nx * d->ny * d->nz;
+  d->step = g->step;
+  d->slot_elems = (int64_t)d->nx * d->ny * 2 * d->P;
+  const size_t es = g->prec == GRID_FP32 ? sizeof(float) : sizeof(double);
+  const int64_t nh = (int64_t)d->nx * d->ny * d->P;
+  bool ok = true;
+  for (int i = 0; i < NSLOT && ok; ++i) ok = (d->slot[i] = dmalloc(es * d->slot_elems)) != nullptr;
+  for (int i = 0; i < 4 && ok; ++i) ok = (d->kern[i] = dmalloc(es * nh)) != nullptr;
+  if (ok) {
+    if (d->prec == GRID_FP32) {
+      auto *p = new Plan3D<float>();
+      p->init(d->nx, d->ny, d->nz, d->step);
+      d->plan = p;
+      ok = build_kernels<float>(otf, (int)min_substeps, (int)max_substeps);
+    } else {
+      auto *p = new Plan3D<double>();
+      p->init(d->nx, d->ny, d->nz, d->step);
+      d->plan = p;
+      ok = build_kernels<double>(otf, (int)min_substeps, (int)max_substeps);
+    }
+  }
+  if (!ok) {
+    fprintf(stderr, "libdft: Error in dft_ot_alloc(): device allocation failed.\n");
+    free_device(d);
+    free(otf);
+    return nullptr;
+  }
+  otf->lennard_jones = spectral_handle(otf, K_LJ, "OT Lennard-Jones");
+  otf->spherical_avg = spectral_handle(otf, K_SA, "OT Sph. average");
+  if (model & DFT_OT_KC) otf->gaussian_tf = spectral_handle(otf, K_F, "OT KC Gauss TF");
+  if (model & DFT_OT_BACKFLOW) otf->backflow_pot = spectral_handle(otf, K_U, "OT Backflow");
+  // x/y/z derivatives of F are applied in k space (i sin(k h)/h F^), not stored
+  otf->gaussian_x_tf = otf->gaussian_y_tf = otf->gaussian_z_tf = nullptr;
+
+  int save = grid_get_precision();
+  grid_set_precision(d->prec);
+  char *nm = (char *)"OT Density";
+  otf->density = rgrid_alloc(g->nx, g->ny, g->nz, g->step, RGRID_PERIODIC_BOUNDARY, nullptr, nm);
+  rgrid **ws[9] = {&otf->workspace1, &otf->workspace2, &otf->workspace3, &otf->workspace4, &otf->workspace5,
+                   &otf->workspace6, &otf->workspace7, &otf->workspace8, &otf->workspace9};
+  int nws = 3;
+  if (model & (DFT_OT_KC | DFT_OT_BACKFLOW)) nws = 6;
+  if (model & DFT_OT_BACKFLOW) nws = 9;
+  for (int i = 0; i < nws; ++i) {
+    char id[32];
+    snprintf(id, sizeof(id), "OT Workspace %d", i + 1);
+    *ws[i] = rgrid_alloc(g->nx, g->ny, g->nz, g->step, RGRID_PERIODIC_BOUNDARY, nullptr, id);
+  }
+  grid_set_precision(save);
+  return otf;
+}
+
+// src/ot.c:268-290
+void dft_ot_free(dft_ot_functional *otf) {
+  if (!otf) return;
+  rgrid **ws[10] = {&otf->workspace1, &otf->workspace2, &otf->workspace3, &otf->workspace4, &otf->workspace5,
+                    &otf->workspace6, &otf->workspace7, &otf->workspace8, &otf->workspace9, &otf->density};
+  for (auto w : ws)
+    if (*w) rgrid_free(*w);
+  for (rgrid *h : {otf->lennard_jones, otf->spherical_avg, otf->gaussian_tf, otf->backflow_pot}) free(h);
+  free_device(otf->device);
+  free(otf);
+}
+
+void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact) { otf->kc_coef23 = exact ? 1.0 / otf->rho_0s : 1.0; }
+
+// src/ot.c:314-408
+void dft_ot_potential(dft_ot_functional *otf, cgrid *potential, wf *gwf) {
+  check_wf(otf, gwf, "dft_ot_potential");
+  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
+  DFT_BY_PREC(otf->device->prec,
+              potential_impl<T>(otf, potential, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev,
+                                nullptr, nullptr, nullptr, parts, false));
+}
+
+// src/ot.c:431-510
+void dft_ot_potential2(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *vx, rgrid *vy, rgrid *vz) {
+  check_r(otf, density, "dft_ot_potential2");
+  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
+  if (parts.bf)
+    for (rgrid *v : {vx, vy, vz}) check_r(otf, v, "dft_ot_potential2");
+  DFT_BY_PREC(otf->device->prec,
+              potential_impl<T>(otf, potential, nullptr, 0.0, (T *)density->dev, parts.bf ? (T *)vx->dev : nullptr,
+                                parts.bf ? (T *)vy->dev : nullptr, parts.bf ? (T *)vz->dev : nullptr, parts, true));
+}
+
+// src/ot.c:840-991 (the workspace arguments are accepted and unused)
+void dft_ot_backflow_potential(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *vx, rgrid *vy,
+                               rgrid *vz, rgrid *, rgrid *, rgrid *, rgrid *, rgrid *, rgrid *) {
+  if (!(otf->model & DFT_OT_BACKFLOW)) {
+    fprintf(stderr, "libdft: dft_ot_backflow_potential: functional allocated without DFT_OT_BACKFLOW\n");
+    abort();
+  }
+  for (rgrid *v : {density, vx, vy, vz}) check_r(otf, v, "dft_ot_backflow_potential");
+  PotParts parts{false, false, true};
+  DFT_BY_PREC(otf->device->prec, potential_impl<T>(otf, potential, nullptr, 0.0, (T *)density->dev, (T *)vx->dev,
+                                                   (T *)vy->dev, (T *)vz->dev, parts, true));
+}
+
+// src/ot-energy.c:38-110
+void dft_ot_energy_density(dft_ot_functional *otf, rgrid *energy_density, wf *gwf) {
+  check_wf(otf, gwf, "dft_ot_energy_density");
+  check_r(otf, energy_density, "dft_ot_energy_density");
+  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
+  DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)energy_density->dev, (const cplx<T> *)gwf->grid->dev,
+                                                gwf->mass, (T *)otf->density->dev, false, parts, false));
+}
+
+// src/ot-energy.c:123-191 (adds the KC part)
+void dft_ot_energy_density_kc(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density) {
+  check_r(otf, energy_density, "dft_ot_energy_density_kc");
+  check_r(otf, density, "dft_ot_energy_density_kc");
+  PotParts parts{false, true, false};
+  DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)energy_density->dev, nullptr, gwf ? gwf->mass : 0.0,
+                                                (T *)density->dev, true, parts, true));
+}
+
+// src/ot-energy.c:202-288 (adds the BF part)
+void dft_ot_energy_density_bf(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density) {
+  check_wf(otf, gwf, "dft_ot_energy_density_bf");
+  check_r(otf, energy_density, "dft_ot_energy_density_bf");
+  check_r(otf, density, "dft_ot_energy_density_bf");
+  PotParts parts{false, false, true};
+  DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)energy_density->dev, (const cplx<T> *)gwf->grid->dev,
+                                                gwf->mass, (T *)density->dev, true, parts, true));
+}
+
+REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf) {
+  dft_ot_energy_density(otf, otf->workspace1, gwf);
+  return rgrid_integral(otf->workspace1);
+}
+
+}  // extern "C"

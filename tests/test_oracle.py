@@ -1,0 +1,245 @@
+"""Pins the FP64 CPU oracle (no GPU needed).
+
+Checks used: the reference's own data fixture (equation of state), exact
+invariants (uniform density), an independent numpy restatement, central
+finite differences of the energy, closed-form kernel transforms, linear
+response, and closed forms of the split-operator step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests import inputs
+from tests.np_ref import NumpyOT
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _ang(step_ang, po):
+    return step_ang / po.AUTOANG
+
+
+# ------------------------------------------------------------------ FFT --
+def test_fft_matches_numpy(oracle):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((8, 16, 32)) + 1j * rng.standard_normal((8, 16, 32))
+    np.testing.assert_allclose(oracle.fft3d(a, -1), np.fft.fftn(a), atol=1e-11)
+    np.testing.assert_allclose(oracle.fft3d(a, +1), np.fft.ifftn(a) * a.size, atol=1e-11)
+
+
+# ------------------------------------------------- bulk closed forms (a13) --
+def _eos_rows():
+    rows = []
+    with open(os.path.join(GOLDEN, "equation_of_state.dat")) as f:
+        for line in f:
+            t = line.replace("\t", " ").split()
+            if t and t[0] == "Pressure:":
+                rows.append((float(t[1]), float(t[4]), float(t[7])))
+    return rows
+
+
+def test_equation_of_state_fixture(oracle):
+    """examples/bulk/equation_of_state.dat (printed by examples/bulk/bulk_properties.c:54-61)."""
+    po = oracle
+    o = po.OracleOT(po.PLAIN, (8, 8, 8), 1.0, 1, 1)
+    rows = _eos_rows()
+    assert len(rows) == 241
+    for P, rho_ref, mu_ref in rows:
+        p_au = P / po.AUTOBAR
+        rho = o.bulk("density_pressurized", p_au) / po.AUTOANG ** 3
+        mu = o.bulk("chempot_pressurized", p_au) * po.AUTOK
+        # density printed with 6 decimals
+        assert abs(rho - rho_ref) < 1e-6, (P, rho, rho_ref)
+        # mu: the Newton stop |P(rho) - P| < 1e-12 au (src/helium-ot-bulk.c:204) alone allows
+        # 1e-12 / rho ~ 1e-4 K, and libgrid's bar constant drifts ~6e-6 K per bar from CODATA
+        assert abs(mu - mu_ref) < 1.2e-4 + 6e-6 * P, (P, mu, mu_ref)
+
+
+def test_bulk_relations(oracle):
+    po = oracle
+    o = po.OracleOT(po.PLAIN, (8, 8, 8), 1.0, 1, 1)
+    rho = o.p.rho0
+    # P = rho dE/drho - E, and dP/drho = rho d2E/drho2 (checked by FD of dEdRho)
+    e = 1e-6 * rho
+    d2 = (o.bulk("dEdRho", rho + e) - o.bulk("dEdRho", rho - e)) / (2 * e)
+    assert abs(o.bulk("dPdRho", rho) - rho * d2) < 1e-6 * abs(rho * d2)
+    # zero-pressure density is a minimum of E/rho
+    r0 = o.bulk("density_pressurized", 0.0)
+    f = lambda r: o.bulk("energy", r) / r
+    assert f(r0) < f(r0 * 1.01) and f(r0) < f(r0 * 0.99)
+
+
+# -------------------------------------------------- uniform invariant (a3) --
+@pytest.mark.parametrize("model", [0, 1, 8, 9])
+def test_uniform_density_invariant(oracle, model):
+    po = oracle
+    shape = (16, 16, 16)
+    o = po.OracleOT(model, shape, _ang(0.5, po), 2, 8)
+    rho0 = o.p.rho0
+    psi = np.full(shape, np.sqrt(rho0), np.complex128)
+    V = o.potential(psi)
+    mu = o.bulk("dEdRho", rho0)
+    assert np.abs(V - mu).max() < 1e-12 * abs(mu)
+    eps = o.energy_density(psi)
+    eb = o.bulk("energy", rho0)
+    assert np.abs(eps - eb).max() < 1e-12 * abs(eb)
+
+
+# ------------------------------------------- independent numpy restatement --
+@pytest.mark.parametrize("model", [0, 1, 8, 9])
+def test_matches_numpy_restatement(oracle, model):
+    po = oracle
+    shape, step = (16, 16, 16), _ang(0.6, po)
+    o = po.OracleOT(model, shape, step, 2, 8)
+    ref = NumpyOT(shape, step, kc=bool(model & 1), bf=bool(model & 8), min_sub=2, max_sub=8)
+    psi = inputs.smooth_modulated(shape, step, o.p.rho0, 0.3)
+    V, Vn = o.potential(psi), ref.potential(psi, po.HELIUM_MASS)
+    assert np.abs(V - Vn).max() < 1e-12 * np.abs(Vn).max()
+    e, en = o.energy_density(psi), ref.energy_density(psi, po.HELIUM_MASS)
+    assert np.abs(e - en).max() < 1e-12 * np.abs(en).max()
+
+
+def test_potential2_matches_potential(oracle):
+    """dft_ot_potential2 with the wave function's density and velocity equals dft_ot_potential."""
+    po = oracle
+    shape, step = (16, 16, 16), _ang(0.6, po)
+    o = po.OracleOT(9, shape, step, 2, 8)
+    psi = inputs.smooth_modulated(shape, step, o.p.rho0, 0.3)
+    v = po.velocity(psi, step, po.HELIUM_MASS, 1e-4)
+    V2 = o.potential2(np.abs(psi) ** 2, *v)
+    V = o.potential(psi)
+    assert np.abs(V - V2).max() < 1e-14 * np.abs(V).max()
+
+
+# --------------------------------------------- functional derivative (FD) --
+def _fd_check(o, psi, step, seed=7):
+    rho = np.abs(psi) ** 2
+    eta = 0.1 * o.p.rho0 * inputs.splitmix64_uniform(rho.size, seed).reshape(rho.shape)
+    V = o.potential(np.sqrt(rho).astype(complex)).real
+    h3 = step ** 3
+    E = lambda r: o.energy_density(np.sqrt(r).astype(complex)).sum() * h3
+    t = 1e-3
+    fd = (E(rho + t * eta) - E(rho - t * eta)) / (2 * t)
+    an = (V * eta).sum() * h3
+    return fd, an
+
+
+def test_plain_potential_is_energy_derivative(oracle):
+    po = oracle
+    shape, step = (16, 16, 16), _ang(0.6, po)
+    o = po.OracleOT(0, shape, step, 2, 8)
+    fd, an = _fd_check(o, inputs.smooth_modulated(shape, step, o.p.rho0, 0.3), step)
+    assert abs(fd - an) < 1e-8 * abs(fd)
+
+
+def test_kc_reference_potential_is_not_exact_derivative(oracle):
+    """The reference KC potential (src/ot.c:660-675) differs from dE/drho of
+    src/ot-energy.c:123-191: terms 2 and 3 lack 1/rho_0s."""
+    po = oracle
+    shape, step = (16, 16, 16), _ang(0.6, po)
+    o = po.OracleOT(1, shape, step, 2, 8)
+    psi = inputs.smooth_modulated(shape, step, o.p.rho0, 0.3)
+    fd, an = _fd_check(o, psi, step)
+    assert abs(fd - an) > 1e-4 * abs(fd)
+
+
+# ------------------------------------------------- kernels vs closed forms --
+def test_kernel_spectra_closed_forms(oracle):
+    """Spectra at small k against the continuum transforms
+    (src/common.c:173-179, src/helium-ot-bulk.c:390-409, gaussian e^{-k^2 l^2/4})."""
+    po = oracle
+    n, step = 32, _ang(0.4, po)
+    o = po.OracleOT(9, (n, n, n), step, 4, 32)
+    h3 = step ** 3
+    L = n * step
+    sgn = lambda m: (-1.0) ** m   # centred sampling: spectra carry (-1)^(i+j+k)
+    for m in (1, 2, 3):
+        k = 2 * np.pi * m / L
+        # averaging over a cell of width h multiplies the transform by sinc(k h / 2)
+        cell = np.sin(k * step / 2) / (k * step / 2)
+        pi_num = o.spectrum("sa_tf")[m, 0, 0].real * sgn(m) * h3
+        kh = k * o.p.lj_h
+        pi_cf = 3 * (np.sin(kh) - kh * np.cos(kh)) / kh ** 3 * cell
+        assert abs(pi_num - pi_cf) < 1e-3, (m, pi_num, pi_cf)
+        g_num = o.spectrum("g_tf")[m, 0, 0].real * sgn(m) * h3
+        assert abs(g_num - np.exp(-k * k * o.p.l_g ** 2 / 4) * cell) < 1e-3
+        # derivative kernel: FFT of the central difference of F = F^ i sin(kh)/h
+        gx = o.spectrum("gx_tf")[m, 0, 0] * sgn(m)
+        ratio = gx / (o.spectrum("g_tf")[m, 0, 0] * sgn(m))
+        assert abs(ratio - 1j * np.sin(k * step) / step) < 1e-9 * abs(ratio)
+
+
+# --------------------------------------------------------- linear response --
+def test_linear_response_plain(oracle):
+    """dV_k/drho_k = (ikai(k) - k^2/4m)/rho0 from dft_ot_bulk_istatic (src/helium-ot-bulk.c:445-463)."""
+    po = oracle
+    n, step = 32, _ang(0.4, po)
+    o = po.OracleOT(0, (n, n, n), step, 4, 32)
+    rho0 = o.p.rho0
+    L = n * step
+    x = np.arange(n) * step
+    for m in (1, 2, 3, 4):
+        k = 2 * np.pi * m / L
+        d = 1e-6 * rho0
+        rho = rho0 + d * np.cos(k * x)[:, None, None] * np.ones((n, n, n))
+        V = o.potential(np.sqrt(rho).astype(complex)).real
+        amp = 2 * np.fft.fft(V[:, 0, 0])[m].real / n / d
+        # continuum transforms: the LJ / c2 / c3 terms nearly cancel, so the 12.8 A box
+        # (LJ tail cut at L/2) and cell averaging leave a few % here
+        pred = (o.bulk("istatic", k, rho0) - k * k / (4 * o.p.mass)) / rho0
+        assert abs(amp - pred) < 0.1 * abs(pred), (m, amp, pred)
+        # the response changes sign with a wrong kernel centring ((-1)^m)
+        assert np.sign(amp) == np.sign(pred)
+
+
+# ------------------------------------------------ split operator closed forms --
+def test_free_plane_wave_phase(oracle):
+    po = oracle
+    shape, step = (8, 16, 32), 0.7
+    x, y, z = np.meshgrid(*[np.arange(s) * step for s in shape], indexing="ij")
+    kv = [2 * np.pi * 2 / (shape[0] * step), 2 * np.pi * -3 / (shape[1] * step), 2 * np.pi * 5 / (shape[2] * step)]
+    psi = np.exp(1j * (kv[0] * x + kv[1] * y + kv[2] * z))
+    dt = 40.0 - 3.0j
+    out = po.propagate(psi, np.zeros(shape, complex), dt, step)
+    k2 = sum(k * k for k in kv)
+    np.testing.assert_allclose(out, psi * np.exp(-1j * k2 / (2 * po.HELIUM_MASS) * dt), rtol=1e-12, atol=1e-12)
+
+
+def test_norm_conservation_real_potential(oracle):
+    po = oracle
+    shape, step = (16, 16, 16), 0.8
+    psi = inputs.smooth_modulated(shape, step, 1.0, 0.5)
+    V = inputs.splitmix64_uniform(psi.size, 3).reshape(shape) * 1e-3
+    out = po.propagate(psi, V.astype(complex), 200.0, step)
+    assert abs(po.norm(out, step) / po.norm(psi, step) - 1) < 1e-13
+
+
+def test_imaginary_time_converges_to_bulk(oracle):
+    """Config-1 pattern at 16^3: noisy bulk relaxes to E/N = E_bulk(rho)/rho, mu = dE/drho."""
+    po = oracle
+    shape, step = (16, 16, 16), _ang(0.5, po)
+    o = po.OracleOT(0, shape, step, 2, 8)
+    rho = 0.0218 * po.AUTOANG ** 3
+    psi = inputs.noisy_bulk(shape, rho, 0.01)
+    n0 = po.norm(psi, step)
+    dt = -1j * 10.0 / po.AUTOFS
+    for _ in range(40):
+        o.pc_step(psi, None, 0.0, dt)
+        psi *= np.sqrt(n0 / po.norm(psi, step))
+    obs = o.observables(psi)
+    rb = n0 / (np.prod(shape) * step ** 3)
+    # the 1 % noise relaxes to within ~1e-6; after that a slow finite-k mode of the
+    # plain functional on this grid lowers E/N by ~1e-7 per 20 steps
+    assert abs(obs["energy"] / obs["norm"] - o.bulk("energy", rb) / rb) < 1e-6 * abs(o.bulk("energy", rb) / rb)
+    assert abs(obs["mu"] - o.bulk("dEdRho", rb)) < 5e-6 * abs(o.bulk("dEdRho", rb))
+
+
+def test_kinetic_energy_plane_wave(oracle):
+    po = oracle
+    shape, step = (8, 8, 16), 0.5
+    z = np.arange(16) * step
+    k = 2 * np.pi * 3 / (16 * step)
+    psi = np.exp(1j * k * z)[None, None, :] * np.ones(shape)
+    ek = po.kinetic_energy(psi, step)
+    assert abs(ek - k * k / (2 * po.HELIUM_MASS) * po.norm(psi, step)) < 1e-12 * ek
