@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Orsay-Trento split-operator steps/s at 256^3 on one MI355X.
+
+Workload: full OT (PLAIN|KC|BACKFLOW), periodic 256^3 grid, h = 0.3 A, bulk
+rho0 with a straight vortex line (phase atan2(y, x) along z) and 1e-3 seeded
+density noise, real-time step dt = 2 fs.  One step is the reference driver's
+single-step pattern (doc/libdft-manual.tex:496-504): zero the potential,
+dft_ot_potential, add -mu, grid_wf_propagate.  Fields are FP32 on the device
+with FP64 reductions; the FP64 device path runs the same steps for the
+precision comparison.  The CPU baseline is the FP64 oracle (OpenMP on the host
+cores) timed on one step of the same workload.
+
+Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import libdft_amd as L  # noqa: E402
+from tests import inputs  # noqa: E402
+
+METRIC = "OT split-op steps/s at 256³ (FP32 vs FP64 GPU, ≤1e-6 rel energy) + HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def profile_table():
+    lib = L.lib()
+    lib.dft_profile_collect.restype = C.c_int
+    lib.dft_profile_entry.argtypes = [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    n = lib.dft_profile_collect()
+    rows = []
+    for i in range(n):
+        name = C.create_string_buffer(128)
+        cnt, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        lib.dft_profile_entry(i, name, 128, C.byref(cnt), C.byref(ms), C.byref(by))
+        rows.append({"kernel": name.value.decode(), "launches": cnt.value, "total_ms": ms.value,
+                     "avg_us": 1e3 * ms.value / max(cnt.value, 1), "bytes_per_launch": by.value / max(cnt.value, 1),
+                     "GBps": by.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0})
+    rows.sort(key=lambda r: -r["total_ms"])
+    return rows
+
+
+class Run:
+    """One device precision of the workload."""
+
+    def __init__(self, n, step, prec, psi0, model=9):
+        self.lib = L.lib()
+        self.w = L.Wave((n, n, n), step, prec)
+        self.w.set(psi0)
+        self.f = L.Functional(model, self.w)
+        self.pot = L.CField((n, n, n), step, prec)
+        self.mu = self.f.bulk("dEdRho", self.f.p.rho0)
+        self.n, self.step = n, step
+
+    def steps(self, k, dt):
+        lib, z = self.lib, L.Complex.of(dt)
+        mmu = L.Complex.of(-self.mu)
+        for _ in range(k):
+            lib.cgrid_zero(self.pot.h)
+            lib.dft_ot_potential(self.f.h, self.pot.h, self.w.h)
+            lib.cgrid_add(self.pot.h, mmu)
+            lib.grid_wf_propagate(self.w.h, self.pot.h, z)
+
+    def timed(self, k, dt):
+        self.lib.grid_synchronize()
+        t0 = time.perf_counter()
+        self.steps(k, dt)
+        self.lib.grid_synchronize()
+        return (time.perf_counter() - t0) / k
+
+    def energy(self):
+        """Total energy (kinetic + OT) and norm, FP64 reductions."""
+        ekin = self.lib.grid_wf_energy(self.w.h, None)
+        eot = self.f.energy()
+        return ekin + eot, self.lib.grid_wf_norm(self.w.h)
+
+    def free(self):
+        self.pot.free()
+        self.f.free()
+        self.w.free()
+
+
+def cpu_baseline(n, step, psi0, dt):
+    """One step of the same workload on the FP64 oracle (OpenMP over the host cores)."""
+    from oracle import pyoracle as po
+    o = po.OracleOT(9, (n, n, n), step, 1, 1)   # kernel sampling cost is setup, not part of a step
+    psi = psi0.copy()
+    mu = o.bulk("dEdRho", o.p.rho0)
+    t0 = time.perf_counter()
+    o.single_step(psi, None, mu, dt)
+    sec = time.perf_counter() - t0
+    o.close()
+    return {"value": 1.0 / sec, "unit": "steps/s", "cores": po.num_threads(), "kind": "port",
+            "sample": "1 full-OT single step of this workload (%d^3, FP64) on the C/OpenMP oracle; "
+                      "kernels sampled once per cell (setup cost excluded, step cost unchanged); %.1f s"
+                      % (n, sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    n = args.grid
+    step = 0.3 / L.AUTOANG
+    dt = 2.0 / L.AUTOFS
+    rho0 = 0.0218360 * L.AUTOANG ** 3
+    psi0 = inputs.vortex_line((n, n, n), step, rho0, noise=1e-3)
+    lib = L.lib()
+
+    # FP32 fields, FP64 reductions: the measured path
+    log("fp32 setup %d^3" % n)
+    r32 = Run(n, step, L.GRID_FP32, psi0)
+    log("fp32 warmup + %d timed steps" % args.steps)
+    r32.steps(args.warmup, dt)
+    t32 = r32.timed(args.steps, dt)
+    # same launches again with per-launch HIP events, for the per-kernel roofline
+    lib.dft_profile_reset()
+    lib.dft_profile_enable(1)
+    tprof = r32.timed(args.profile_steps, dt)
+    lib.dft_profile_enable(0)
+    rows = profile_table()
+    nsteps32 = args.warmup + args.steps + args.profile_steps
+    e32, n32 = r32.energy()
+    r32.free()
+
+    out = {"metric": METRIC, "value": 1.0 / t32, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * t32, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "full OT (KC+BF) real-time Strang step, vortex line in bulk He, %d^3, "
+                                  "h=0.3 A, dt=2 fs, FP32 fields + FP64 reductions" % n,
+                      "grid": [n, n, n], "model": "PLAIN|KC|BACKFLOW", "step_kind": "single Strang step"}}
+
+    # roofline of the dominant kernel (largest total time in the profiled steps)
+    dom = rows[0]
+    out["roofline"] = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                       "avg_us": dom["avg_us"], "bytes_per_launch": dom["bytes_per_launch"]}
+    # whole-step roofline against the canonical 134 R per full-OT step (R = 4N bytes)
+    step_bytes = 134 * 4.0 * n ** 3
+    out["step_roofline"] = {"algorithmic_bytes": step_bytes, "achieved": step_bytes / t32 / 1e9,
+                            "frac": step_bytes / t32 / 1e9 / HBM_PEAK_GBS, "unit": "GB/s",
+                            "moved_bytes_per_step": sum(r["bytes_per_launch"] * r["launches"] for r in rows)
+                            / args.profile_steps}
+    out["profiled_ms_per_step"] = 1e3 * tprof
+    out["kernels"] = [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows[:12]]
+
+    if not args.no_fp64:
+        log("fp64 setup and %d timed steps" % args.steps)
+        r64 = Run(n, step, L.GRID_FP64, psi0)
+        r64.steps(args.warmup, dt)
+        t64 = r64.timed(args.steps, dt)
+        r64.steps(args.profile_steps, dt)
+        e64, n64 = r64.energy()
+        r64.free()
+        out["fp64_steps_per_s"] = 1.0 / t64
+        out["fp32_over_fp64_speedup"] = t64 / t32
+        out["energy_rel_err_fp32_vs_fp64"] = abs(e32 - e64) / abs(e64)
+        out["norm_rel_err_fp32_vs_fp64"] = abs(n32 - n64) / abs(n64)
+        out["energy_K"] = e64 * L.AUTOK
+        out["steps_compared"] = nsteps32
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, one step)")
+        out["cpu_baseline"] = cpu_baseline(n, step, psi0, dt)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

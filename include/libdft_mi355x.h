@@ -268,6 +268,24 @@ REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z);
 void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact);
 /* Energy of the functional, h^3 sum eps_OT, in FP64 (kinetic part excluded). */
 REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf);
+/* Hand-written backward of E[rho] = h^3 sum eps (LJ, c2/c3, KC parts of
+   src/ot-energy.c:38-191): grad = dE/drho, the exact derivative of the
+   discrete energy.  For PLAIN it equals dft_ot_potential's output; for KC it
+   differs from it (src/ot.c:660-675 omits 1/rho_0s on terms 2 and 3). */
+void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density);
+/* E[rho] of the same parts, FP64 reduction. */
+REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density);
+
+/* ---- measurement (extensions; no counterpart in the reference) ---- */
+/* Per-launch HIP-event timing of the library's kernels on its stream. */
+void dft_profile_enable(int on);
+void dft_profile_reset(void);
+/* Aggregates recorded launches per kernel name; returns the number of names. */
+int dft_profile_collect(void);
+/* Entry i: name, launches, total milliseconds, algorithmic bytes (inputs + outputs once). */
+int dft_profile_entry(int i, char *name, int len, INT *launches, REAL *ms, REAL *bytes);
+/* Plain device copy between two grids (traffic-counter calibration). */
+void grid_bench_copy(rgrid *dst, rgrid *src);
 
 /* Atomic-unit conversions (CODATA 2018; libgrid's au.h). */
 #define GRID_AUTOANG 0.529177210903

@@ -113,6 +113,9 @@ _SIGS = {
     "dft_ot_energy_density_kc": (None, ["otf", "rg", "wf", "rg"]),
     "dft_ot_energy_density_bf": (None, ["otf", "rg", "wf", "rg"]),
     "dft_ot_set_kc_mode": (None, ["otf", "i"]), "dft_ot_energy": ("d", ["otf", "wf"]),
+    "dft_ot_energy_gradient": (None, ["otf", "rg", "rg"]), "dft_ot_energy_rho": ("d", ["otf", "rg"]),
+    "dft_profile_enable": (None, ["i"]), "dft_profile_reset": (None, []), "dft_profile_collect": ("i", []),
+    "grid_bench_copy": (None, ["rg", "rg"]),
 }
 for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "chempot_pressurized",
            "compressibility", "sound_speed", "chempot3"):

@@ -20,6 +20,20 @@
 #define DFT_LAUNCH_CHECK() DFT_HIP_CHECK(hipGetLastError())
 
 namespace dft {
+// Per-launch timing with HIP events on the launch stream (off unless enabled
+// through dft_profile_enable).  `bytes` is the launch's algorithmic traffic:
+// every input and output field counted once.
+extern bool g_prof_on;
+void prof_begin(hipStream_t s);
+void prof_end(hipStream_t s, const char *name, double bytes);
+}  // namespace dft
+
+#define DFT_PROF_BEGIN(s) \
+  if (dft::g_prof_on) dft::prof_begin(s)
+#define DFT_PROF_END(s, name, bytes) \
+  if (dft::g_prof_on) dft::prof_end(s, name, bytes)
+
+namespace dft {
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<float> { using type = float2; };

@@ -79,8 +79,10 @@ void launch_zr2c(const Plan3D<T> &p, const fft::Batch &b, int nb, hipStream_t s)
   using TL = fft::Tile<T, N>;
   const int64_t npairs = (int64_t)p.nx * p.ny / 2;
   dim3 grid((unsigned)((npairs + TL::TILE - 1) / TL::TILE), nb);
+  DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_zr2c<T, N>), grid, dim3(TL::THREADS), 0, s, b, npairs, p.P, p.twz);
   DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "fft_z_r2c", (double)nb * (p.n() * sizeof(T) + p.nhalf() * sizeof(cplx<T>)));
 }
 
 template <typename T, int N>
@@ -88,8 +90,10 @@ void launch_zc2r(const Plan3D<T> &p, const fft::Batch &b, int nb, T scale, hipSt
   using TL = fft::Tile<T, N, 2>;
   const int64_t npairs = (int64_t)p.nx * p.ny / 2;
   dim3 grid((unsigned)((npairs + TL::TILE - 1) / TL::TILE), nb);
+  DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_zc2r<T, N>), grid, dim3(TL::THREADS), 0, s, b, npairs, p.P, p.twz, scale);
   DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "fft_z_c2r", (double)nb * (p.n() * sizeof(T) + p.nhalf() * sizeof(cplx<T>)));
 }
 
 // Strided pass along y (axis 1) or x (axis 0) over columns [0, cols).
@@ -110,8 +114,11 @@ void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, i
   }
   dim3 grid((unsigned)(outer * ntile), nb);
   const cplx<T> *tw = axis == 1 ? p.twy : p.twx;
+  DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_strided<T, N, DIR>), grid, dim3(TL::THREADS), 0, s, b, cols, es, os, ntile, tw, scale);
   DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, axis == 1 ? "fft_y_c2c" : "fft_x_c2c",
+               2.0 * nb * (double)p.nx * p.ny * cols * sizeof(cplx<T>));
 }
 
 template <typename T, int N>
@@ -120,9 +127,18 @@ void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::Combine
   using TL = fft::Tile<T, N>;
   const int ntile = (p.P + TL::TILE - 1) / TL::TILE;
   dim3 grid((unsigned)(p.ny * ntile), nb);
+  // traffic: each term's spectrum and kernel, plus the output
+  double per = 0;
+  for (int i = 0; i < nb; ++i) {
+    per += sizeof(cplx<T>);
+    for (int t = 0; t < cs.c[i].nterms; ++t)
+      per += sizeof(cplx<T>) + (cs.c[i].t[t].kern >= 0 ? sizeof(T) : 0);
+  }
+  DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_xcombine_inv<T, N>), grid, dim3(TL::THREADS), 0, s, b, cs, kt, p.ny, p.P, ntile,
                      p.twx);
   DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "fft_x_combine_c2c", per * (double)p.nhalf());
 }
 
 template <typename T, int N, int DIR>
@@ -131,9 +147,12 @@ void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cpl
   using TL = fft::Tile<T, N>;
   const int64_t nlines = (int64_t)p.nx * p.ny;
   dim3 grid((unsigned)((nlines + TL::TILE - 1) / TL::TILE));
+  DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, DIR>), grid, dim3(TL::THREADS), 0, s, src, dst, pot, nlines, p.twz, dtr,
                      dti, scale, partial);
   DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, DIR < 0 ? "fft_z_c2c_vphase_fwd" : "fft_z_c2c_vphase_inv",
+               (double)p.n() * sizeof(cplx<T>) * (pot ? 3.0 : 2.0));
 }
 
 template <typename T, int N>
@@ -141,9 +160,11 @@ void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dt
   using TL = fft::Tile<T, N>;
   const int ntile = (p.nz + TL::TILE - 1) / TL::TILE;
   dim3 grid((unsigned)(p.ny * ntile));
+  DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_xkin<T, N>), grid, dim3(TL::THREADS), 0, s, psi, p.ny, p.nz, ntile, p.twx, cx, cy, cz,
                      dtr, dti);
   DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "fft_x_kinetic_c2c", 2.0 * (double)p.n() * sizeof(cplx<T>));
 }
 
 template <typename T> int zc2c_tile(int nz) {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -35,6 +36,44 @@ void *dmalloc(size_t bytes) {
 
 void dfree(void *p) {
   if (p) DFT_HIP_CHECK(hipFree(p));
+}
+
+// ------------------------------------------------------------- profiling --
+bool g_prof_on = false;
+struct ProfRec {
+  const char *name;
+  hipEvent_t a, b;
+  double bytes;
+};
+struct ProfAgg {
+  std::string name;
+  int64_t launches;
+  double ms, bytes;
+};
+static std::vector<ProfRec> g_prof;
+static std::vector<ProfAgg> g_agg;
+static std::vector<hipEvent_t> g_evpool;
+static size_t g_evnext = 0;
+static hipEvent_t g_pending = nullptr;
+
+static hipEvent_t ev_get() {
+  if (g_evnext == g_evpool.size()) {
+    hipEvent_t e;
+    DFT_HIP_CHECK(hipEventCreate(&e));
+    g_evpool.push_back(e);
+  }
+  return g_evpool[g_evnext++];
+}
+
+void prof_begin(hipStream_t s) {
+  g_pending = ev_get();
+  DFT_HIP_CHECK(hipEventRecord(g_pending, s));
+}
+
+void prof_end(hipStream_t s, const char *name, double bytes) {
+  hipEvent_t b = ev_get();
+  DFT_HIP_CHECK(hipEventRecord(b, s));
+  g_prof.push_back({name, g_pending, b, bytes});
 }
 
 // --------------------------------------------------------------- kernels --
@@ -736,6 +775,65 @@ void grid_wf_propagate_predict(wf *w, wf *wp, cgrid *potential, REAL_COMPLEX tim
 }
 
 void grid_wf_propagate_correct(wf *w, cgrid *potential, REAL_COMPLEX time) { grid_wf_propagate(w, potential, time); }
+
+// ------------------------------------------------------------- profiling --
+// Per-launch HIP-event timing of the library's kernels (all on one stream).
+void dft_profile_enable(int on) {
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  g_prof_on = on != 0;
+}
+
+void dft_profile_reset(void) {
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  g_prof.clear();
+  g_agg.clear();
+  g_evnext = 0;
+}
+
+// Folds the recorded launches into per-kernel totals; returns the number of kernels.
+int dft_profile_collect(void) {
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  for (const ProfRec &r : g_prof) {
+    float ms = 0.f;
+    DFT_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    ProfAgg *a = nullptr;
+    for (ProfAgg &x : g_agg)
+      if (x.name == r.name) a = &x;
+    if (!a) {
+      g_agg.push_back({r.name, 0, 0.0, 0.0});
+      a = &g_agg.back();
+    }
+    a->launches += 1;
+    a->ms += ms;
+    a->bytes += r.bytes;
+  }
+  g_prof.clear();
+  g_evnext = 0;
+  return (int)g_agg.size();
+}
+
+int dft_profile_entry(int i, char *name, int len, int64_t *launches, double *ms, double *bytes) {
+  if (i < 0 || i >= (int)g_agg.size()) return -1;
+  snprintf(name, len, "%s", g_agg[i].name.c_str());
+  *launches = g_agg[i].launches;
+  *ms = g_agg[i].ms;
+  *bytes = g_agg[i].bytes;
+  return 0;
+}
+
+// Plain copy of n doubles (device buffers of the library's grids), for
+// calibrating hardware traffic counters against a kernel of known traffic.
+static __global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
+  GRID_STRIDE(i, n) b[i] = a[i];
+}
+void grid_bench_copy(rgrid *dst, rgrid *src) {
+  const int64_t n = npts(src->nx, src->ny, src->nz) * (int64_t)esize(src->prec) / (int64_t)sizeof(double);
+  need_same("grid_bench_copy", npts(dst->nx, dst->ny, dst->nz), npts(src->nx, src->ny, src->nz), dst->prec,
+            src->prec);
+  DFT_PROF_BEGIN(lib_stream());
+  LAUNCH(k_copy_d, n, (const double *)src->dev, (double *)dst->dev, n);
+  DFT_PROF_END(lib_stream(), "copy", 2.0 * n * sizeof(double));
+}
 
 // ---------------------------------------------------------------- timers --
 void grid_timer_start(grid_timer *t) {

@@ -359,6 +359,45 @@ __global__ void k_energyC(T *eps, const T *__restrict__ Jx, const T *__restrict_
   }
 }
 
+// Backward of E[rho], KC part: H = sum_d (d_d rho) J_d and M_d = rho_st J_d.
+template <typename T>
+__global__ void k_grad_kc(const T *__restrict__ rho, const T *__restrict__ rhot, const T *__restrict__ Jx,
+                          const T *__restrict__ Jy, const T *__restrict__ Jz, T *__restrict__ H, T *__restrict__ Mx,
+                          T *__restrict__ My, T *__restrict__ Mz, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    const T rst = rhot[c] * (-pw.inv_rho0s) + (T)1;
+    const T J[3] = {Jx[c], Jy[c], Jz[c]};
+    T h = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) h += (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h * J[d];
+    H[c] = h;
+    Mx[c] = rst * J[0];
+    My[c] = rst * J[1];
+    Mz[c] = rst * J[2];
+  }
+}
+
+// grad = local + linear terms + c sum_d D_d M_d (M = null: no KC).
+template <typename T>
+__global__ void k_grad_final(T *__restrict__ grad, const T *__restrict__ vre, const T *__restrict__ lin,
+                             const T *__restrict__ Mx, const T *__restrict__ My, const T *__restrict__ Mz, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    T g = vre[c] + lin[c];
+    if (Mx) {
+      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+      const T *M[3] = {Mx, My, Mz};
+      T div = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) div += (M[d][nb.p[d]] - M[d][nb.m[d]]) * pw.inv2h;
+      g += pw.ckc * div;
+    }
+    grad[c] = g;
+  }
+}
+
 }  // namespace dft
 
 // ============================================================ device state ==
@@ -436,10 +475,13 @@ template <typename T> PW<T> make_pw(dft_ot_functional *otf, double wf_mass, doub
   return pw;
 }
 
-#define LAUNCH_PW(kern, ...)                                                            \
-  do {                                                                                  \
+// Pointwise launch; nf = number of N-point real fields read plus written (algorithmic traffic).
+#define LAUNCH_PW(name, nf, kern, ...)                                                        \
+  do {                                                                                        \
+    DFT_PROF_BEGIN(lib_stream());                                                             \
     hipLaunchKernelGGL(kern, dim3(grid_for(d->n)), dim3(kBlock), 0, lib_stream(), __VA_ARGS__); \
-    DFT_LAUNCH_CHECK();                                                                  \
+    DFT_LAUNCH_CHECK();                                                                       \
+    DFT_PROF_END(lib_stream(), name, (double)(nf) * (double)d->n * sizeof(T));               \
   } while (0)
 
 // Which parts of the potential to evaluate.
@@ -466,12 +508,12 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
       vx = S<T>(d, VX);
       vy = S<T>(d, VY);
       vz = S<T>(d, VZ);
-      LAUNCH_PW((k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+      LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
     } else {
-      LAUNCH_PW((k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
+      LAUNCH_PW("ot_prep_density", 3, (k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
     }
   } else if (BF) {
-    LAUNCH_PW((k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+    LAUNCH_PW("ot_prep_products", 8, (k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
   }
   // F1
   {
@@ -520,7 +562,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
                 S<T>(d, SC)};
     StageBOut bo{S<T>(d, Q4), S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, VRE), S<T>(d, VIM),
                  S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
-#define SB(L, K, B) LAUNCH_PW((k_stageB<T, L, K, B>), bi, bo, pw)
+#define SB(L, K, B) LAUNCH_PW("ot_stage_local_kc_bf", 2 + 2 * L + 4 * K + 12 * B, (k_stageB<T, L, K, B>), bi, bo, pw)
     if (LOC && KC && BF) SB(true, true, true);
     else if (LOC && KC) SB(true, true, false);
     else if (LOC && BF) SB(true, false, true);
@@ -561,9 +603,9 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     StageCIn ci{rho, S<T>(d, RT), S<T>(d, SA_), S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ), S<T>(d, PX),
                 S<T>(d, PY), S<T>(d, PZ), S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
     StageCOut co{S<T>(d, SC), S<T>(d, VRE), S<T>(d, VIM)};
-    if (KC && BF) LAUNCH_PW((k_stageC<T, true, true>), ci, co, pw);
-    else if (KC) LAUNCH_PW((k_stageC<T, true, false>), ci, co, pw);
-    else LAUNCH_PW((k_stageC<T, false, true>), ci, co, pw);
+    if (KC && BF) LAUNCH_PW("ot_stage_kc_terms_bf_div", 17, (k_stageC<T, true, true>), ci, co, pw);
+    else if (KC) LAUNCH_PW("ot_stage_kc_terms_bf_div", 12, (k_stageC<T, true, false>), ci, co, pw);
+    else LAUNCH_PW("ot_stage_kc_terms_bf_div", 5, (k_stageC<T, false, true>), ci, co, pw);
   }
   // F3 + I3
   if (LOC || KC) {
@@ -586,7 +628,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     c2r3d<T>(p, 3, in, 1, &c, kt, tmp, out, s);
   }
   // P4
-  LAUNCH_PW((k_final<T>), (cplx<T> *)V->dev, S<T>(d, VRE), (LOC || KC) ? S<T>(d, RB) : (T *)nullptr,
+  LAUNCH_PW("ot_accumulate_potential", 5 + ((LOC || KC) ? 1 : 0) + (BF ? 1 : 0), (k_final<T>), (cplx<T> *)V->dev, S<T>(d, VRE), (LOC || KC) ? S<T>(d, RB) : (T *)nullptr,
             BF ? S<T>(d, VIM) : (T *)nullptr, d->n);
   if (write_back_w && BF) {
     // the reference leaves v_d A - B_d in the velocity inputs (src/ot.c:923-931)
@@ -616,14 +658,14 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
   if (!rho_given) {
     // density (and velocities) from psi
     if (BF)
-      LAUNCH_PW((k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+      LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
     else
-      LAUNCH_PW((k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
+      LAUNCH_PW("ot_prep_density", 3, (k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
   } else if (BF) {
     // velocities from psi (grid_wf_velocity, src/ot-energy.c:231), products with the given density
-    LAUNCH_PW((k_prep<T, true>), psi, S<T>(d, VRE), vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4),
+    LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, S<T>(d, VRE), vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4),
               pw);
-    LAUNCH_PW((k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+    LAUNCH_PW("ot_prep_products", 8, (k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
   }
   {
     std::vector<const T *> in{rho};
@@ -667,7 +709,7 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
                 S<T>(d, SBZ), S<T>(d, SC)};
     T *G[3] = {S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ)};
     const int a = acc ? 1 : 0;
-#define EB(L, K, B) LAUNCH_PW((k_energyB<T, L, K, B>), ei, eps, G[0], G[1], G[2], pw, a)
+#define EB(L, K, B) LAUNCH_PW("ot_energy_local_bf", 2 + 2 * L + 4 * K + 8 * B + a, (k_energyB<T, L, K, B>), ei, eps, G[0], G[1], G[2], pw, a)
     if (LOC && KC && BF) EB(true, true, true);
     else if (LOC && KC) EB(true, true, false);
     else if (LOC && BF) EB(true, false, true);
@@ -686,10 +728,73 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
       T *J[3] = {S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ)};
       cplx<T> *tmp[3] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2)};
       c2r3d<T>(p, 3, cin, 3, c, kt, tmp, J, s);
-      LAUNCH_PW((k_energyC<T>), eps, J[0], J[1], J[2], G[0], G[1], G[2], (T)(-otf->alpha_s / (4.0 * otf->mass)),
+      LAUNCH_PW("ot_energy_kc", 8, (k_energyC<T>), eps, J[0], J[1], J[2], G[0], G[1], G[2], (T)(-otf->alpha_s / (4.0 * otf->mass)),
                 d->n);
     }
   }
+}
+
+// Backward of E[rho] (LJ, c2/c3, KC): exact discrete dE/drho.  Reverse-mode
+// differentiation of the energy of src/ot-energy.c:64-93, 144-190 with the
+// central difference antisymmetric and the kernels symmetric:
+//   dE/drho = V_LJ*rho + c2/2 rb^2 + c3/3 rb^3 + Pi*[(c2 rb + c3 rb^2) rho]
+//           + c sum_d D_d(rho_st J_d) + (c/rho_0s) F*(sum_d (D_d rho) J_d).
+template <typename T> void gradient_impl(dft_ot_functional *otf, T *grad, const T *rho) {
+  dft_ot_device *d = otf->device;
+  Plan3D<T> &p = plan_of<T>(d);
+  hipStream_t s = lib_stream();
+  const double h3N = d->step * d->step * d->step / (double)d->n;
+  const bool KC = (otf->model & DFT_OT_KC) != 0;
+  fft::KernTab kt = kerntab<T>(d);
+  PW<T> pw = make_pw<T>(otf, 1.0, otf->div_epsilon, false);
+  {
+    const T *in[1] = {rho};
+    cplx<T> *out[1] = {CS<T>(d, SP0)};
+    r2c3d<T>(p, 1, in, out, s);
+  }
+  {
+    const cplx<T> *in[1] = {CS<T>(d, SP0)};
+    fft::Combine c[2] = {cmb1(0, K_SA, -1, h3N), cmb1(0, K_F, -1, h3N)};
+    T *out[2] = {S<T>(d, RB), S<T>(d, RT)};
+    cplx<T> *tmp[2] = {CS<T>(d, TMP0), CS<T>(d, TMP1)};
+    c2r3d<T>(p, 1, in, KC ? 2 : 1, c, kt, tmp, out, s);
+  }
+  {
+    StageBIn bi{rho, S<T>(d, RB), S<T>(d, RT), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr};
+    StageBOut bo{S<T>(d, Q4), S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, VRE), nullptr, nullptr, nullptr,
+                 nullptr};
+    if (KC) LAUNCH_PW("ot_stage_local_kc_bf", 10, (k_stageB<T, true, true, false>), bi, bo, pw);
+    else LAUNCH_PW("ot_stage_local_kc_bf", 4, (k_stageB<T, true, false, false>), bi, bo, pw);
+  }
+  {
+    const T *in[4] = {S<T>(d, Q4), S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ)};
+    cplx<T> *out[4] = {CS<T>(d, SP4), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3)};
+    r2c3d<T>(p, KC ? 4 : 1, in, out, s);
+  }
+  if (KC) {
+    const cplx<T> *in[3] = {CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3)};
+    fft::Combine c[3] = {cmb1(0, K_F, -1, h3N), cmb1(1, K_F, -1, h3N), cmb1(2, K_F, -1, h3N)};
+    T *J[3] = {S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ)};
+    cplx<T> *tmp[3] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2)};
+    c2r3d<T>(p, 3, in, 3, c, kt, tmp, J, s);
+    LAUNCH_PW("ot_grad_kc", 9, (k_grad_kc<T>), rho, S<T>(d, RT), J[0], J[1], J[2], S<T>(d, SC), S<T>(d, WX),
+              S<T>(d, WY), S<T>(d, WZ), pw);
+    const T *hin[1] = {S<T>(d, SC)};
+    cplx<T> *hout[1] = {CS<T>(d, SH)};
+    r2c3d<T>(p, 1, hin, hout, s);
+  }
+  {
+    const cplx<T> *in[3] = {CS<T>(d, SP0), CS<T>(d, SP4), CS<T>(d, SH)};
+    fft::Term t[3] = {{0, K_LJ, -1, h3N}, {1, K_SA, -1, h3N},
+                      {2, K_F, -1, h3N * otf->alpha_s / (2.0 * otf->mass) / otf->rho_0s}};
+    fft::Combine c = cmb(KC ? 3 : 2, t);
+    T *out[1] = {S<T>(d, RB)};
+    cplx<T> *tmp[1] = {CS<T>(d, TMP0)};
+    c2r3d<T>(p, 3, in, 1, &c, kt, tmp, out, s);
+  }
+  LAUNCH_PW("ot_grad_final", KC ? 6 : 3, (k_grad_final<T>), grad, S<T>(d, VRE), S<T>(d, RB),
+            KC ? S<T>(d, WX) : (T *)nullptr, S<T>(d, WY), S<T>(d, WZ), pw);
 }
 
 // Build the kernel spectra (FP64 on the device), src/ot.c:138-217.
@@ -1005,6 +1110,20 @@ void dft_ot_energy_density_bf(dft_ot_functional *otf, rgrid *energy_density, wf 
 
 REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf) {
   dft_ot_energy_density(otf, otf->workspace1, gwf);
+  return rgrid_integral(otf->workspace1);
+}
+
+void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density) {
+  check_r(otf, grad, "dft_ot_energy_gradient");
+  check_r(otf, density, "dft_ot_energy_gradient");
+  DFT_BY_PREC(otf->device->prec, gradient_impl<T>(otf, (T *)grad->dev, (const T *)density->dev));
+}
+
+REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
+  check_r(otf, density, "dft_ot_energy_rho");
+  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, false};
+  DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)otf->workspace1->dev, nullptr, 1.0, (T *)density->dev,
+                                                true, parts, false));
   return rgrid_integral(otf->workspace1);
 }
 

@@ -619,6 +619,70 @@ void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, do
   free(rho); free(w1); free(w2); free(cw1); free(cw2);
 }
 
+/* ------------------------------------------------- backward of E[rho] -- */
+
+/* Exact discrete functional derivative dE/drho of the density parts of
+   src/ot-energy.c (LJ, c2/c3, KC), derived by reverse-mode differentiation of
+   the discrete energy (central differences are antisymmetric, the kernels
+   symmetric):
+     dE/drho = V_LJ*rho + c2/2 rb^2 + c3/3 rb^3 + Pi*[(c2 rb + c3 rb^2) rho]
+             + c sum_d D_d(rho_st J_d) + (c/rho_0s) F*(sum_d (D_d rho) J_d),
+   c = alpha_s/(2M), J_d = F*((D_d rho) rho_st), rho_st = 1 - F*rho/rho_0s.
+   The backflow energy depends on the current and is not part of E[rho]. */
+void orc_ot_energy_gradient(orc_ot *o, double *grad, const double *rho) {
+  geom g = G(o);
+  long n = g.n;
+  double complex *rho_tf = cbuf(n), *cw = cbuf(n);
+  double *w1 = rbuf(n), *w2 = rbuf(n);
+  memset(grad, 0, sizeof(double) * (size_t) n);
+  double complex *pot = cbuf(n);
+  r_fft(&g, rho, rho_tf);
+  add_lj_local(o, &g, pot, rho, rho_tf, cw, w1, w2);
+  for (long i = 0; i < n; i++) grad[i] = creal(pot[i]);
+  if (o->model & ORC_OT_KC) {
+    double c = o->alpha_s / (2.0 * o->mass);
+    double *rho_st = rbuf(n), *gd = rbuf(n), *J = rbuf(n), *H = rbuf(n), *M = rbuf(n);
+    r_convolute(&g, cw, o->g_tf, rho_tf);
+    r_inv_norm2(&g, cw, rho_st);
+    for (long i = 0; i < n; i++) rho_st[i] = rho_st[i] * (-1.0 / o->rho_0s) + 1.0;
+    for (int d = 0; d < 3; d++) {
+      grad_dir(rho, gd, g.nx, g.ny, g.nz, g.step, d);
+      for (long i = 0; i < n; i++) w1[i] = gd[i] * rho_st[i];
+      r_fft(&g, w1, cw);
+      r_convolute(&g, cw, o->g_tf, cw);
+      r_inv_norm2(&g, cw, J);
+      for (long i = 0; i < n; i++) {
+        H[i] += gd[i] * J[i];
+        M[i] = rho_st[i] * J[i];
+      }
+      grad_dir(M, w2, g.nx, g.ny, g.nz, g.step, d);
+      for (long i = 0; i < n; i++) grad[i] += c * w2[i];
+    }
+    r_fft(&g, H, cw);
+    r_convolute(&g, cw, o->g_tf, cw);
+    r_inv_norm2(&g, cw, w1);
+    for (long i = 0; i < n; i++) grad[i] += (c / o->rho_0s) * w1[i];
+    free(rho_st); free(gd); free(J); free(H); free(M);
+  }
+  free(rho_tf); free(cw); free(w1); free(w2); free(pot);
+}
+
+/* E[rho] of the density parts (LJ, c2/c3, KC): h^3 sum eps. */
+double orc_ot_energy_rho(orc_ot *o, const double *rho) {
+  geom g = G(o);
+  long n = g.n;
+  double complex *psi = cbuf(n);
+  double *eps = rbuf(n);
+  for (long i = 0; i < n; i++) psi[i] = sqrt(rho[i] > 0 ? rho[i] : 0.0);
+  int64_t save = o->model;
+  o->model &= ~(int64_t) ORC_OT_BACKFLOW;
+  orc_ot_energy_density(o, eps, psi, ORC_HELIUM_MASS);
+  o->model = save;
+  double e = orc_integral(eps, n, g.step);
+  free(psi); free(eps);
+  return e;
+}
+
 /* ------------------------------------------------- propagation (a11) -- */
 
 static inline double kcomp(int i, int n, double step) {

@@ -30,7 +30,7 @@
 #define ORC_OT_KC       1
 #define ORC_OT_BACKFLOW 8
 
-/* Unit conversions (CODATA 2018; libgrid's au.h is absent). */
+/* Unit conversions: CODATA 2018 (libgrid's au.h, which the reference uses, is not available). */
 #define ORC_AUTOANG 0.529177210903
 #define ORC_AUTOK   315775.02480407
 #define ORC_AUTOFS  0.024188843265857
@@ -67,6 +67,11 @@ void orc_ot_potential(orc_ot *otf, double complex *pot, const double complex *ps
 void orc_ot_potential2(orc_ot *otf, double complex *pot, const double *rho, double *vx, double *vy, double *vz);
 /* Overwrites eps with the OT energy density (kinetic part excluded). */
 void orc_ot_energy_density(orc_ot *otf, double *eps, const double complex *psi, double wf_mass);
+
+/* Backward of E[rho]: exact discrete dE/drho of the LJ, c2/c3 and KC energy. */
+void orc_ot_energy_gradient(orc_ot *otf, double *grad, const double *rho);
+/* E[rho] (LJ, c2/c3, KC parts; h^3 sum eps). */
+double orc_ot_energy_rho(orc_ot *otf, const double *rho);
 
 /* libgrid-equivalent primitives used on the path. */
 void orc_fft3d(double complex *a, int nx, int ny, int nz, int sign);

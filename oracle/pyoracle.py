@@ -45,6 +45,8 @@ def lib():
             "orc_ot_potential": (None, [P, P, P, D, P]),
             "orc_ot_potential2": (None, [P, P, P, P, P, P]),
             "orc_ot_energy_density": (None, [P, P, P, D]),
+            "orc_ot_energy_gradient": (None, [P, P, P]),
+            "orc_ot_energy_rho": (D, [P, P]),
             "orc_fft3d": (None, [P, I, I, I, I]),
             "orc_wf_velocity": (None, [P, P, P, P, I, I, I, D, D, D]),
             "orc_gradient": (None, [P, P, P, P, I, I, I, D]),
@@ -141,6 +143,17 @@ class OracleOT:
         eps = np.empty(self.shape)
         lib().orc_ot_energy_density(self._h, _p(eps), _p(psi), wf_mass)
         return eps
+
+    def energy_gradient(self, rho):
+        """Backward of E[rho]: exact discrete dE/drho (LJ, c2/c3, KC)."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        g = np.empty(self.shape)
+        lib().orc_ot_energy_gradient(self._h, _p(g), _p(rho))
+        return g
+
+    def energy_rho(self, rho):
+        rho = np.ascontiguousarray(rho, np.float64)
+        return lib().orc_ot_energy_rho(self._h, _p(rho))
 
     def kernel_map(self, which, min_sub=4, max_sub=32):
         out = np.empty(self.shape)

@@ -204,3 +204,79 @@ def test_predict_correct_step_matches_oracle(oracle, prec):
         assert relerr(w.get(), want) < (1e-10 if prec == L.GRID_FP64 else 2e-5)
     for x in (pot, ve, f, wp, w):
         x.free()
+
+
+# ------------------------------------------------------- backward of E[rho] --
+def _gpu_gradient(model, shape, step, prec, rho, min_sub=2, max_sub=8):
+    w = L.Wave(shape, step, prec)
+    f = L.Functional(model, w, min_sub, max_sub)
+    r = L.RField(shape, step, prec)
+    g = L.RField(shape, step, prec)
+    r.set(rho)
+    L.lib().dft_ot_energy_gradient(f.h, g.h, r.h)
+    out = g.get()
+    E = L.lib().dft_ot_energy_rho(f.h, r.h)
+    for x in (g, r, f, w):
+        x.free()
+    return out, E
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+@pytest.mark.parametrize("model", [0, 1], ids=["plain", "kc"])
+def test_energy_gradient_matches_oracle(oracle, prec, model):
+    shape, step = (16, 16, 32), 0.6 / L.AUTOANG
+    o = oracle.OracleOT(model, shape, step, 2, 8)
+    rho = np.abs(inputs.smooth_modulated(shape, step, o.p.rho0, 0.3)) ** 2
+    g, E = _gpu_gradient(model, shape, step, prec, rho)
+    assert relerr(g, o.energy_gradient(rho)) < TOL[prec]
+    Eo = o.energy_rho(rho)
+    assert abs(E - Eo) < 10 * TOL[prec] * abs(Eo)
+
+
+def test_plain_gradient_equals_potential(oracle):
+    """For PLAIN the backward and dft_ot_potential coincide (src/ot.c:524-586 is dE/drho)."""
+    shape, step = (16, 16, 16), 0.6 / L.AUTOANG
+    o = oracle.OracleOT(0, shape, step, 2, 8)
+    psi = inputs.smooth_modulated(shape, step, o.p.rho0, 0.3)
+    g, _ = _gpu_gradient(0, shape, step, L.GRID_FP64, np.abs(psi) ** 2)
+    V, _, _, _ = _gpu_potential(0, shape, step, L.GRID_FP64, np.abs(psi).astype(complex))
+    assert relerr(g, V.real) < 1e-12
+
+
+def test_energy_gradient_fd_128(oracle):
+    """Central FD of E[rho] (FP64 device) on 20 seeded perturbations of amplitude 1e-4 rho0 at 128^3,
+    against the FP64 and FP32 backward; <= 1e-4 relative."""
+    n, step = 128, 0.4 / L.AUTOANG
+    shape = (n, n, n)
+    lib = L.lib()
+    w64 = L.Wave(shape, step, L.GRID_FP64)
+    f64 = L.Functional(9, w64)
+    rho0 = f64.p.rho0
+    rho = np.abs(inputs.droplet(shape, step)) ** 2 + 0.05 * rho0
+    r = L.RField(shape, step, L.GRID_FP64)
+    g = L.RField(shape, step, L.GRID_FP64)
+    r.set(rho)
+    lib.dft_ot_energy_gradient(f64.h, g.h, r.h)
+    grad64 = g.get()
+    w32 = L.Wave(shape, step, L.GRID_FP32)
+    f32 = L.Functional(9, w32)
+    r32 = L.RField(shape, step, L.GRID_FP32)
+    g32 = L.RField(shape, step, L.GRID_FP32)
+    r32.set(rho)
+    lib.dft_ot_energy_gradient(f32.h, g32.h, r32.h)
+    grad32 = g32.get()
+    worst64 = worst32 = 0.0
+    for j in range(20):
+        eta = 1e-4 * rho0 * inputs.splitmix64_uniform(rho.size, inputs.SEED + j).reshape(shape)
+        r.set(rho + eta)
+        ep = lib.dft_ot_energy_rho(f64.h, r.h)
+        r.set(rho - eta)
+        em = lib.dft_ot_energy_rho(f64.h, r.h)
+        fd = (ep - em) / 2.0
+        worst64 = max(worst64, abs(fd - (grad64 * eta).sum() * step ** 3) / abs(fd))
+        worst32 = max(worst32, abs(fd - (grad32 * eta).sum() * step ** 3) / abs(fd))
+    print("fd rel err fp64 %.2e fp32 %.2e" % (worst64, worst32))
+    assert worst64 < 1e-6
+    assert worst32 < 1e-4
+    for x in (g, r, g32, r32, f64, f32, w64, w32):
+        x.free()
