@@ -5,7 +5,9 @@ Workload: full OT (PLAIN|KC|BACKFLOW), periodic 256^3 grid, h = 0.3 A, bulk
 rho0 with a straight vortex line (phase atan2(y, x) along z) and 1e-3 seeded
 density noise, real-time step dt = 2 fs.  One step is the reference driver's
 single-step pattern (doc/libdft-manual.tex:496-504): zero the potential,
-dft_ot_potential, add -mu, grid_wf_propagate.  Fields are FP32 on the device
+dft_ot_potential, add -mu, grid_wf_propagate; the value is measured on the
+library's fused form of that sequence (dft_ot_propagate), and the four-call
+form is timed beside it.  Fields are FP32 on the device
 with FP64 reductions; the FP64 device path runs the same steps for the
 precision comparison.  The CPU baseline is the FP64 oracle (OpenMP on the host
 cores) timed on one step of the same workload.
@@ -65,8 +67,14 @@ class Run:
         self.mu = self.f.bulk("dEdRho", self.f.p.rho0)
         self.n, self.step = n, step
 
-    def steps(self, k, dt):
+    def steps(self, k, dt, pattern=False):
+        """k single steps: the fused dft_ot_propagate, or (pattern) the reference driver's
+        cgrid_zero / dft_ot_potential / cgrid_add(-mu) / grid_wf_propagate sequence."""
         lib, z = self.lib, L.Complex.of(dt)
+        if not pattern:
+            for _ in range(k):
+                lib.dft_ot_propagate(self.f.h, self.w.h, None, self.mu, z)
+            return
         mmu = L.Complex.of(-self.mu)
         for _ in range(k):
             lib.cgrid_zero(self.pot.h)
@@ -74,10 +82,10 @@ class Run:
             lib.cgrid_add(self.pot.h, mmu)
             lib.grid_wf_propagate(self.w.h, self.pot.h, z)
 
-    def timed(self, k, dt):
+    def timed(self, k, dt, pattern=False):
         self.lib.grid_synchronize()
         t0 = time.perf_counter()
-        self.steps(k, dt)
+        self.steps(k, dt, pattern)
         self.lib.grid_synchronize()
         return (time.perf_counter() - t0) / k
 
@@ -143,7 +151,9 @@ def main():
     tprof = r32.timed(args.profile_steps, dt)
     lib.dft_profile_enable(0)
     rows = profile_table()
-    nsteps32 = args.warmup + args.steps + args.profile_steps
+    # the same steps through the reference driver's four calls (unfused potential accumulate + propagate)
+    tpat = r32.timed(max(5, args.steps // 5), dt, pattern=True)
+    nsteps32 = args.warmup + args.steps + args.profile_steps + max(5, args.steps // 5)
     e32, n32 = r32.energy()
     r32.free()
 
@@ -166,6 +176,7 @@ def main():
                             "moved_bytes_per_step": sum(r["bytes_per_launch"] * r["launches"] for r in rows)
                             / args.profile_steps}
     out["profiled_ms_per_step"] = 1e3 * tprof
+    out["driver_pattern_ms_per_step"] = 1e3 * tpat
     out["kernels"] = [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows[:12]]
 
     if not args.no_fp64:
@@ -174,6 +185,7 @@ def main():
         r64.steps(args.warmup, dt)
         t64 = r64.timed(args.steps, dt)
         r64.steps(args.profile_steps, dt)
+        r64.steps(max(5, args.steps // 5), dt, pattern=True)
         e64, n64 = r64.energy()
         r64.free()
         out["fp64_steps_per_s"] = 1.0 / t64

@@ -160,6 +160,9 @@ void rgrid_gradient_z(rgrid *src, rgrid *dst);
 
 /* Reductions: FP64 accumulation with a compensated final sum. */
 REAL rgrid_integral(rgrid *grid);
+REAL rgrid_integral_of_product(rgrid *a, rgrid *b);
+/* h^3 sum |psi|^2 Re V */
+REAL grid_wf_potential_energy(wf *gwf, cgrid *potential);
 REAL grid_wf_norm(wf *gwf);
 void grid_wf_normalize(wf *gwf);
 /* Kinetic energy h^3/N sum_k (k^2/2M)|psi_k|^2 (the potential argument must be NULL). */
@@ -264,6 +267,13 @@ REAL dft_common_gaussian(void *arg, REAL x, REAL y, REAL z);
 REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z);
 
 /* ---- extensions ---- */
+/* One split-operator step with V = V_OT[psi] + V_ext - mu (vext may be NULL);
+   equal to the reference single-step driver sequence cgrid_zero /
+   grid_real_to_complex_re / dft_ot_potential / cgrid_add(-mu) /
+   grid_wf_propagate (doc/libdft-manual.tex:496-504), fused on the device. */
+void dft_ot_propagate(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REAL_COMPLEX time);
+/* Predict-correct step of examples/imag_time/imag_time.c:231-239 (gwfp gets the predictor). */
+void dft_ot_propagate_pc(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu, REAL_COMPLEX time);
 /* KC potential mode: 0 = reference formula (default), 1 = exact dE/drho. */
 void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact);
 /* Energy of the functional, h^3 sum eps_OT, in FP64 (kinetic part excluded). */

@@ -100,7 +100,8 @@ _SIGS = {
     "grid_wf_density": (None, ["wf", "rg"]), "grid_wf_velocity": (None, ["wf", "rg", "rg", "rg", "d"]),
     "rgrid_gradient_x": (None, ["rg", "rg"]), "rgrid_gradient_y": (None, ["rg", "rg"]),
     "rgrid_gradient_z": (None, ["rg", "rg"]),
-    "rgrid_integral": ("d", ["rg"]), "grid_wf_norm": ("d", ["wf"]), "grid_wf_normalize": (None, ["wf"]),
+    "rgrid_integral": ("d", ["rg"]), "rgrid_integral_of_product": ("d", ["rg", "rg"]),
+    "grid_wf_potential_energy": ("d", ["wf", "cg"]), "grid_wf_norm": ("d", ["wf"]), "grid_wf_normalize": (None, ["wf"]),
     "grid_wf_energy": ("d", ["wf", "cg"]),
     "grid_wf_propagate": (None, ["wf", "cg", "z"]),
     "grid_wf_propagate_predict": (None, ["wf", "wf", "cg", "z"]),
@@ -116,6 +117,8 @@ _SIGS = {
     "dft_ot_energy_gradient": (None, ["otf", "rg", "rg"]), "dft_ot_energy_rho": ("d", ["otf", "rg"]),
     "dft_profile_enable": (None, ["i"]), "dft_profile_reset": (None, []), "dft_profile_collect": ("i", []),
     "grid_bench_copy": (None, ["rg", "rg"]),
+    "dft_ot_propagate": (None, ["otf", "wf", "rg", "d", "z"]),
+    "dft_ot_propagate_pc": (None, ["otf", "wf", "wf", "rg", "d", "z"]),
 }
 for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "chempot_pressurized",
            "compressibility", "sound_speed", "chempot3"):

@@ -57,21 +57,6 @@ template <typename T> void Plan3D<T>::destroy() {
   sinx = siny = sinz = nullptr;
 }
 
-#define DFT_DISPATCH_N(n, CALL)                                   \
-  switch (n) {                                                    \
-    case 4: { constexpr int N_ = 4; CALL; } break;                \
-    case 8: { constexpr int N_ = 8; CALL; } break;                \
-    case 16: { constexpr int N_ = 16; CALL; } break;              \
-    case 32: { constexpr int N_ = 32; CALL; } break;              \
-    case 64: { constexpr int N_ = 64; CALL; } break;              \
-    case 128: { constexpr int N_ = 128; CALL; } break;            \
-    case 256: { constexpr int N_ = 256; CALL; } break;            \
-    case 512: { constexpr int N_ = 512; CALL; } break;            \
-    default:                                                      \
-      fprintf(stderr, "libdft: unsupported FFT length %d\n", n);  \
-      abort();                                                    \
-  }
-
 namespace {
 
 template <typename T, int N>
@@ -167,6 +152,18 @@ void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dt
   DFT_PROF_END(s, "fft_x_kinetic_c2c", 2.0 * (double)p.n() * sizeof(cplx<T>));
 }
 
+template <typename T, int N>
+void launch_zc2c_phase_inv(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, T scale, hipStream_t s) {
+  using TL = fft::Tile<T, N>;
+  const int64_t nlines = (int64_t)p.nx * p.ny;
+  dim3 grid((unsigned)((nlines + TL::TILE - 1) / TL::TILE));
+  DFT_PROF_BEGIN(s);
+  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, +1, true>), grid, dim3(TL::THREADS), 0, s, psi, psi, phase, nlines,
+                     p.twz, (T)0, (T)0, scale, (double *)nullptr);
+  DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "fft_z_c2c_phase_inv", 3.0 * (double)p.n() * sizeof(cplx<T>));
+}
+
 template <typename T> int zc2c_tile(int nz) {
   int t = 1;
   DFT_DISPATCH_N(nz, (t = fft::Tile<T, N_>::TILE));
@@ -240,6 +237,36 @@ void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, do
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, psi, psi, pot, (T)dtr, (T)dti, inv, partial, s)));
 }
 
+template <typename T>
+void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, hipStream_t s) {
+  for (int b0 = 0; b0 < nb; b0 += fft::kMaxBatch) {
+    const int cnt = nb - b0 < fft::kMaxBatch ? nb - b0 : fft::kMaxBatch;
+    fft::Batch b{};
+    for (int i = 0; i < cnt; ++i) b.in[i] = b.out[i] = bufs[b0 + i];
+    if (dir < 0) {
+      DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, cnt, 1, cols, (T)1, s)));
+    } else {
+      DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, b, cnt, 1, cols, (T)1, s)));
+    }
+  }
+}
+
+template <typename T>
+void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double dtr, double dti, double mass,
+                 hipStream_t s) {
+  fft::Batch by{};
+  by.in[0] = psi;
+  by.out[0] = psi;
+  const double L2x = std::pow(2.0 * M_PI / (p.nx * p.step), 2), L2y = std::pow(2.0 * M_PI / (p.ny * p.step), 2),
+               L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
+  const T cx = (T)(L2x / (2.0 * mass)), cy = (T)(L2y / (2.0 * mass)), cz = (T)(L2z / (2.0 * mass));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, (T)dtr, (T)dti, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, (T)1, s)));
+  const T inv = (T)(1.0 / (double)p.n());
+  DFT_DISPATCH_N(p.nz, (launch_zc2c_phase_inv<T, N_>(p, psi, phase, inv, s)));
+}
+
 template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s) {
   fft::Batch b{};
   b.in[0] = out;
@@ -256,7 +283,10 @@ template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T
                          const fft::KernTab &, cplx<T> *const *, T *const *, hipStream_t);                   \
   template void strang<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double, double *,   \
                           hipStream_t);                                                                      \
-  template void c2c_fwd<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);
+  template void c2c_fwd<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
+  template void ypass<T>(const Plan3D<T> &, int, cplx<T> *const *, int, int, hipStream_t);                  \
+  template void strang_tail<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double,        \
+                               hipStream_t);
 DFT_INST(float)
 DFT_INST(double)
 

@@ -22,6 +22,22 @@
 #pragma once
 #include "common.hpp"
 
+// Instantiate CALL with `N_` bound to a supported line length n.
+#define DFT_DISPATCH_N(n, CALL)                                   \
+  switch (n) {                                                    \
+    case 4: { constexpr int N_ = 4; CALL; } break;                \
+    case 8: { constexpr int N_ = 8; CALL; } break;                \
+    case 16: { constexpr int N_ = 16; CALL; } break;              \
+    case 32: { constexpr int N_ = 32; CALL; } break;              \
+    case 64: { constexpr int N_ = 64; CALL; } break;              \
+    case 128: { constexpr int N_ = 128; CALL; } break;            \
+    case 256: { constexpr int N_ = 256; CALL; } break;            \
+    case 512: { constexpr int N_ = 512; CALL; } break;            \
+    default:                                                      \
+      fprintf(stderr, "libdft: unsupported FFT length %d\n", n);  \
+      abort();                                                    \
+  }
+
 namespace dft {
 namespace fft {
 
@@ -166,6 +182,17 @@ __device__ inline void line_fft(cplx<T> *L, const cplx<T> *tw, int tl, bool acti
 
 template <typename T, int N> __device__ inline void load_twiddles(cplx<T> *s_tw, const cplx<T> *g_tw) {
   for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = g_tw[i];
+}
+
+// e^{-i V dt/2} for complex V and complex dt.
+template <typename T>
+__device__ inline cplx<T> vphase(cplx<T> v, T dtr, T dti) {
+  const T a = (T)0.5 * (v.x * dti + v.y * dtr);
+  const T ph = (T)-0.5 * (v.x * dtr - v.y * dti);
+  T s, c;
+  dsincos(ph, &s, &c);
+  const T m = dexp(a);
+  return cmk<cplx<T>>(m * c, m * s);
 }
 
 // -------------------------------------------------------- pass parameters --

@@ -158,21 +158,11 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
 }
 
 // ---------------------------------------- propagator passes (complex psi) --
-// e^{-i V dt/2} for complex V and complex dt.
-template <typename T>
-__device__ inline cplx<T> vphase(cplx<T> v, T dtr, T dti) {
-  const T a = (T)0.5 * (v.x * dti + v.y * dtr);
-  const T ph = (T)-0.5 * (v.x * dtr - v.y * dti);
-  T s, c;
-  dsincos(ph, &s, &c);
-  const T m = dexp(a);
-  return cmk<cplx<T>>(m * c, m * s);
-}
-
 // z pass over a full complex field, multiplying by e^{-iV dt/2} before (DIR<0)
-// or after (DIR>0, together with `scale`) the transform.  After the inverse
-// pass each block writes sum |psi|^2 to partial[block] (FP64).
-template <typename T, int N, int DIR>
+// or after (DIR>0, together with `scale`) the transform.  With PHASE the
+// `pot` array already holds e^{-iV dt/2}.  After the inverse pass each block
+// writes sum |psi|^2 to partial[block] (FP64).
+template <typename T, int N, int DIR, bool PHASE = false>
 __global__ void __launch_bounds__((Tile<T, N>::THREADS))
     k_zc2c_pot(const cplx<T> *src, cplx<T> *dst, const cplx<T> *__restrict__ pot, int64_t nlines,
                const cplx<T> *__restrict__ g_tw, T dtr, T dti, T scale, double *__restrict__ partial) {
@@ -188,7 +178,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
   for (int e = threadIdx.x; e < nl * N; e += BS) {
     const int ln = e / N, el = e - ln * N;
     C v = src[g0 + e];
-    if (DIR < 0 && pot) v = cmul(v, vphase<T>(pot[g0 + e], dtr, dti));
+    if (DIR < 0 && pot) v = cmul(v, PHASE ? pot[g0 + e] : vphase<T>(pot[g0 + e], dtr, dti));
     s_line[ln * PITCH + el] = v;
   }
   __syncthreads();
@@ -200,7 +190,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
     C v = s_line[ln * PITCH + el];
     if (DIR > 0) {
       v = cscale(v, scale);
-      if (pot) v = cmul(v, vphase<T>(pot[g0 + e], dtr, dti));
+      if (pot) v = cmul(v, PHASE ? pot[g0 + e] : vphase<T>(pot[g0 + e], dtr, dti));
       acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
     }
     dst[g0 + e] = v;

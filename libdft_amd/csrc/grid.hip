@@ -233,6 +233,19 @@ void reduce_kinetic(const cplx<T> *spec, int nx, int ny, int nz, double cx, doub
                      nx, ny, nz, cx, cy, cz, partial);
   DFT_LAUNCH_CHECK();
 }
+template <typename T> __global__ void k_red_density_re(const cplx<T> *a, const cplx<T> *v, int64_t n, double *partial) {
+  double acc = 0.0;
+  GRID_STRIDE(i, n) acc += ((double)a[i].x * (double)a[i].x + (double)a[i].y * (double)a[i].y) * (double)v[i].x;
+  block_sum_store(acc, partial);
+}
+template <typename T>
+void reduce_density_re(const cplx<T> *a, const cplx<T> *v, int64_t n, double *partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_red_density_re<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, a, v, n, partial);
+  DFT_LAUNCH_CHECK();
+}
+template void reduce_density_re<float>(const float2 *, const float2 *, int64_t, double *, hipStream_t);
+template void reduce_density_re<double>(const double2 *, const double2 *, int64_t, double *, hipStream_t);
+
 template <typename T> void ew_density(const cplx<T> *psi, T *rho, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_density<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, psi, rho, n);
   DFT_LAUNCH_CHECK();
@@ -674,6 +687,27 @@ REAL rgrid_integral(rgrid *g) {
   const int64_t n = npts(g->nx, g->ny, g->nz);
   double *part = dft_scratch_partials(red_blocks(n));
   DFT_BY_PREC(g->prec, reduce_sum<T>((const T *)g->dev, nullptr, n, part, lib_stream()));
+  return sum_partials(part, red_blocks(n), lib_stream()) * g->step * g->step * g->step;
+}
+
+// h^3 sum a b (libgrid rgrid_integral_of_product)
+REAL rgrid_integral_of_product(rgrid *a, rgrid *b) {
+  const int64_t n = npts(a->nx, a->ny, a->nz);
+  need_same("rgrid_integral_of_product", n, npts(b->nx, b->ny, b->nz), a->prec, b->prec);
+  double *part = dft_scratch_partials(red_blocks(n));
+  DFT_BY_PREC(a->prec, reduce_sum<T>((const T *)a->dev, (const T *)b->dev, n, part, lib_stream()));
+  return sum_partials(part, red_blocks(n), lib_stream()) * a->step * a->step * a->step;
+}
+
+// h^3 sum |psi|^2 Re V (libgrid grid_wf_potential_energy)
+REAL grid_wf_potential_energy(wf *w, cgrid *potential) {
+  cgrid *g = w->grid;
+  const int64_t n = npts(g->nx, g->ny, g->nz);
+  need_same("grid_wf_potential_energy", n, npts(potential->nx, potential->ny, potential->nz), g->prec,
+            potential->prec);
+  double *part = dft_scratch_partials(red_blocks(n));
+  DFT_BY_PREC(g->prec, reduce_density_re<T>((const cplx<T> *)g->dev, (const cplx<T> *)potential->dev, n, part,
+                                            lib_stream()));
   return sum_partials(part, red_blocks(n), lib_stream()) * g->step * g->step * g->step;
 }
 

@@ -37,6 +37,14 @@ template <typename T>
 void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, double dti, double mass,
             double *partial, hipStream_t s);
 int64_t strang_partials(int nx, int ny, int nz, int prec);
+// y pass (dir -1 forward, +1 inverse) over nb fields of `cols` columns, in place.
+template <typename T>
+void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, hipStream_t s);
+// Rest of a Strang step after the fused z pass: y, x (with kinetic phase), y,
+// and the inverse z pass multiplying by `phase` = e^{-iV dt/2}.
+template <typename T>
+void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double dtr, double dti, double mass,
+                 hipStream_t s);
 // Full complex forward transform (out of place).
 template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s);
 
@@ -46,6 +54,8 @@ double sum_partials(const double *dpartials, int64_t n, hipStream_t s);
 int64_t red_blocks(int64_t n);
 template <typename T> void reduce_sum(const T *a, const T *b, int64_t n, double *partial, hipStream_t s);
 template <typename T> void reduce_abs2(const cplx<T> *a, int64_t n, double *partial, hipStream_t s);
+template <typename T>
+void reduce_density_re(const cplx<T> *a, const cplx<T> *v, int64_t n, double *partial, hipStream_t s);
 template <typename T>
 void reduce_kinetic(const cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz, double *partial,
                     hipStream_t s);

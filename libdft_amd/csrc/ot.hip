@@ -22,6 +22,8 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "ot_common.hpp"
+#include "ot_fused.hpp"
 
 using namespace dft;
 
@@ -97,45 +99,6 @@ __global__ void k_kernel_map(double *out, int nx, int ny, int nz, double h, int 
 
 template <typename T> __global__ void k_spec_real(const double2 *spec, T *out, int64_t nh, double scale) {
   GRID_STRIDE(i, nh) out[i] = (T)(spec[i].x * scale);
-}
-
-// --------------------------------------------------------- pointwise math --
-template <typename T> struct PW {
-  int nx, ny, nz;
-  T inv2h;                 // 1 / 2h
-  T invM;                  // 1 / wavefunction mass (velocity)
-  T eps_v;                 // density epsilon in the velocity
-  T c2, c3;                // local correlation
-  T inv_rho0s;             // 1 / rho_0s
-  T ckc;                   // alpha_s / (2 mass)
-  T k23;                   // KC terms 2,3 coefficient
-  T mass;                  // functional mass
-  T div_eps;               // backflow density epsilon
-  T cbf;                   // backflow weight
-  T vmax, bfmax;           // clamps
-  int clamp_v, clamp_bf;
-};
-
-struct Nbr {
-  int64_t p[3], m[3];
-};
-__device__ inline Nbr neighbours(int64_t c, int nx, int ny, int nz) {
-  const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
-  const int64_t sx = (int64_t)ny * nz;
-  Nbr r;
-  r.p[0] = c + (i == nx - 1 ? -(int64_t)(nx - 1) * sx : sx);
-  r.m[0] = c + (i == 0 ? (int64_t)(nx - 1) * sx : -sx);
-  r.p[1] = c + (j == ny - 1 ? -(int64_t)(ny - 1) * nz : (int64_t)nz);
-  r.m[1] = c + (j == 0 ? (int64_t)(ny - 1) * nz : -(int64_t)nz);
-  r.p[2] = c + (k == nz - 1 ? -(int64_t)(nz - 1) : 1);
-  r.m[2] = c + (k == 0 ? (int64_t)(nz - 1) : -1);
-  return r;
-}
-
-template <typename T> __device__ inline T clampv(T v, T lim) {
-  if (v > lim) v = lim;
-  if (v < -lim) v = -lim;
-  return v;
 }
 
 // P1: psi -> rho and, for backflow, v_d (grid_wf_velocity), q_d = v_d rho,
@@ -422,6 +385,8 @@ struct dft_ot_device {
   void *plan;                // Plan3D<float> or Plan3D<double>
   void *slot[NSLOT];
   void *kern[4];             // real half spectra (LJ, SA, F, U)
+  void *phase;               // complex: e^{-iV dt/2} of the current step (allocated on first step)
+  void *vsave;               // complex: predictor potential V_n of a predict-correct step
 };
 
 namespace {
@@ -635,6 +600,288 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     DFT_HIP_CHECK(hipMemcpyAsync(vx, S<T>(d, WX), sizeof(T) * d->n, hipMemcpyDeviceToDevice, s));
     DFT_HIP_CHECK(hipMemcpyAsync(vy, S<T>(d, WY), sizeof(T) * d->n, hipMemcpyDeviceToDevice, s));
     DFT_HIP_CHECK(hipMemcpyAsync(vz, S<T>(d, WZ), sizeof(T) * d->n, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+// ------------------------------------------------------------ fused path --
+// dft_ot_potential on a wave function and the split-operator steps built on
+// it.  Schedule (stage kernels in ot_fused.hpp):
+//   zA, y, xmid{Pi, F, U products}, y^-1, zAB, y, [xmid{J_d, T1}, y^-1, zBC, y],
+//   xmid{V_LJ + Pi Q + k23 c F H}, y^-1, zC
+// 7 + 5 forward and 7 + 4 + 1 inverse real 3-D transforms for full OT, with
+// all pointwise stages inside the z passes.
+struct StepIn {
+  const void *vext;   // real field or null
+  double mu;
+  const void *vavg;   // predictor potential (corrector step) or null
+  void *vsave;        // where to keep V (predictor step) or null
+  void *phase;
+  const void *psi_src;
+  void *psi_dst;
+  double dtr, dti;
+};
+
+template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho, const fused::Ptrs8 &o,
+                                            int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw,
+                                            hipStream_t s, double bytes) {
+  DFT_PROF_BEGIN(s);
+  if (BF) {
+    constexpr int TILE = fused::ZCfg<T, N, 5>::TILE;
+    hipLaunchKernelGGL((fused::k_zA<T, N, true>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, psi,
+                       rho, o, npairs, P, pw, tw);
+  } else {
+    constexpr int TILE = fused::ZCfg<T, N, 1>::TILE;
+    hipLaunchKernelGGL((fused::k_zA<T, N, false>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s,
+                       psi, rho, o, npairs, P, pw, tw);
+  }
+  DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "ot_zA_density_r2c", bytes);
+}
+
+template <typename T, int N>
+void launch_xmid(const fused::MidProg &pg, int ny, int P, const cplx<T> *tw, hipStream_t s, double bytes) {
+  using Cfg = fused::MidCfg<T, N>;
+  const int ntile = (P + Cfg::TILE - 1) / Cfg::TILE;
+  DFT_PROF_BEGIN(s);
+  hipLaunchKernelGGL((fused::k_xmid<T, N>), dim3((unsigned)(ny * ntile)), dim3(Cfg::BS), 0, s, pg, ny, P, ntile, tw);
+  DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "ot_xmid_fwd_kernel_inv", bytes);
+}
+
+template <typename T, int N, bool KC, bool BF>
+void launch_zAB1(const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw, hipStream_t s) {
+  constexpr int NIN = 1 + (KC ? 1 : 0) + (BF ? 5 : 0), NOUT = 1 + (KC ? 3 : 0);
+  constexpr int TILE = fused::ZCfg<T, N, (NIN > NOUT ? NIN : NOUT)>::TILE;
+  hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, a,
+                     npairs, P, pw, tw);
+}
+
+template <typename T, int N>
+void launch_zAB(bool KC, bool BF, const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw,
+                const cplx<T> *tw, hipStream_t s, double bytes) {
+  DFT_PROF_BEGIN(s);
+  if (KC && BF) launch_zAB1<T, N, true, true>(a, npairs, P, pw, tw, s);
+  else if (KC) launch_zAB1<T, N, true, false>(a, npairs, P, pw, tw, s);
+  else if (BF) launch_zAB1<T, N, false, true>(a, npairs, P, pw, tw, s);
+  else launch_zAB1<T, N, false, false>(a, npairs, P, pw, tw, s);
+  DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "ot_zAB_c2r_local_kc_bf_r2c", bytes);
+}
+
+template <typename T, int N>
+void launch_zBC(bool BF, const fused::ZBCArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw,
+                hipStream_t s, double bytes) {
+  constexpr int TILE = fused::ZCfg<T, N, 4>::TILE;
+  const dim3 grid((unsigned)((npairs + TILE - 1) / TILE));
+  DFT_PROF_BEGIN(s);
+  if (BF) hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+  else hipLaunchKernelGGL((fused::k_zBC<T, N, false>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+  DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, "ot_zBC_c2r_kc_terms_r2c", bytes);
+}
+
+template <typename T, int N>
+void launch_zC(bool step, const fused::ZCArgs &a, int64_t npairs, int P, const cplx<T> *tw, hipStream_t s,
+               double bytes) {
+  DFT_PROF_BEGIN(s);
+  if (step) {
+    constexpr int TILE = fused::ZCfg<T, N, 3>::TILE;
+    hipLaunchKernelGGL((fused::k_zC<T, N, true>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, a,
+                       npairs, P, tw);
+  } else {
+    constexpr int TILE = fused::ZCfg<T, N, 1>::TILE;
+    hipLaunchKernelGGL((fused::k_zC<T, N, false>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, a,
+                       npairs, P, tw);
+  }
+  DFT_LAUNCH_CHECK();
+  DFT_PROF_END(s, step ? "ot_zC_c2r_phase_z_fwd" : "ot_zC_c2r_accumulate", bytes);
+}
+
+fused::MidTerm mterm(int in, int kern, int deriv, double scale) {
+  fused::MidTerm t{};
+  t.in = in;
+  t.kern = kern;
+  t.deriv = deriv;
+  t.scale = scale;
+  return t;
+}
+
+// Algorithmic traffic of an x-middle program.
+template <typename T> double mid_bytes(const fused::MidProg &pg, int64_t nh) {
+  double b = (pg.nin + pg.nout) * (double)nh * sizeof(cplx<T>);
+  for (int m = 0; m < pg.nout; ++m)
+    for (int t = 0; t < pg.o[m].nterms; ++t)
+      if (pg.o[m].t[t].kern >= 0) b += (double)nh * sizeof(T);
+  return b;
+}
+
+template <typename T>
+void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *rho, cplx<T> *V, const StepIn *st) {
+  dft_ot_device *d = otf->device;
+  Plan3D<T> &p = plan_of<T>(d);
+  hipStream_t s = lib_stream();
+  const double h3N = d->step * d->step * d->step / (double)d->n;
+  const bool KC = (otf->model & DFT_OT_KC) != 0, BF = (otf->model & DFT_OT_BACKFLOW) != 0;
+  const PW<T> pw = make_pw<T>(otf, wf_mass, otf->div_epsilon, true);
+  const int64_t npairs = (int64_t)d->nx * d->ny / 2, nh = (int64_t)d->nx * d->ny * d->P, n = d->n;
+  const int P = d->P;
+  const double R = (double)n * sizeof(T), H = (double)nh * sizeof(cplx<T>);
+  fft::KernTab kt = kerntab<T>(d);
+  auto setk = [&](fused::MidProg &pg) {
+    for (int i = 0; i < 4; ++i) pg.kern[i] = kt.k[i];
+    pg.sinx = kt.sinx;
+    pg.siny = kt.siny;
+    pg.sinz = kt.sinz;
+  };
+
+  // ---- stage A: psi -> {rho, q_d, q4}, forward z and y
+  const int nA = BF ? 5 : 1;
+  fused::Ptrs8 oA{};
+  for (int i = 0; i < nA; ++i) oA.p[i] = d->slot[SP0 + i];
+  DFT_DISPATCH_N(d->nz, (launch_zA<T, N_>(BF, psi, rho, oA, npairs, P, pw, p.twz, s, 3 * R + nA * H)));
+  {
+    cplx<T> *b[5];
+    for (int i = 0; i < nA; ++i) b[i] = CS<T>(d, SP0 + i);
+    ypass<T>(p, nA, b, -1, P, s);
+  }
+  // x middle: Pi*rho, [F*rho], [U*rho, U*q_x, U*q_y, U*q_z, U*q4]
+  int nB = 0;
+  {
+    fused::MidProg pg{};
+    pg.nin = nA;
+    for (int i = 0; i < nA; ++i) pg.in[i] = d->slot[SP0 + i];
+    pg.multi = -1;
+    auto add = [&](int in, int kern) {
+      pg.o[pg.nout].nterms = 1;
+      pg.o[pg.nout].t[0] = mterm(in, kern, -1, h3N);
+      pg.o[pg.nout].out = d->slot[TMP0 + pg.nout];
+      pg.nout++;
+    };
+    add(0, K_SA);
+    if (KC) add(0, K_F);
+    if (BF) {
+      add(0, K_U);
+      for (int i = 1; i <= 4; ++i) add(i, K_U);
+    }
+    setk(pg);
+    nB = pg.nout;
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(pg, d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    cplx<T> *b[8];
+    for (int i = 0; i < nB; ++i) b[i] = CS<T>(d, TMP0 + i);
+    ypass<T>(p, nB, b, +1, P, s);
+  }
+  // ---- stage B: z^-1, local / KC / BF pointwise, z of {Q, G_d}
+  const int nQ = KC ? 4 : 1;
+  {
+    fused::ZABArgs a{};
+    for (int i = 0; i < nB; ++i) a.in.p[i] = d->slot[TMP0 + i];
+    a.out.p[0] = d->slot[Q4];
+    a.out.p[1] = d->slot[QX];
+    a.out.p[2] = d->slot[QY];
+    a.out.p[3] = d->slot[QZ];
+    a.rho = rho;
+    a.psi = psi;
+    a.rhot = d->slot[RT];
+    a.vre = d->slot[VRE];
+    a.vim = d->slot[VIM];
+    a.wx = d->slot[WX];
+    a.wy = d->slot[WY];
+    a.wz = d->slot[WZ];
+    const double bytes = nB * H + R + (BF ? 2 * R : 0) + nQ * H + R + (KC ? R : 0) + (BF ? 4 * R : 0);
+    DFT_DISPATCH_N(d->nz, (launch_zAB<T, N_>(KC, BF, a, npairs, P, pw, p.twz, s, bytes)));
+    cplx<T> *b[4] = {CS<T>(d, Q4), CS<T>(d, QX), CS<T>(d, QY), CS<T>(d, QZ)};
+    ypass<T>(p, nQ, b, -1, P, s);
+  }
+  // ---- KC: J_d = F*G_d, T1 = F*sum_d D_d G_d; then H, terms 1 and 3, BF divergence
+  if (KC) {
+    fused::MidProg pg{};
+    pg.nin = 3;
+    pg.in[0] = d->slot[QX];
+    pg.in[1] = d->slot[QY];
+    pg.in[2] = d->slot[QZ];
+    pg.nout = 4;
+    for (int dd = 0; dd < 3; ++dd) {
+      pg.o[dd].nterms = 1;
+      pg.o[dd].t[0] = mterm(dd, K_F, -1, h3N);
+      pg.o[dd].out = d->slot[TMP0 + dd];
+    }
+    pg.o[3].nterms = 3;
+    for (int dd = 0; dd < 3; ++dd) pg.o[3].t[dd] = mterm(dd, K_F, dd, h3N);
+    pg.o[3].out = d->slot[TMP3];
+    pg.multi = 3;
+    setk(pg);
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(pg, d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    cplx<T> *b[4] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2), CS<T>(d, TMP3)};
+    ypass<T>(p, 4, b, +1, P, s);
+    fused::ZBCArgs a{};
+    for (int i = 0; i < 4; ++i) a.in.p[i] = d->slot[TMP0 + i];
+    a.out.p[0] = d->slot[SH];
+    a.rho = rho;
+    a.rhot = d->slot[RT];
+    a.wx = d->slot[WX];
+    a.wy = d->slot[WY];
+    a.wz = d->slot[WZ];
+    a.vre = d->slot[VRE];
+    a.vim = d->slot[VIM];
+    const double bytes = 4 * H + 3 * R + (BF ? 4 * R : 0) + H + R + (BF ? R : 0);
+    DFT_DISPATCH_N(d->nz, (launch_zBC<T, N_>(BF, a, npairs, P, pw, p.twz, s, bytes)));
+    cplx<T> *hb[1] = {CS<T>(d, SH)};
+    ypass<T>(p, 1, hb, -1, P, s);
+  } else if (BF) {
+    StageCIn ci{rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
+    StageCOut co{nullptr, S<T>(d, VRE), S<T>(d, VIM)};
+    LAUNCH_PW("ot_stage_kc_terms_bf_div", 5, (k_stageC<T, false, true>), ci, co, pw);
+  }
+  // ---- stage C: linear terms summed in k space, z^-1, accumulate or start the step
+  {
+    fused::MidProg pg{};
+    pg.nin = KC ? 3 : 2;
+    pg.in[0] = d->slot[SP0];
+    pg.in[1] = d->slot[Q4];
+    pg.in[2] = d->slot[SH];
+    pg.nout = 1;
+    pg.multi = 0;
+    pg.o[0].nterms = KC ? 3 : 2;
+    pg.o[0].t[0] = mterm(0, K_LJ, -1, h3N);
+    pg.o[0].t[1] = mterm(1, K_SA, -1, h3N);
+    pg.o[0].t[2] = mterm(2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass));
+    pg.o[0].out = d->slot[TMP4];
+    setk(pg);
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(pg, d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    cplx<T> *b[1] = {CS<T>(d, TMP4)};
+    ypass<T>(p, 1, b, +1, P, s);
+    fused::ZCArgs a{};
+    a.in.p[0] = d->slot[TMP4];
+    a.vre = d->slot[VRE];
+    a.vim = BF ? d->slot[VIM] : nullptr;
+    double bytes = H + R + (BF ? R : 0);
+    if (!st) {
+      a.V = V;
+      bytes += 4 * R;
+    } else {
+      a.vext = st->vext;
+      a.mu = st->mu;
+      a.vavg = st->vavg;
+      a.vsave = st->vsave;
+      a.phase = st->phase;
+      a.psi_src = st->psi_src;
+      a.psi_dst = st->psi_dst;
+      a.dtr = st->dtr;
+      a.dti = st->dti;
+      bytes += (st->vext ? R : 0) + (st->vavg ? 2 * R : 0) + (st->vsave ? 2 * R : 0) + 6 * R;
+    }
+    DFT_DISPATCH_N(d->nz, (launch_zC<T, N_>(st != nullptr, a, npairs, P, p.twz, s, bytes)));
+  }
+}
+
+void ensure_step_buffers(dft_ot_device *d) {
+  const size_t bytes = (d->prec == GRID_FP32 ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)d->n;
+  if (!d->phase) d->phase = dmalloc(bytes);
+  if (!d->vsave) d->vsave = dmalloc(bytes);
+  if (!d->phase || !d->vsave) {
+    fprintf(stderr, "libdft: out of device memory for the propagation buffers\n");
+    abort();
   }
 }
 
@@ -852,6 +1099,8 @@ void free_device(dft_ot_device *d) {
   DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
   for (int i = 0; i < NSLOT; ++i) dfree(d->slot[i]);
   for (int i = 0; i < 4; ++i) dfree(d->kern[i]);
+  dfree(d->phase);
+  dfree(d->vsave);
   if (d->plan) {
     if (d->prec == GRID_FP32) {
       ((Plan3D<float> *)d->plan)->destroy();
@@ -888,6 +1137,14 @@ void check_wf(dft_ot_functional *otf, wf *gwf, const char *fn) {
   cgrid *g = gwf->grid;
   if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
     fprintf(stderr, "libdft: %s: wave function does not match the functional's grid\n", fn);
+    abort();
+  }
+}
+
+void check_c(dft_ot_functional *otf, cgrid *g, const char *fn) {
+  dft_ot_device *d = otf->device;
+  if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
+    fprintf(stderr, "libdft: %s: grid does not match the functional's grid\n", fn);
     abort();
   }
 }
@@ -1051,9 +1308,51 @@ void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact) { otf->kc_coef23 = ex
 void dft_ot_potential(dft_ot_functional *otf, cgrid *potential, wf *gwf) {
   check_wf(otf, gwf, "dft_ot_potential");
   PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
-  DFT_BY_PREC(otf->device->prec,
-              potential_impl<T>(otf, potential, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev,
-                                nullptr, nullptr, nullptr, parts, false));
+  (void)parts;
+  check_c(otf, potential, "dft_ot_potential");
+  DFT_BY_PREC(otf->device->prec, fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass,
+                                               (T *)otf->density->dev, (cplx<T> *)potential->dev, nullptr));
+}
+
+// One split-operator step with the OT potential of the wave function itself:
+// the result equals cgrid_zero(pot); [grid_real_to_complex_re(pot, vext)];
+// dft_ot_potential(otf, pot, gwf); cgrid_add(pot, -mu); grid_wf_propagate(gwf, pot, time)
+// (single-step driver pattern, doc/libdft-manual.tex:496-504).
+void dft_ot_propagate(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REAL_COMPLEX time) {
+  check_wf(otf, gwf, "dft_ot_propagate");
+  if (vext) check_r(otf, vext, "dft_ot_propagate");
+  dft_ot_device *d = otf->device;
+  ensure_step_buffers(d);
+  StepIn st{vext ? vext->dev : nullptr, mu, nullptr, nullptr, d->phase, gwf->grid->dev, gwf->grid->dev,
+            __real__ time, __imag__ time};
+  DFT_BY_PREC(d->prec, {
+    fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &st);
+    strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwf->grid->dev, (const cplx<T> *)d->phase, __real__ time,
+                   __imag__ time, gwf->mass, lib_stream());
+  });
+}
+
+// Predict-correct step as in examples/imag_time/imag_time.c:231-239:
+//   V_n = V_OT[psi_n] + V_ext - mu;  psi_p = Step(psi_n; V_n)
+//   V_p = V_OT[psi_p] + V_ext - mu;  psi_n+1 = Step(psi_n; (V_n + V_p)/2)
+// gwfp receives psi_p.
+void dft_ot_propagate_pc(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu, REAL_COMPLEX time) {
+  check_wf(otf, gwf, "dft_ot_propagate_pc");
+  check_wf(otf, gwfp, "dft_ot_propagate_pc");
+  if (vext) check_r(otf, vext, "dft_ot_propagate_pc");
+  dft_ot_device *d = otf->device;
+  ensure_step_buffers(d);
+  const void *ve = vext ? vext->dev : nullptr;
+  StepIn pred{ve, mu, nullptr, d->vsave, d->phase, gwf->grid->dev, gwfp->grid->dev, __real__ time, __imag__ time};
+  StepIn corr{ve, mu, d->vsave, nullptr, d->phase, gwf->grid->dev, gwf->grid->dev, __real__ time, __imag__ time};
+  DFT_BY_PREC(d->prec, {
+    fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &pred);
+    strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwfp->grid->dev, (const cplx<T> *)d->phase, __real__ time,
+                   __imag__ time, gwf->mass, lib_stream());
+    fused_impl<T>(otf, (const cplx<T> *)gwfp->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &corr);
+    strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwf->grid->dev, (const cplx<T> *)d->phase, __real__ time,
+                   __imag__ time, gwf->mass, lib_stream());
+  });
 }
 
 // src/ot.c:431-510

@@ -280,3 +280,75 @@ def test_energy_gradient_fd_128(oracle):
     assert worst32 < 1e-4
     for x in (g, r, g32, r32, f64, f32, w64, w32):
         x.free()
+
+
+# ---------------------------------------------------------- fused stepping --
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+@pytest.mark.parametrize("model", [0, 1, 8, 9], ids=["plain", "kc", "bf", "full"])
+def test_fused_single_step_matches_oracle(oracle, prec, model):
+    """dft_ot_propagate == zero/potential/-mu/propagate of the reference single-step driver."""
+    shape, step = (16, 32, 32), 0.5 / L.AUTOANG
+    o = oracle.OracleOT(model, shape, step, 2, 8)
+    psi0 = inputs.smooth_modulated(shape, step, o.p.rho0, 0.2)
+    vext = inputs.gaussian_well(shape, step)
+    mu0 = o.bulk("dEdRho", o.p.rho0)
+    lib = L.lib()
+    w = L.Wave(shape, step, prec)
+    f = L.Functional(model, w, 2, 8)
+    ve = L.RField(shape, step, prec)
+    ve.set(vext)
+    for dt in (-1j * 10.0 / L.AUTOFS, 5.0 / L.AUTOFS):
+        want = psi0.copy()
+        o.single_step(want, vext, mu0, dt)
+        w.set(psi0)
+        lib.dft_ot_propagate(f.h, w.h, ve.h, mu0, L.Complex.of(dt))
+        assert relerr(w.get(), want) < (1e-10 if prec == L.GRID_FP64 else 2e-5)
+        assert relerr(f.density(), np.abs(psi0) ** 2) < TOL[prec]
+    for x in (ve, f, w):
+        x.free()
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+def test_fused_pc_step_matches_oracle(oracle, prec):
+    shape, step = (32, 16, 32), 0.5 / L.AUTOANG
+    o = oracle.OracleOT(9, shape, step, 2, 8)
+    psi0 = inputs.smooth_modulated(shape, step, o.p.rho0, 0.2)
+    vext = inputs.gaussian_well(shape, step)
+    mu0 = o.bulk("dEdRho", o.p.rho0)
+    lib = L.lib()
+    w = L.Wave(shape, step, prec)
+    wp = L.Wave(shape, step, prec)
+    f = L.Functional(9, w, 2, 8)
+    ve = L.RField(shape, step, prec)
+    ve.set(vext)
+    for dt in (-1j * 10.0 / L.AUTOFS, 5.0 / L.AUTOFS):
+        want = psi0.copy()
+        o.pc_step(want, vext, mu0, dt)
+        w.set(psi0)
+        lib.dft_ot_propagate_pc(f.h, w.h, wp.h, ve.h, mu0, L.Complex.of(dt))
+        assert relerr(w.get(), want) < (1e-10 if prec == L.GRID_FP64 else 2e-5)
+    for x in (ve, f, wp, w):
+        x.free()
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+def test_fused_steps_large_grid_consistent(prec):
+    """256 x 128 x 64 (non-cubic, exercises every tile shape): fused step == grid API step."""
+    shape, step = (64, 128, 256), 0.4 / L.AUTOANG
+    lib = L.lib()
+    w1 = L.Wave(shape, step, prec)
+    w2 = L.Wave(shape, step, prec)
+    f = L.Functional(9, w1)
+    psi0 = inputs.vortex_line(shape, step, f.p.rho0)
+    w1.set(psi0), w2.set(psi0)
+    pot = L.CField(shape, step, prec)
+    mu = f.bulk("dEdRho", f.p.rho0)
+    dt = L.Complex.of(2.0 / L.AUTOFS)
+    lib.dft_ot_propagate(f.h, w1.h, None, mu, dt)
+    lib.cgrid_zero(pot.h)
+    lib.dft_ot_potential(f.h, pot.h, w2.h)
+    lib.cgrid_add(pot.h, L.Complex.of(-mu))
+    lib.grid_wf_propagate(w2.h, pot.h, dt)
+    assert relerr(w1.get(), w2.get()) < (1e-13 if prec == L.GRID_FP64 else 1e-6)
+    for x in (pot, f, w1, w2):
+        x.free()
