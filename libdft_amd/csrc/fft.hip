@@ -12,14 +12,20 @@ bool fft_size_ok(int64_t n) {
   return n == 4 || n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
 }
 
+// Twiddle table of 2n entries: [0, n) holds exp(-2 pi i m / n) rounded to T;
+// for FP32, [n, 2n) holds the rounding residual of each entry, so that the
+// kernels can multiply by the twiddle to ~1e-14 in modulus (the inverse
+// transform uses the conjugates).  For FP64 the second half is zero and unused.
 template <typename T> static cplx<T> *make_twiddles(int n) {
-  std::vector<cplx<T>> h(n);
+  std::vector<cplx<T>> h(2 * n);
   for (int m = 0; m < n; ++m) {
-    const double a = -2.0 * M_PI * (double)m / (double)n;
-    h[m] = cmk<cplx<T>>((T)std::cos(a), (T)std::sin(a));
+    const double a = 2.0 * M_PI * (double)m / (double)n, c = std::cos(a), s = -std::sin(a);
+    const T ch = (T)c, sh = (T)s;
+    h[m] = cmk<cplx<T>>(ch, sh);
+    h[n + m] = sizeof(T) == 4 ? cmk<cplx<T>>((T)(c - (double)ch), (T)(s - (double)sh)) : cmk<cplx<T>>(0, 0);
   }
-  auto *d = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n);
-  DFT_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(cplx<T>) * n, hipMemcpyHostToDevice));
+  auto *d = (cplx<T> *)dmalloc(sizeof(cplx<T>) * 2 * n);
+  DFT_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(cplx<T>) * 2 * n, hipMemcpyHostToDevice));
   return d;
 }
 

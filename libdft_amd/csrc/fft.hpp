@@ -64,7 +64,7 @@ template <typename T, int N, int LINES_FACTOR = 1> struct Tile {
   static constexpr int budget = 80 * 1024;
   static constexpr int want = 256 / Shape<N>::TPL;
   static constexpr int fit(int t) {
-    return (t > 1 && t * LINES_FACTOR * Shape<N>::PITCH * csize + N * csize > budget) ? fit(t / 2) : t;
+    return (t > 1 && t * LINES_FACTOR * Shape<N>::PITCH * csize + 2 * N * csize > budget) ? fit(t / 2) : t;
   }
   static constexpr int TILE = fit(want);
   static constexpr int THREADS = TILE * Shape<N>::TPL;
@@ -79,20 +79,39 @@ __host__ __device__ constexpr int brev(int i, int bits) {
 }
 __host__ __device__ constexpr int lg2(int n) { return n <= 1 ? 0 : 1 + lg2(n / 2); }
 
+// cos and sin of 2 pi m / 16 (m = 0..7), the constants of the in-register DFTs.
+constexpr double kC16[8] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+                            0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674};
+constexpr double kS16[8] = {0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674,
+                            1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977};
+
+// v * (hi + lo) for a twiddle carried as an FP32 pair: the rounded value plus
+// its rounding residual.  A twiddle rounded to FP32 alone has a modulus error
+// of up to ~6e-8 that is the same on every call, so each Fourier mode picks up
+// a fixed gain error per transform; over many split-operator steps that adds
+// up linearly in the norm and kinetic energy.  With the residual the modulus is
+// exact to ~1e-14 and only the (unbiased) rounding of the products remains.
+__device__ inline float2 cmul_hl(float2 v, float2 hi, float2 lo) {
+  const float cr = v.x * lo.x - v.y * lo.y;
+  const float ci = v.x * lo.y + v.y * lo.x;
+  return make_float2(__builtin_fmaf(v.x, hi.x, __builtin_fmaf(-v.y, hi.y, cr)),
+                     __builtin_fmaf(v.x, hi.y, __builtin_fmaf(v.y, hi.x, ci)));
+}
+
 // v * exp(DIR * 2 pi i m / 16), DIR = -1 forward, +1 inverse.
 template <int M, int DIR, typename C> __device__ inline C tw16(C a) {
   using T = decltype(a.x);
-  constexpr double c16[8] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
-                             0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674};
-  constexpr double s16[8] = {0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674,
-                             1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977};
   if constexpr (M == 0) {
     return a;
   } else if constexpr (M == 4) {
     // multiply by DIR * i
     return DIR > 0 ? cmk<C>(-a.y, a.x) : cmk<C>(a.y, -a.x);
+  } else if constexpr (sizeof(T) == 4) {
+    constexpr float c = (float)kC16[M], s = (float)kS16[M];
+    constexpr float cl = (float)(kC16[M] - (double)c), sl = (float)(kS16[M] - (double)s);
+    return cmul_hl(a, cmk<C>(c, DIR * s), cmk<C>(cl, DIR * sl));
   } else {
-    const T c = (T)c16[M], s = (T)(DIR * s16[M]);
+    const T c = (T)kC16[M], s = (T)(DIR * kS16[M]);
     return cmk<C>(a.x * c - a.y * s, a.x * s + a.y * c);
   }
 }
@@ -150,9 +169,17 @@ __device__ inline void stage(cplx<T> *L, const cplx<T> *tw, int tl, bool active)
         const int k = j & (Ns - 1);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          C w = tw[k * r * (N / (Ns * R))];
+          // table: forward twiddles in [0, N); for FP32 their residuals in [N, 2N)
+          const int ti = k * r * (N / (Ns * R));
+          C w = tw[ti];
           if (DIR > 0) w.y = -w.y;
-          v[p][r] = cmul(v[p][r], w);
+          if constexpr (sizeof(T) == 4) {
+            C wl = tw[N + ti];
+            if (DIR > 0) wl.y = -wl.y;
+            v[p][r] = cmul_hl(v[p][r], w, wl);
+          } else {
+            v[p][r] = cmul(v[p][r], w);
+          }
         }
       }
       dft<R, DIR>(v[p]);
@@ -181,7 +208,7 @@ __device__ inline void line_fft(cplx<T> *L, const cplx<T> *tw, int tl, bool acti
 }
 
 template <typename T, int N> __device__ inline void load_twiddles(cplx<T> *s_tw, const cplx<T> *g_tw) {
-  for (int i = threadIdx.x; i < N; i += blockDim.x) s_tw[i] = g_tw[i];
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) s_tw[i] = g_tw[i];
 }
 
 // e^{-i V dt/2} for complex V and complex dt.

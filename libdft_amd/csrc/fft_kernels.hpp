@@ -13,7 +13,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
   using C = cplx<T>;
   constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
   __shared__ C s_line[TILE * PITCH];
-  __shared__ C s_tw[N];
+  __shared__ C s_tw[2 * N];
   const T *__restrict__ in = (const T *)b.in[blockIdx.y];
   C *__restrict__ out = (C *)b.out[blockIdx.y];
   const int64_t p0 = (int64_t)blockIdx.x * TILE;
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__((Tile<T, N, 2>::THREADS))
   constexpr int RP = N / 2 + 1;
   __shared__ C s_line[TILE * PITCH];
   __shared__ C s_raw[TILE * 2 * RP];
-  __shared__ C s_tw[N];
+  __shared__ C s_tw[2 * N];
   const C *__restrict__ in = (const C *)b.in[blockIdx.y];
   T *__restrict__ out = (T *)b.out[blockIdx.y];
   const int64_t p0 = (int64_t)blockIdx.x * TILE;
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
   using C = cplx<T>;
   constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
   __shared__ C s_line[TILE * PITCH];
-  __shared__ C s_tw[N];
+  __shared__ C s_tw[2 * N];
   const C *__restrict__ in = (const C *)b.in[blockIdx.y];
   C *__restrict__ out = (C *)b.out[blockIdx.y];
   const int o = blockIdx.x / ntile, ct = blockIdx.x - o * ntile, c0 = ct * TILE;
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
   using C = cplx<T>;
   constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
   __shared__ C s_line[TILE * PITCH];
-  __shared__ C s_tw[N];
+  __shared__ C s_tw[2 * N];
   const Combine cb = cs.c[blockIdx.y];
   C *__restrict__ out = (C *)b.out[blockIdx.y];
   const int o = blockIdx.x / ntile, ct = blockIdx.x - o * ntile, c0 = ct * TILE;
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
   using C = cplx<T>;
   constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
   __shared__ C s_line[TILE * PITCH];
-  __shared__ C s_tw[N];
+  __shared__ C s_tw[2 * N];
   __shared__ double s_red[BS];
   const int64_t l0 = (int64_t)blockIdx.x * TILE;
   const int nl = (int)((nlines - l0) < TILE ? (nlines - l0) : TILE);
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__((Tile<T, N>::THREADS))
   using C = cplx<T>;
   constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
   __shared__ C s_line[TILE * PITCH];
-  __shared__ C s_tw[N];
+  __shared__ C s_tw[2 * N];
   const int o = blockIdx.x / ntile, ct = blockIdx.x - o * ntile, c0 = ct * TILE;
   const int ncol = (nz - c0) < TILE ? (nz - c0) : TILE;
   const int64_t es = (int64_t)ny * nz, base = (int64_t)o * nz + c0;

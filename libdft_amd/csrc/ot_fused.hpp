@@ -31,7 +31,7 @@ template <typename T, int N, int LINES> struct ZCfg {
   static constexpr int csize = 2 * sizeof(T);
   static constexpr int PITCH = Shape<N>::PITCH;
   static constexpr int fit(int t) {
-    return (t > 1 && t * LINES * PITCH * csize + N * csize > 80 * 1024) ? fit(t / 2) : t;
+    return (t > 1 && t * LINES * PITCH * csize + 2 * N * csize > 80 * 1024) ? fit(t / 2) : t;
   }
   static constexpr int TILE = fit(8);
   static constexpr int BS = 256;
@@ -42,7 +42,7 @@ template <typename T, int N> struct MidCfg {
   static constexpr int csize = 2 * sizeof(T);
   static constexpr int PITCH = Shape<N>::PITCH;
   static constexpr int fit(int t) {
-    return (t > 1 && (3 * t * PITCH * csize + N * csize > 80 * 1024 || t * Shape<N>::TPL > 256)) ? fit(t / 2) : t;
+    return (t > 1 && (3 * t * PITCH * csize + 2 * N * csize > 80 * 1024 || t * Shape<N>::TPL > 256)) ? fit(t / 2) : t;
   }
   static constexpr int TILE = fit(16);
   static constexpr int BS = TILE * Shape<N>::TPL;
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(256)
   using Cfg = ZCfg<T, N, NF>;
   constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
   __shared__ cplx<T> L[NF * TILE * PITCH];
-  __shared__ cplx<T> s_tw[N];
+  __shared__ cplx<T> s_tw[2 * N];
   const int64_t p0 = (int64_t)blockIdx.x * TILE;
   const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
   fft::load_twiddles<T, N>(s_tw, g_tw);
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS))
   __shared__ cplx<T> A[TILE * PITCH];
   __shared__ cplx<T> B[TILE * PITCH];
   __shared__ cplx<T> Cc[TILE * PITCH];
-  __shared__ cplx<T> s_tw[N];
+  __shared__ cplx<T> s_tw[2 * N];
   // consecutive column tiles of one ky on one XCD, back to back (shared L2 lines)
   int b = blockIdx.x;
   const int nblk = gridDim.x;
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(256) k_zAB(ZABArgs a, int64_t npairs, int P, P
   constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
   constexpr int IA = KC ? 2 : 1;  // first backflow input
   __shared__ cplx<T> L[NL * TILE * PITCH];
-  __shared__ cplx<T> s_tw[N];
+  __shared__ cplx<T> s_tw[2 * N];
   const int64_t p0 = (int64_t)blockIdx.x * TILE;
   const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
   fft::load_twiddles<T, N>(s_tw, g_tw);
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256) k_zBC(ZBCArgs a, int64_t npairs, int P, P
   using Cfg = ZCfg<T, N, NIN>;
   constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
   __shared__ cplx<T> L[NIN * TILE * PITCH];
-  __shared__ cplx<T> s_tw[N];
+  __shared__ cplx<T> s_tw[2 * N];
   const int64_t p0 = (int64_t)blockIdx.x * TILE;
   const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
   fft::load_twiddles<T, N>(s_tw, g_tw);
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, con
   using Cfg = ZCfg<T, N, NL>;
   constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
   __shared__ cplx<T> L[NL * TILE * PITCH];
-  __shared__ cplx<T> s_tw[N];
+  __shared__ cplx<T> s_tw[2 * N];
   const int64_t p0 = (int64_t)blockIdx.x * TILE;
   const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
   fft::load_twiddles<T, N>(s_tw, g_tw);
