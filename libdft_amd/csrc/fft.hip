@@ -67,22 +67,22 @@ namespace {
 
 template <typename T, int N>
 void launch_zr2c(const Plan3D<T> &p, const fft::Batch &b, int nb, hipStream_t s) {
-  using TL = fft::Tile<T, N>;
+  using G = rf::LineCfg<T, N>;
   const int64_t npairs = (int64_t)p.nx * p.ny / 2;
-  dim3 grid((unsigned)((npairs + TL::TILE - 1) / TL::TILE), nb);
+  dim3 grid((unsigned)((npairs + G::LINES - 1) / G::LINES), nb);
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zr2c<T, N>), grid, dim3(TL::THREADS), 0, s, b, npairs, p.P, p.twz);
+  hipLaunchKernelGGL((fft::k_zr2c<T, N>), grid, dim3(G::BS), 0, s, b, npairs, p.P, p.twz);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_z_r2c", (double)nb * (p.n() * sizeof(T) + p.nhalf() * sizeof(cplx<T>)));
 }
 
 template <typename T, int N>
 void launch_zc2r(const Plan3D<T> &p, const fft::Batch &b, int nb, T scale, hipStream_t s) {
-  using TL = fft::Tile<T, N, 2>;
+  using G = rf::LineCfg<T, N>;
   const int64_t npairs = (int64_t)p.nx * p.ny / 2;
-  dim3 grid((unsigned)((npairs + TL::TILE - 1) / TL::TILE), nb);
+  dim3 grid((unsigned)((npairs + G::LINES - 1) / G::LINES), nb);
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zc2r<T, N>), grid, dim3(TL::THREADS), 0, s, b, npairs, p.P, p.twz, scale);
+  hipLaunchKernelGGL((fft::k_zc2r<T, N>), grid, dim3(G::BS), 0, s, b, npairs, p.P, p.twz, scale);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_z_c2r", (double)nb * (p.n() * sizeof(T) + p.nhalf() * sizeof(cplx<T>)));
 }
@@ -90,8 +90,8 @@ void launch_zc2r(const Plan3D<T> &p, const fft::Batch &b, int nb, T scale, hipSt
 // Strided pass along y (axis 1) or x (axis 0) over columns [0, cols).
 template <typename T, int N, int DIR>
 void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, int cols, T scale, hipStream_t s) {
-  using TL = fft::Tile<T, N>;
-  const int ntile = (cols + TL::TILE - 1) / TL::TILE;
+  using G = rf::ColCfg<T, N>;
+  const int ntile = (cols + G::COLS - 1) / G::COLS;
   int64_t es, os;
   int outer;
   if (axis == 1) {
@@ -106,7 +106,7 @@ void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, i
   dim3 grid((unsigned)(outer * ntile), nb);
   const cplx<T> *tw = axis == 1 ? p.twy : p.twx;
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_strided<T, N, DIR>), grid, dim3(TL::THREADS), 0, s, b, cols, es, os, ntile, tw, scale);
+  hipLaunchKernelGGL((fft::k_strided<T, N, DIR>), grid, dim3(G::BS), 0, s, b, cols, es, os, ntile, tw, scale);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, axis == 1 ? "fft_y_c2c" : "fft_x_c2c",
                2.0 * nb * (double)p.nx * p.ny * cols * sizeof(cplx<T>));
@@ -115,8 +115,8 @@ void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, i
 template <typename T, int N>
 void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::CombineSet &cs, const fft::KernTab &kt,
                      int nb, hipStream_t s) {
-  using TL = fft::Tile<T, N>;
-  const int ntile = (p.P + TL::TILE - 1) / TL::TILE;
+  using G = rf::ColCfg<T, N>;
+  const int ntile = (p.P + G::COLS - 1) / G::COLS;
   dim3 grid((unsigned)(p.ny * ntile), nb);
   // traffic: each term's spectrum and kernel, plus the output
   double per = 0;
@@ -126,7 +126,7 @@ void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::Combine
       per += sizeof(cplx<T>) + (cs.c[i].t[t].kern >= 0 ? sizeof(T) : 0);
   }
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_xcombine_inv<T, N>), grid, dim3(TL::THREADS), 0, s, b, cs, kt, p.ny, p.P, ntile,
+  hipLaunchKernelGGL((fft::k_xcombine_inv<T, N>), grid, dim3(G::BS), 0, s, b, cs, kt, p.ny, p.P, ntile,
                      p.twx);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_x_combine_c2c", per * (double)p.nhalf());
@@ -135,11 +135,11 @@ void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::Combine
 template <typename T, int N, int DIR>
 void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cplx<T> *pot, T dtr, T dti, T scale,
                  double *partial, hipStream_t s) {
-  using TL = fft::Tile<T, N>;
+  using G = rf::LineCfg<T, N>;
   const int64_t nlines = (int64_t)p.nx * p.ny;
-  dim3 grid((unsigned)((nlines + TL::TILE - 1) / TL::TILE));
+  dim3 grid((unsigned)((nlines + G::LINES - 1) / G::LINES));
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, DIR>), grid, dim3(TL::THREADS), 0, s, src, dst, pot, nlines, p.twz, dtr,
+  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, DIR>), grid, dim3(G::BS), 0, s, src, dst, pot, nlines, p.twz, dtr,
                      dti, scale, partial);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, DIR < 0 ? "fft_z_c2c_vphase_fwd" : "fft_z_c2c_vphase_inv",
@@ -148,11 +148,11 @@ void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cpl
 
 template <typename T, int N>
 void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dti, hipStream_t s) {
-  using TL = fft::Tile<T, N>;
-  const int ntile = (p.nz + TL::TILE - 1) / TL::TILE;
+  using G = rf::ColCfg<T, N>;
+  const int ntile = (p.nz + G::COLS - 1) / G::COLS;
   dim3 grid((unsigned)(p.ny * ntile));
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_xkin<T, N>), grid, dim3(TL::THREADS), 0, s, psi, p.ny, p.nz, ntile, p.twx, cx, cy, cz,
+  hipLaunchKernelGGL((fft::k_xkin<T, N>), grid, dim3(G::BS), 0, s, psi, p.ny, p.nz, ntile, p.twx, cx, cy, cz,
                      dtr, dti);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_x_kinetic_c2c", 2.0 * (double)p.n() * sizeof(cplx<T>));
@@ -160,26 +160,26 @@ void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dt
 
 template <typename T, int N>
 void launch_zc2c_phase_inv(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, T scale, hipStream_t s) {
-  using TL = fft::Tile<T, N>;
+  using G = rf::LineCfg<T, N>;
   const int64_t nlines = (int64_t)p.nx * p.ny;
-  dim3 grid((unsigned)((nlines + TL::TILE - 1) / TL::TILE));
+  dim3 grid((unsigned)((nlines + G::LINES - 1) / G::LINES));
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, +1, true>), grid, dim3(TL::THREADS), 0, s, psi, psi, phase, nlines,
+  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, +1, true>), grid, dim3(G::BS), 0, s, psi, psi, phase, nlines,
                      p.twz, (T)0, (T)0, scale, (double *)nullptr);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_z_c2c_phase_inv", 3.0 * (double)p.n() * sizeof(cplx<T>));
 }
 
-template <typename T> int zc2c_tile(int nz) {
+template <typename T> int zc2c_lines(int nz) {
   int t = 1;
-  DFT_DISPATCH_N(nz, (t = fft::Tile<T, N_>::TILE));
+  DFT_DISPATCH_N(nz, (t = rf::LineCfg<T, N_>::LINES));
   return t;
 }
 
 }  // namespace
 
 int64_t strang_partials(int nx, int ny, int nz, int prec) {
-  const int t = prec == GRID_FP32 ? zc2c_tile<float>(nz) : zc2c_tile<double>(nz);
+  const int t = prec == GRID_FP32 ? zc2c_lines<float>(nz) : zc2c_lines<double>(nz);
   return ((int64_t)nx * ny + t - 1) / t;
 }
 

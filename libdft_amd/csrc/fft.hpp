@@ -1,10 +1,8 @@
-// Batched 3-D FFT passes for gfx950, hand-written (no rocFFT / hipFFT).
+// Batched 3-D FFT building blocks for gfx950, hand-written (no rocFFT / hipFFT).
 //
-// A 3-D transform is three passes, one per axis.  Each pass stages a tile of
-// whole lines in LDS and runs a Stockham autosort FFT on them.  Each stage is
-// a radix-R butterfly held in VGPRs.  Global <-> LDS copies use a coalesced
-// thread mapping.  The butterflies use a line-fastest mapping over an odd
-// LDS pitch, so lanes of one wavefront hit different banks.
+// A 3-D transform is three passes, one per axis; each pass transforms whole
+// lines held in registers (rfft.hpp) with a Stockham autosort radix plan per
+// length (below) and in-register radix-2..16 DFTs.
 //
 //  * z pass, real -> half spectrum: two adjacent real z-lines are packed
 //    as one complex line (a + ib), transformed, and split with
@@ -17,8 +15,8 @@
 //    out(k) = sum_t s_t K_t(k) D_t(k) in_t(k), with K_t a real kernel
 //    spectrum and D_t = i sin(k_d h)/h, the symbol of the periodic central
 //    difference.
-// Twiddles are computed in FP64 on the host and rounded once to the field
-// precision.  Forward transforms are unnormalised (e^{-i}).
+// Twiddles are computed in FP64 on the host; FP32 carries each as a rounded
+// value plus its residual.  Forward transforms are unnormalised (e^{-i}).
 #pragma once
 #include "common.hpp"
 
@@ -40,35 +38,6 @@
 
 namespace dft {
 namespace fft {
-
-// ------------------------------------------------------------ radix plans --
-template <int N> struct Plan;
-template <> struct Plan<4> { static constexpr int S = 1, R0 = 4, R1 = 1, R2 = 1; };
-template <> struct Plan<8> { static constexpr int S = 1, R0 = 8, R1 = 1, R2 = 1; };
-template <> struct Plan<16> { static constexpr int S = 1, R0 = 16, R1 = 1, R2 = 1; };
-template <> struct Plan<32> { static constexpr int S = 2, R0 = 8, R1 = 4, R2 = 1; };
-template <> struct Plan<64> { static constexpr int S = 2, R0 = 8, R1 = 8, R2 = 1; };
-template <> struct Plan<128> { static constexpr int S = 2, R0 = 16, R1 = 8, R2 = 1; };
-template <> struct Plan<256> { static constexpr int S = 2, R0 = 16, R1 = 16, R2 = 1; };
-template <> struct Plan<512> { static constexpr int S = 3, R0 = 8, R1 = 8, R2 = 8; };
-template <> struct Plan<1024> { static constexpr int S = 3, R0 = 16, R1 = 8, R2 = 8; };
-
-// Threads cooperating on one line, LDS pitch (complex elements) and lines per tile.
-template <int N> struct Shape {
-  static constexpr int TPL = N / 16 < 1 ? 1 : (N / 16 > 16 ? 16 : N / 16);
-  static constexpr int PITCH = N + 4;
-};
-// Lines per block: 256 threads when the tile fits in `budget` bytes of LDS.
-template <typename T, int N, int LINES_FACTOR = 1> struct Tile {
-  static constexpr int csize = 2 * sizeof(T);
-  static constexpr int budget = 80 * 1024;
-  static constexpr int want = 256 / Shape<N>::TPL;
-  static constexpr int fit(int t) {
-    return (t > 1 && t * LINES_FACTOR * Shape<N>::PITCH * csize + 2 * N * csize > budget) ? fit(t / 2) : t;
-  }
-  static constexpr int TILE = fit(want);
-  static constexpr int THREADS = TILE * Shape<N>::TPL;
-};
 
 // ------------------------------------------------- in-register small DFTs --
 __host__ __device__ constexpr int brev(int i, int bits) {
@@ -149,77 +118,24 @@ template <int R, int DIR, typename C> __device__ inline void dft(C *v) {
   bfly_levels<R, DIR, 2>(v);
 }
 
-// ---------------------------------------------------- Stockham LDS stages --
-// One radix-R stage over a line in LDS.  Every thread of the block must call
-// it (it contains barriers); `active` masks threads whose line is padding.
-template <typename T, int N, int R, int Ns, int DIR>
-__device__ inline void stage(cplx<T> *L, const cplx<T> *tw, int tl, bool active) {
-  using C = cplx<T>;
-  constexpr int TPL = Shape<N>::TPL;
-  constexpr int NB = N / R;
-  constexpr int PER = (NB + TPL - 1) / TPL;
-  C v[PER][R];
-  if (active) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int j = tl + p * TPL;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[p][r] = L[j + r * NB];
-      if constexpr (Ns > 1) {
-        const int k = j & (Ns - 1);
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          // table: forward twiddles in [0, N); for FP32 their residuals in [N, 2N)
-          const int ti = k * r * (N / (Ns * R));
-          C w = tw[ti];
-          if (DIR > 0) w.y = -w.y;
-          if constexpr (sizeof(T) == 4) {
-            C wl = tw[N + ti];
-            if (DIR > 0) wl.y = -wl.y;
-            v[p][r] = cmul_hl(v[p][r], w, wl);
-          } else {
-            v[p][r] = cmul(v[p][r], w);
-          }
-        }
-      }
-      dft<R, DIR>(v[p]);
-    }
-  }
-  __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int j = tl + p * TPL;
-      const int idxD = (j / Ns) * Ns * R + (j & (Ns - 1));
-#pragma unroll
-      for (int r = 0; r < R; ++r) L[idxD + r * Ns] = v[p][r];
-    }
-  }
-  __syncthreads();
-}
-
-// Full line FFT (all stages).  Data in natural order before and after.
-template <typename T, int N, int DIR>
-__device__ inline void line_fft(cplx<T> *L, const cplx<T> *tw, int tl, bool active) {
-  using P = Plan<N>;
-  stage<T, N, P::R0, 1, DIR>(L, tw, tl, active);
-  if constexpr (P::S > 1) stage<T, N, P::R1, P::R0, DIR>(L, tw, tl, active);
-  if constexpr (P::S > 2) stage<T, N, P::R2, P::R0 * P::R1, DIR>(L, tw, tl, active);
-}
-
-template <typename T, int N> __device__ inline void load_twiddles(cplx<T> *s_tw, const cplx<T> *g_tw) {
-  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) s_tw[i] = g_tw[i];
+// m (cos p, sin p) with m = e^a, evaluated in FP64 and rounded once to T.
+// A phase factor is applied to the same mode or point every step; evaluated
+// in FP32, its modulus error is systematic (~3e-8 here) and the norm drifts by
+// that much per step, while a correctly rounded FP64 value only leaves the
+// unbiased error of the final rounding.
+template <typename T> __device__ inline cplx<T> unit_phase(double a, double p) {
+  double s, c;
+  sincos(p, &s, &c);
+  const double m = exp(a);
+  return cmk<cplx<T>>((T)(m * c), (T)(m * s));
 }
 
 // e^{-i V dt/2} for complex V and complex dt.
 template <typename T>
 __device__ inline cplx<T> vphase(cplx<T> v, T dtr, T dti) {
-  const T a = (T)0.5 * (v.x * dti + v.y * dtr);
-  const T ph = (T)-0.5 * (v.x * dtr - v.y * dti);
-  T s, c;
-  dsincos(ph, &s, &c);
-  const T m = dexp(a);
-  return cmk<cplx<T>>(m * c, m * s);
+  const double a = 0.5 * ((double)v.x * dti + (double)v.y * dtr);
+  const double ph = -0.5 * ((double)v.x * dtr - (double)v.y * dti);
+  return unit_phase<T>(a, ph);
 }
 
 // -------------------------------------------------------- pass parameters --

@@ -1,251 +1,321 @@
-// FFT pass kernels (see fft.hpp for the scheme).  Included by fft.hip only.
+// FFT pass kernels on the register-resident line transforms of rfft.hpp.
+// Included by fft.hip (and, for the pair helpers, ot_fused.hpp).
 #pragma once
-#include "fft.hpp"
+#include "rfft.hpp"
 
 namespace dft {
 namespace fft {
 
+using rf::ColCfg;
+using rf::gld;
+using rf::gst;
+using rf::lde;
+using rf::ste;
+using rf::LineCfg;
+using rf::Regs;
+using rf::RP;
+
+// Half spectrum of line `which` of a packed pair from Z = FFT(a + ib):
+// A(k) = (Z(k) + Z*(N-k))/2, B(k) = (Z(k) - Z*(N-k))/2i.
+template <typename C> __device__ __forceinline__ C split(C z, C zm, int which) {
+  using T = decltype(z.x);
+  return which == 0 ? cmk<C>((T)0.5 * (z.x + zm.x), (T)0.5 * (z.y - zm.y))
+                    : cmk<C>((T)0.5 * (z.y + zm.y), (T)0.5 * (zm.x - z.x));
+}
+
+// Split the spectrum of a packed pair held in v (decimated layout of plan
+// RP<N, EM>) and store the half spectra (k = 0..N/2) of rows 2p and 2p+1 of
+// the half-spectrum field `out` (row length P; skipped when !act).  X is the
+// pair's LDS exchange buffer.
+template <typename T, int N, int EM, class SYNC>
+__device__ __forceinline__ void split_store(const Regs<T, N, EM> &v, cplx<T> *X, int t, void *out,
+                                            uint32_t p, int P, bool act) {
+  using C = cplx<T>;
+  constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL;
+#pragma unroll
+  for (int i = 0; i < E; ++i) X[rf::pad(t + TPL * i)] = v[i];
+  SYNC::sync();
+  const uint32_t ea = p * 2u * (uint32_t)P + (uint32_t)t, eb = ea + (uint32_t)P;  // rows 2p, 2p+1 at point t
+  // points k = t + TPL*i < N/2 for i < E/2; k = N/2 is point i = E/2 of thread 0
+  C zm[E / 2 + 1];
+#pragma unroll
+  for (int i = 0; i <= E / 2; ++i) zm[i] = X[rf::pad((N - t - TPL * i) & (N - 1))];
+  SYNC::sync();
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E / 2; ++i) {
+      ste<C>(out, ea, split(v[i], zm[i], 0), TPL * i);
+      ste<C>(out, eb, split(v[i], zm[i], 1), TPL * i);
+    }
+    if (t == 0) {
+      ste<C>(out, ea, split(v[E / 2], zm[E / 2], 0), N / 2);
+      ste<C>(out, eb, split(v[E / 2], zm[E / 2], 1), N / 2);
+    }
+  }
+}
+
+// Packed pair Z(m) = A(m) + i B(m) from rows 2p, 2p+1 of a half-spectrum field
+// (Hermitian extension for m > N/2).
+template <typename T, int N, int EM>
+__device__ __forceinline__ void load_merge(Regs<T, N, EM> &v, const void *in, uint32_t p, int P, int t) {
+  using C = cplx<T>;
+  constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL;
+  // point m = t + TPL*i: row index m for i < E/2, N - m = (N - t) - TPL*i for i >= E/2
+  const uint32_t ra = p * 2u * (uint32_t)P + rf::opaque_zero(), rb = ra + (uint32_t)P;
+  const uint32_t fa = ra + (uint32_t)t, fb = rb + (uint32_t)t;
+  const uint32_t ma = ra + (uint32_t)(N - t), mb = rb + (uint32_t)(N - t);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    C a, b;
+    if (i < E / 2) {
+      a = lde<C>(in, fa, TPL * i);
+      b = lde<C>(in, fb, TPL * i);
+    } else {
+      a = lde<C>(in, ma, -TPL * i);
+      b = lde<C>(in, mb, -TPL * i);
+      if (i > E / 2 || t > 0) {  // m > N/2: conjugate of the mirrored point
+        a.y = -a.y;
+        b.y = -b.y;
+      }
+    }
+    v[i] = cmk<C>(a.x - b.y, a.y + b.x);
+  }
+}
+
 // ------------------------------------------------ z pass: real -> half spec --
 // Pairs of adjacent real z-lines (2p, 2p+1) -> half spectra rows 2p, 2p+1.
 template <typename T, int N>
-__global__ void __launch_bounds__((Tile<T, N>::THREADS))
-    k_zr2c(Batch b, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
   using C = cplx<T>;
-  constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
-  __shared__ C s_line[TILE * PITCH];
+  using G = LineCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
+  __shared__ C s_x[G::LINES * LP];
   __shared__ C s_tw[2 * N];
-  const T *__restrict__ in = (const T *)b.in[blockIdx.y];
-  C *__restrict__ out = (C *)b.out[blockIdx.y];
-  const int64_t p0 = (int64_t)blockIdx.x * TILE;
-  const int npair = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
-  load_twiddles<T, N>(s_tw, g_tw);
-  const T *src = in + p0 * 2 * N;
-  for (int e = threadIdx.x; e < npair * 2 * N; e += BS) {
-    const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, el = w - which * N;
-    T *dst = (T *)&s_line[pr * PITCH + el];
-    dst[which] = src[e];
-  }
+  const void *in = b.in[blockIdx.y];
+  void *out = b.out[blockIdx.y];
+  const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
+  const bool act = p0 < npairs;
+  const uint32_t p = act ? p0 : 0;  // out-of-range lanes read pair 0 and store nothing
+  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  const uint32_t e0 = p * 2u * N + (uint32_t)t;
+  C v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = cmk<C>(lde<T>(in, e0, TPL * i), lde<T>(in, e0, N + TPL * i));
   __syncthreads();
-  const int line = threadIdx.x % TILE, tl = threadIdx.x / TILE;
-  line_fft<T, N, -1>(&s_line[line * PITCH], s_tw, tl, line < npair);
-  C *dst = out + p0 * 2 * P;
-  for (int e = threadIdx.x; e < npair * 2 * P; e += BS) {
-    const int pr = e / (2 * P), w = e - pr * 2 * P, which = w / P, k = w - which * P;
-    const C z = s_line[pr * PITCH + k], zm = s_line[pr * PITCH + ((N - k) & (N - 1))];
-    C r;
-    if (which == 0) r = cmk<C>((T)0.5 * (z.x + zm.x), (T)0.5 * (z.y - zm.y));
-    else r = cmk<C>((T)0.5 * (z.y + zm.y), (T)0.5 * (zm.x - z.x));
-    dst[e] = r;
-  }
+  rf::rfft<T, N, G::EM, -1, rf::WaveSync>(v, s_x + ln * LP, s_tw, t);
+  split_store<T, N, G::EM, rf::WaveSync>(v, s_x + ln * LP, t, out, p, P, act);
 }
 
 // ------------------------------------------------ z pass: half spec -> real --
 template <typename T, int N>
-__global__ void __launch_bounds__((Tile<T, N, 2>::THREADS))
+__global__ void __launch_bounds__(256)
     k_zc2r(Batch b, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw, T scale) {
   using C = cplx<T>;
-  constexpr int TILE = Tile<T, N, 2>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N, 2>::THREADS;
-  constexpr int RP = N / 2 + 1;
-  __shared__ C s_line[TILE * PITCH];
-  __shared__ C s_raw[TILE * 2 * RP];
+  using G = LineCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
+  __shared__ C s_x[G::LINES * LP];
   __shared__ C s_tw[2 * N];
-  const C *__restrict__ in = (const C *)b.in[blockIdx.y];
-  T *__restrict__ out = (T *)b.out[blockIdx.y];
-  const int64_t p0 = (int64_t)blockIdx.x * TILE;
-  const int npair = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
-  load_twiddles<T, N>(s_tw, g_tw);
-  const C *src = in + p0 * 2 * P;
-  for (int e = threadIdx.x; e < npair * 2 * P; e += BS) {
-    const int pr = e / (2 * P), w = e - pr * 2 * P, which = w / P, k = w - which * P;
-    s_raw[(pr * 2 + which) * RP + k] = src[e];
-  }
+  const void *in = b.in[blockIdx.y];
+  void *out = b.out[blockIdx.y];
+  const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
+  const bool act = p0 < npairs;
+  const uint32_t p = act ? p0 : 0;
+  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  C v[E];
+  load_merge<T, N, G::EM>(v, in, p, P, t);
   __syncthreads();
-  for (int m = threadIdx.x; m < npair * N; m += BS) {
-    const int pr = m / N, idx = m - pr * N;
-    C a, bb;
-    if (idx <= N / 2) {
-      a = s_raw[(pr * 2) * RP + idx];
-      bb = s_raw[(pr * 2 + 1) * RP + idx];
-    } else {
-      a = cconj(s_raw[(pr * 2) * RP + N - idx]);
-      bb = cconj(s_raw[(pr * 2 + 1) * RP + N - idx]);
+  rf::rfft<T, N, G::EM, +1, rf::WaveSync>(v, s_x + ln * LP, s_tw, t);
+  if (act) {
+    const uint32_t e0 = p * 2u * N + (uint32_t)t;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      ste<T>(out, e0, scale * v[i].x, TPL * i);
+      ste<T>(out, e0, scale * v[i].y, N + TPL * i);
     }
-    s_line[pr * PITCH + idx] = cmk<C>(a.x - bb.y, a.y + bb.x);
-  }
-  __syncthreads();
-  const int line = threadIdx.x % TILE, tl = threadIdx.x / TILE;
-  line_fft<T, N, +1>(&s_line[line * PITCH], s_tw, tl, line < npair);
-  T *dst = out + p0 * 2 * N;
-  for (int e = threadIdx.x; e < npair * 2 * N; e += BS) {
-    const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, el = w - which * N;
-    const C z = s_line[pr * PITCH + el];
-    dst[e] = scale * (which ? z.y : z.x);
   }
 }
 
 // ------------------------------------------- strided c2c pass (y or x axis) --
 // Line (o, c): element e at o*os + e*es + c, c in [0, P).  blockIdx.x = o*ntile + tile.
 template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((Tile<T, N>::THREADS))
+__global__ void __launch_bounds__((ColCfg<T, N>::BS))
     k_strided(Batch b, int P, int64_t es, int64_t os, int ntile, const cplx<T> *__restrict__ g_tw, T scale) {
   using C = cplx<T>;
-  constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
-  __shared__ C s_line[TILE * PITCH];
+  using G = ColCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
+  __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
-  const C *__restrict__ in = (const C *)b.in[blockIdx.y];
-  C *__restrict__ out = (C *)b.out[blockIdx.y];
-  const int o = blockIdx.x / ntile, ct = blockIdx.x - o * ntile, c0 = ct * TILE;
-  const int ncol = (P - c0) < TILE ? (P - c0) : TILE;
-  load_twiddles<T, N>(s_tw, g_tw);
-  const int64_t base = (int64_t)o * os + c0;
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col < ncol) s_line[col * PITCH + el] = in[base + el * es + col];
-  }
+  const void *in = b.in[blockIdx.y];
+  void *out = b.out[blockIdx.y];
+  const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
+  const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
+  const bool act = c0 + col < P;
+  const int cc = act ? c0 + col : P - 1;  // out-of-range lanes read the last column and store nothing
+  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  const uint32_t ob = (uint32_t)((o * os + cc + t * es) * (int64_t)sizeof(C));
+  const uint32_t st = (uint32_t)(TPL * es * (int64_t)sizeof(C));
+  C v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = gld<C>((const char *)in + (size_t)i * st, ob);
   __syncthreads();
-  const int line = threadIdx.x % TILE, tl = threadIdx.x / TILE;
-  line_fft<T, N, DIR>(&s_line[line * PITCH], s_tw, tl, line < ncol);
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col < ncol) out[base + el * es + col] = cscale(s_line[col * PITCH + el], scale);
+  rf::rfft<T, N, G::EM, DIR, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) gst<C>((char *)out + (size_t)i * st, ob, cscale(v[i], scale));
   }
 }
 
 // ------------------------------- inverse x pass with fused k-space combine --
 // Spectra are [nx][ny][P]; x line (ky, kz) at ky*P + kx*ny*P + kz.
 template <typename T, int N>
-__global__ void __launch_bounds__((Tile<T, N>::THREADS))
+__global__ void __launch_bounds__((ColCfg<T, N>::BS))
     k_xcombine_inv(Batch b, CombineSet cs, KernTab kt, int ny, int P, int ntile, const cplx<T> *__restrict__ g_tw) {
   using C = cplx<T>;
-  constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
-  __shared__ C s_line[TILE * PITCH];
+  using G = ColCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
+  __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
   const Combine cb = cs.c[blockIdx.y];
-  C *__restrict__ out = (C *)b.out[blockIdx.y];
-  const int o = blockIdx.x / ntile, ct = blockIdx.x - o * ntile, c0 = ct * TILE;
-  const int ncol = (P - c0) < TILE ? (P - c0) : TILE;
-  const int64_t es = (int64_t)ny * P;
-  load_twiddles<T, N>(s_tw, g_tw);
-  const int64_t base = (int64_t)o * P + c0;
+  void *out = b.out[blockIdx.y];
+  const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
+  const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
+  const bool act = c0 + col < P;
+  const int cc = act ? c0 + col : P - 1;
+  const uint32_t es = (uint32_t)ny * P;
+  const uint32_t e0 = (uint32_t)o * P + cc + t * es;  // element index of point t
+  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
   const T *sx = (const T *)kt.sinx, *sy = (const T *)kt.siny, *sz = (const T *)kt.sinz;
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col >= ncol) continue;
-    const int64_t gi = base + el * es + col;
-    T accx = 0, accy = 0;
-    for (int t = 0; t < cb.nterms; ++t) {
-      const Term tm = cb.t[t];
-      C v = ((const C *)b.in[tm.in])[gi];
+  C v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = cmk<C>(0, 0);
+  const size_t stC = (size_t)TPL * es * sizeof(C), stR = (size_t)TPL * es * sizeof(T);
+  for (int q = 0; q < cb.nterms; ++q) {
+    const Term tm = cb.t[q];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int el = t + TPL * i;
+      C a = gld<C>((const char *)b.in[tm.in] + i * stC, e0 * (uint32_t)sizeof(C));
       T s = (T)tm.scale;
-      if (tm.kern >= 0) s *= ((const T *)kt.k[tm.kern])[gi];
+      if (tm.kern >= 0) s *= gld<T>((const char *)kt.k[tm.kern] + i * stR, e0 * (uint32_t)sizeof(T));
       if (tm.deriv >= 0) {
-        const T d = tm.deriv == 0 ? sx[el] : (tm.deriv == 1 ? sy[o] : sz[c0 + col]);
-        s *= d;
-        v = cmk<C>(-v.y, v.x);  // times i
+        s *= tm.deriv == 0 ? sx[el] : (tm.deriv == 1 ? sy[o] : sz[cc]);
+        a = cmk<C>(-a.y, a.x);  // times i
       }
-      accx += s * v.x;
-      accy += s * v.y;
+      v[i].x += s * a.x;
+      v[i].y += s * a.y;
     }
-    s_line[col * PITCH + el] = cmk<C>(accx, accy);
   }
   __syncthreads();
-  const int line = threadIdx.x % TILE, tl = threadIdx.x / TILE;
-  line_fft<T, N, +1>(&s_line[line * PITCH], s_tw, tl, line < ncol);
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col < ncol) out[base + el * es + col] = s_line[col * PITCH + el];
+  rf::rfft<T, N, G::EM, +1, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) gst<C>((char *)out + i * stC, e0 * (uint32_t)sizeof(C), v[i]);
   }
 }
 
 // ---------------------------------------- propagator passes (complex psi) --
+// Sum of an FP64 value over the block (256 threads) into partial[blockIdx.x].
+__device__ __forceinline__ void block_sum_256(double acc, double *partial) {
+  __shared__ double s_red[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+}
+
 // z pass over a full complex field, multiplying by e^{-iV dt/2} before (DIR<0)
 // or after (DIR>0, together with `scale`) the transform.  With PHASE the
 // `pot` array already holds e^{-iV dt/2}.  After the inverse pass each block
 // writes sum |psi|^2 to partial[block] (FP64).
 template <typename T, int N, int DIR, bool PHASE = false>
-__global__ void __launch_bounds__((Tile<T, N>::THREADS))
+__global__ void __launch_bounds__(256)
     k_zc2c_pot(const cplx<T> *src, cplx<T> *dst, const cplx<T> *__restrict__ pot, int64_t nlines,
                const cplx<T> *__restrict__ g_tw, T dtr, T dti, T scale, double *__restrict__ partial) {
   using C = cplx<T>;
-  constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
-  __shared__ C s_line[TILE * PITCH];
+  using G = LineCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
+  __shared__ C s_x[G::S > 1 ? G::LINES * LP : 1];
   __shared__ C s_tw[2 * N];
-  __shared__ double s_red[BS];
-  const int64_t l0 = (int64_t)blockIdx.x * TILE;
-  const int nl = (int)((nlines - l0) < TILE ? (nlines - l0) : TILE);
-  load_twiddles<T, N>(s_tw, g_tw);
-  const int64_t g0 = l0 * N;
-  for (int e = threadIdx.x; e < nl * N; e += BS) {
-    const int ln = e / N, el = e - ln * N;
-    C v = src[g0 + e];
-    if (DIR < 0 && pot) v = cmul(v, PHASE ? pot[g0 + e] : vphase<T>(pot[g0 + e], dtr, dti));
-    s_line[ln * PITCH + el] = v;
+  const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const uint32_t l0 = (uint32_t)blockIdx.x * G::LINES + ln;
+  const bool act = l0 < nlines;
+  const uint32_t l = act ? l0 : 0;
+  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  const uint32_t e0 = l * (uint32_t)N + (uint32_t)t;
+  C v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = lde<C>(src, e0, TPL * i);
+  if (DIR < 0 && pot) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const C ph = lde<C>(pot, e0, TPL * i);
+      v[i] = cmul(v[i], PHASE ? ph : vphase<T>(ph, dtr, dti));
+    }
   }
   __syncthreads();
-  const int line = threadIdx.x % TILE, tl = threadIdx.x / TILE;
-  line_fft<T, N, DIR>(&s_line[line * PITCH], s_tw, tl, line < nl);
+  rf::rfft<T, N, G::EM, DIR, rf::WaveSync>(v, s_x + ln * LP, s_tw, t);
   double acc = 0.0;
-  for (int e = threadIdx.x; e < nl * N; e += BS) {
-    const int ln = e / N, el = e - ln * N;
-    C v = s_line[ln * PITCH + el];
-    if (DIR > 0) {
-      v = cscale(v, scale);
-      if (pot) v = cmul(v, PHASE ? pot[g0 + e] : vphase<T>(pot[g0 + e], dtr, dti));
-      acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+  if (DIR > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      v[i] = cscale(v[i], scale);
+      if (pot) {
+        const C ph = lde<C>(pot, e0, TPL * i);
+        v[i] = cmul(v[i], PHASE ? ph : vphase<T>(ph, dtr, dti));
+      }
+      acc += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
     }
-    dst[g0 + e] = v;
+    if (!act) acc = 0.0;
   }
-  if (DIR > 0 && partial) {
-    s_red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = s_red[0];
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) ste<C>(dst, e0, v[i], TPL * i);
   }
+  if (DIR > 0 && partial) block_sum_256(acc, partial);
 }
 
 // Fused x pass of the propagator: forward x FFT, kinetic phase
 // exp(-i k^2 dt / 2M), inverse x FFT.  Field is [nx][ny][nz] complex.
 template <typename T, int N>
-__global__ void __launch_bounds__((Tile<T, N>::THREADS))
-    k_xkin(cplx<T> *__restrict__ psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw,
-           T cx, T cy, T cz, T dtr, T dti) {
+__global__ void __launch_bounds__((ColCfg<T, N>::BS))
+    k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, T cx, T cy, T cz, T dtr,
+           T dti) {
   using C = cplx<T>;
-  constexpr int TILE = Tile<T, N>::TILE, PITCH = Shape<N>::PITCH, BS = Tile<T, N>::THREADS;
-  __shared__ C s_line[TILE * PITCH];
+  using G = ColCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
+  __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
-  const int o = blockIdx.x / ntile, ct = blockIdx.x - o * ntile, c0 = ct * TILE;
-  const int ncol = (nz - c0) < TILE ? (nz - c0) : TILE;
-  const int64_t es = (int64_t)ny * nz, base = (int64_t)o * nz + c0;
-  load_twiddles<T, N>(s_tw, g_tw);
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col < ncol) s_line[col * PITCH + el] = psi[base + el * es + col];
-  }
+  const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
+  const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
+  const bool act = c0 + col < nz;
+  const int cc = act ? c0 + col : nz - 1;
+  const uint32_t es = (uint32_t)ny * nz;
+  const uint32_t ob = ((uint32_t)o * nz + cc + t * es) * (uint32_t)sizeof(C);
+  const uint32_t st = (uint32_t)TPL * es * (uint32_t)sizeof(C);
+  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  C v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = gld<C>((const char *)psi + (size_t)i * st, ob);
   __syncthreads();
-  const int line = threadIdx.x % TILE, tl = threadIdx.x / TILE;
-  line_fft<T, N, -1>(&s_line[line * PITCH], s_tw, tl, line < ncol);
+  rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
   // kinetic phase: K = cx*mx^2 + cy*my^2 + cz*mz^2 (signed frequency indices)
   const int my = o < ny / 2 ? o : o - ny;
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col >= ncol) continue;
-    const int mx = el < N / 2 ? el : el - N;
-    const int kz = c0 + col, mz = kz < nz / 2 ? kz : kz - nz;
-    const T K = cx * (T)(mx * mx) + cy * (T)(my * my) + cz * (T)(mz * mz);
+  const int kz = cc, mz = kz < nz / 2 ? kz : kz - nz;
+  const double Kyz = (double)cy * (double)(my * my) + (double)cz * (double)(mz * mz);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int el = t + TPL * i, mx = el < N / 2 ? el : el - N;
+    const double K = (double)cx * (double)(mx * mx) + Kyz;
     // exp(-i K dt) = exp(K dti) (cos(K dtr) - i sin(K dtr))
-    T s, c;
-    dsincos(K * dtr, &s, &c);
-    const T m = dexp(K * dti);
-    s_line[col * PITCH + el] = cmul(s_line[col * PITCH + el], cmk<C>(m * c, -m * s));
+    v[i] = cmul(v[i], unit_phase<T>(K * (double)dti, -K * (double)dtr));
   }
-  __syncthreads();
-  line_fft<T, N, +1>(&s_line[line * PITCH], s_tw, tl, line < ncol);
-  for (int e = threadIdx.x; e < TILE * N; e += BS) {
-    const int col = e % TILE, el = e / TILE;
-    if (col < ncol) psi[base + el * es + col] = s_line[col * PITCH + el];
+  rf::rfft<T, N, G::EM, +1, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) gst<C>((char *)psi + (size_t)i * st, ob, v[i]);
   }
 }
 

@@ -387,6 +387,11 @@ struct dft_ot_device {
   void *kern[4];             // real half spectra (LJ, SA, F, U)
   void *phase;               // complex: e^{-iV dt/2} of the current step (allocated on first step)
   void *vsave;               // complex: predictor potential V_n of a predict-correct step
+  // x-middle programs of the fused schedule in device memory, and the host
+  // copies they were uploaded from (re-uploaded only when one changes)
+  fused::MidProg *mid_dev;
+  fused::MidProg mid_host[3];
+  bool mid_valid[3];
 };
 
 namespace {
@@ -624,24 +629,19 @@ struct StepIn {
 template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho, const fused::Ptrs8 &o,
                                             int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw,
                                             hipStream_t s, double bytes) {
+  constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
+  const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (BF) {
-    constexpr int TILE = fused::ZCfg<T, N, 5>::TILE;
-    hipLaunchKernelGGL((fused::k_zA<T, N, true>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, psi,
-                       rho, o, npairs, P, pw, tw);
-  } else {
-    constexpr int TILE = fused::ZCfg<T, N, 1>::TILE;
-    hipLaunchKernelGGL((fused::k_zA<T, N, false>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s,
-                       psi, rho, o, npairs, P, pw, tw);
-  }
+  if (BF) hipLaunchKernelGGL((fused::k_zA<T, N, true>), grid, dim3(256), 0, s, psi, rho, o, npairs, P, pw, tw);
+  else hipLaunchKernelGGL((fused::k_zA<T, N, false>), grid, dim3(256), 0, s, psi, rho, o, npairs, P, pw, tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_zA_density_r2c", bytes);
 }
 
 template <typename T, int N>
-void launch_xmid(const fused::MidProg &pg, int ny, int P, const cplx<T> *tw, hipStream_t s, double bytes) {
+void launch_xmid(const fused::MidProg *pg, int ny, int P, const cplx<T> *tw, hipStream_t s, double bytes) {
   using Cfg = fused::MidCfg<T, N>;
-  const int ntile = (P + Cfg::TILE - 1) / Cfg::TILE;
+  const int ntile = (P + Cfg::COLS - 1) / Cfg::COLS;
   DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fused::k_xmid<T, N>), dim3((unsigned)(ny * ntile)), dim3(Cfg::BS), 0, s, pg, ny, P, ntile, tw);
   DFT_LAUNCH_CHECK();
@@ -650,9 +650,8 @@ void launch_xmid(const fused::MidProg &pg, int ny, int P, const cplx<T> *tw, hip
 
 template <typename T, int N, bool KC, bool BF>
 void launch_zAB1(const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw, hipStream_t s) {
-  constexpr int NIN = 1 + (KC ? 1 : 0) + (BF ? 5 : 0), NOUT = 1 + (KC ? 3 : 0);
-  constexpr int TILE = fused::ZCfg<T, N, (NIN > NOUT ? NIN : NOUT)>::TILE;
-  hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, a,
+  constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
+  hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), dim3((unsigned)((npairs + LN - 1) / LN)), dim3(256), 0, s, a,
                      npairs, P, pw, tw);
 }
 
@@ -671,8 +670,8 @@ void launch_zAB(bool KC, bool BF, const fused::ZABArgs &a, int64_t npairs, int P
 template <typename T, int N>
 void launch_zBC(bool BF, const fused::ZBCArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw,
                 hipStream_t s, double bytes) {
-  constexpr int TILE = fused::ZCfg<T, N, 4>::TILE;
-  const dim3 grid((unsigned)((npairs + TILE - 1) / TILE));
+  constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
+  const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
   if (BF) hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
   else hipLaunchKernelGGL((fused::k_zBC<T, N, false>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
@@ -683,18 +682,38 @@ void launch_zBC(bool BF, const fused::ZBCArgs &a, int64_t npairs, int P, const P
 template <typename T, int N>
 void launch_zC(bool step, const fused::ZCArgs &a, int64_t npairs, int P, const cplx<T> *tw, hipStream_t s,
                double bytes) {
+  constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
+  const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (step) {
-    constexpr int TILE = fused::ZCfg<T, N, 3>::TILE;
-    hipLaunchKernelGGL((fused::k_zC<T, N, true>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, a,
-                       npairs, P, tw);
-  } else {
-    constexpr int TILE = fused::ZCfg<T, N, 1>::TILE;
-    hipLaunchKernelGGL((fused::k_zC<T, N, false>), dim3((unsigned)((npairs + TILE - 1) / TILE)), dim3(256), 0, s, a,
-                       npairs, P, tw);
-  }
+  if (step) hipLaunchKernelGGL((fused::k_zC<T, N, true>), grid, dim3(256), 0, s, a, npairs, P, tw);
+  else hipLaunchKernelGGL((fused::k_zC<T, N, false>), grid, dim3(256), 0, s, a, npairs, P, tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, step ? "ot_zC_c2r_phase_z_fwd" : "ot_zC_c2r_accumulate", bytes);
+}
+
+// Device copy of x-middle program k (uploaded when it differs from the last one).
+const fused::MidProg *mid_prog(dft_ot_device *d, int k, const fused::MidProg &pg, hipStream_t s) {
+  if (!d->mid_dev) {
+    d->mid_dev = (fused::MidProg *)dmalloc(3 * sizeof(fused::MidProg));
+    if (!d->mid_dev) {
+      fprintf(stderr, "libdft: out of device memory for the x-middle programs\n");
+      abort();
+    }
+  }
+  if (!d->mid_valid[k] || memcmp(&d->mid_host[k], &pg, sizeof pg) != 0) {
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    memcpy(&d->mid_host[k], &pg, sizeof pg);
+    DFT_HIP_CHECK(hipMemcpy(d->mid_dev + k, &d->mid_host[k], sizeof pg, hipMemcpyHostToDevice));
+    d->mid_valid[k] = true;
+  }
+  return d->mid_dev + k;
+}
+
+// A zero-filled program (padding included, so that programs compare bytewise).
+fused::MidProg new_prog() {
+  fused::MidProg pg;
+  memset(&pg, 0, sizeof pg);
+  return pg;
 }
 
 fused::MidTerm mterm(int in, int kern, int deriv, double scale) {
@@ -747,7 +766,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   // x middle: Pi*rho, [F*rho], [U*rho, U*q_x, U*q_y, U*q_z, U*q4]
   int nB = 0;
   {
-    fused::MidProg pg{};
+    fused::MidProg pg = new_prog();
     pg.nin = nA;
     for (int i = 0; i < nA; ++i) pg.in[i] = d->slot[SP0 + i];
     pg.multi = -1;
@@ -765,7 +784,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     }
     setk(pg);
     nB = pg.nout;
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(pg, d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 0, pg, s), d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[8];
     for (int i = 0; i < nB; ++i) b[i] = CS<T>(d, TMP0 + i);
     ypass<T>(p, nB, b, +1, P, s);
@@ -794,7 +813,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   }
   // ---- KC: J_d = F*G_d, T1 = F*sum_d D_d G_d; then H, terms 1 and 3, BF divergence
   if (KC) {
-    fused::MidProg pg{};
+    fused::MidProg pg = new_prog();
     pg.nin = 3;
     pg.in[0] = d->slot[QX];
     pg.in[1] = d->slot[QY];
@@ -810,7 +829,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.o[3].out = d->slot[TMP3];
     pg.multi = 3;
     setk(pg);
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(pg, d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 1, pg, s), d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[4] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2), CS<T>(d, TMP3)};
     ypass<T>(p, 4, b, +1, P, s);
     fused::ZBCArgs a{};
@@ -835,7 +854,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   }
   // ---- stage C: linear terms summed in k space, z^-1, accumulate or start the step
   {
-    fused::MidProg pg{};
+    fused::MidProg pg = new_prog();
     pg.nin = KC ? 3 : 2;
     pg.in[0] = d->slot[SP0];
     pg.in[1] = d->slot[Q4];
@@ -848,7 +867,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.o[0].t[2] = mterm(2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass));
     pg.o[0].out = d->slot[TMP4];
     setk(pg);
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(pg, d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 2, pg, s), d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[1] = {CS<T>(d, TMP4)};
     ypass<T>(p, 1, b, +1, P, s);
     fused::ZCArgs a{};
@@ -1101,6 +1120,7 @@ void free_device(dft_ot_device *d) {
   for (int i = 0; i < 4; ++i) dfree(d->kern[i]);
   dfree(d->phase);
   dfree(d->vsave);
+  dfree(d->mid_dev);
   if (d->plan) {
     if (d->prec == GRID_FP32) {
       ((Plan3D<float> *)d->plan)->destroy();

@@ -1,10 +1,14 @@
-// Fused stage kernels of the OT potential and the split-operator step.
+// Fused stage kernels of the OT potential and the split-operator step, on the
+// register-resident line transforms of rfft.hpp.
 //
-// Every z pass works on tiles of z-line pairs (adjacent y lines (x, 2p) and
-// (x, 2p+1)), which are contiguous in memory.  The pointwise stages run as the
-// prologue or epilogue of these passes, so their inputs and outputs never
-// make an extra trip to HBM.  Stencils in x and y read neighbouring lines
-// from global memory (they are in L2/MALL: adjacent planes are processed
+// Every z pass works on pairs of adjacent z lines (x, 2p) and (x, 2p+1),
+// contiguous in memory and packed as one complex line a + ib.  TPL lanes of
+// one wavefront hold a pair, E points each (z = t + TPL*i), so the pointwise
+// stages run in registers as the prologue or epilogue of the z transforms,
+// field by field: a stage that consumes several spectra transforms them one
+// after another and accumulates; a stage that produces several transforms and
+// stores each as soon as it is formed.  Stencils in x and y read neighbouring
+// lines from global memory (L2/MALL resident: adjacent planes are processed
 // together).
 //
 //   zA    psi -> rho (stored) and, with backflow, q_d = rho v_d, q4 = rho |v|^2;
@@ -12,153 +16,188 @@
 //   xmid  x forward FFT, k-space products with real kernel spectra and the
 //         central-difference symbol i sin(k h)/h, x inverse FFT (a small
 //         program: several outputs per input, one multi-term output)
-//   zAB   z c2r of {Pi*rho, F*rho, U*rho, U*q_d, U*q4}; local terms, G_d, BF real
-//         part, W_d; z r2c of {Q, G_x, G_y, G_z}
-//   zBC   z c2r of {J_d, T1}; H, KC terms 1 and 3, BF divergence; z r2c of {H}
+//   zAB   z c2r of {U*rho, U*q_d, U*q4} -> BF real part, W_d, BF imaginary part;
+//         z c2r of Pi*rho -> local terms, Q, z r2c of Q;
+//         z c2r of F*rho -> rho_st, G_d, z r2c of G_d
+//   zBC   z c2r of {J_d, T1} -> H, KC terms 1 and 3, BF divergence; z r2c of H
 //   zC    z c2r of the summed linear terms; V += (API), or phase = e^{-iV dt/2}
 //         and the first z pass of the propagator (step)
 #pragma once
-#include "fft.hpp"
+#include "fft_kernels.hpp"
 #include "ot_common.hpp"
 
 namespace dft {
 namespace fused {
 
-using fft::Shape;
-
-// Line-pair tiles for z passes holding LINES LDS lines per pair.
-template <typename T, int N, int LINES> struct ZCfg {
-  static constexpr int csize = 2 * sizeof(T);
-  static constexpr int PITCH = Shape<N>::PITCH;
-  static constexpr int fit(int t) {
-    return (t > 1 && t * LINES * PITCH * csize + 2 * N * csize > 80 * 1024) ? fit(t / 2) : t;
-  }
-  static constexpr int TILE = fit(8);
-  static constexpr int BS = 256;
-};
-
-// x-middle tiles: columns per block, three LDS buffers.
-template <typename T, int N> struct MidCfg {
-  static constexpr int csize = 2 * sizeof(T);
-  static constexpr int PITCH = Shape<N>::PITCH;
-  static constexpr int fit(int t) {
-    return (t > 1 && (3 * t * PITCH * csize + 2 * N * csize > 80 * 1024 || t * Shape<N>::TPL > 256)) ? fit(t / 2) : t;
-  }
-  static constexpr int TILE = fit(16);
-  static constexpr int BS = TILE * Shape<N>::TPL;
-};
-
-// All block threads transform NL lines (line l at L + l*PITCH), BS threads.
-template <typename T, int N, int DIR, int BS>
-__device__ inline void lines_fft(cplx<T> *L, const cplx<T> *tw, int NL) {
-  constexpr int TPL = Shape<N>::TPL, PITCH = Shape<N>::PITCH, LPB = BS / TPL;
-  for (int l0 = 0; l0 < NL; l0 += LPB) {
-    const int line = l0 + (int)threadIdx.x % LPB;
-    fft::line_fft<T, N, DIR>(L + line * PITCH, tw, (int)threadIdx.x / LPB, line < NL);
-  }
-}
-
-// Half spectrum of line `which` of a packed pair from Z = FFT(a + ib).
-template <typename C> __device__ inline C split(C z, C zm, int which) {
-  using T = decltype(z.x);
-  return which == 0 ? cmk<C>((T)0.5 * (z.x + zm.x), (T)0.5 * (z.y - zm.y))
-                    : cmk<C>((T)0.5 * (z.y + zm.y), (T)0.5 * (zm.x - z.x));
-}
-
-// Z(m) = A(m) + i B(m) with the Hermitian extension of the half spectra A, B.
-template <typename C> __device__ inline C merge(const C *A, const C *B, int m, int N) {
-  C a, b;
-  if (m <= N / 2) {
-    a = A[m];
-    b = B[m];
-  } else {
-    a = cconj(A[N - m]);
-    b = cconj(B[N - m]);
-  }
-  return cmk<C>(a.x - b.y, a.y + b.x);
-}
-
-template <typename C> __device__ inline auto &comp(C &c, int which) { return ((decltype(c.x) *)&c)[which]; }
-
-// v_d = Im(psi* (psi_{+d} - psi_{-d})/2h) / (M (rho + eps)), clamped when enabled.
-template <typename T>
-__device__ inline void velocity_at(const cplx<T> *__restrict__ psi, int64_t c, const PW<T> &pw, T r, T v[3]) {
-  const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
-  const cplx<T> s = psi[c];
-  const T den = pw.invM / (r + pw.eps_v);
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const cplx<T> a = psi[nb.p[d]], b = psi[nb.m[d]];
-    const T dx = (a.x - b.x) * pw.inv2h, dy = (a.y - b.y) * pw.inv2h;
-    v[d] = (s.x * dy - s.y * dx) * den;
-    if (pw.clamp_v) v[d] = clampv(v[d], pw.vmax);
-  }
-}
+using rf::ColCfg;
+using rf::LineCfg;
+using rf::RP;
 
 struct Ptrs8 {
   void *p[8];
 };
 
-// Write NF split half spectra (fields f at lines f*TILE + pr) for np pairs from p0.
-template <typename T, int N, int TILE>
-__device__ inline void store_split(const cplx<T> *L, const Ptrs8 &outs, int NF, int64_t p0, int np, int P) {
-  constexpr int PITCH = Shape<N>::PITCH;
-  for (int f = 0; f < NF; ++f) {
-    cplx<T> *out = (cplx<T> *)outs.p[f] + p0 * 2 * P;
-    for (int e = threadIdx.x; e < np * 2 * P; e += 256) {
-      const int pr = e / (2 * P), w = e - pr * 2 * P, which = w / P, k = w - which * P;
-      const cplx<T> *ln = L + (f * TILE + pr) * PITCH;
-      out[e] = split(ln[k], ln[(N - k) & (N - 1)], which);
+// Addresses of a pair's points and of their periodic neighbours.  Line a is
+// (ix, j), line b is (ix, j + 1); thread t holds points z = t + TPL*i of both.
+// Every access is (element index of point t of some line) + a constant delta,
+// so all points of a thread share a handful of per-thread offsets.
+template <typename T, int N, int TPL> struct PairPos {
+  uint32_t a0;         // point t of line a (line b is a0 + N)
+  uint32_t xp0, xm0;   // point t of lines (ix +- 1, j)
+  uint32_t ym0, yp1;   // point t of lines (ix, j - 1) and (ix, j + 2)
+  int t;
+  PairPos() = default;
+  __device__ PairPos(uint32_t p, int t_, const PW<T> &pw) : t(t_) {
+    const uint32_t l = 2 * p;
+    const int j = (int)(l & (uint32_t)(pw.ny - 1)), ix = (int)(l >> __builtin_ctz(pw.ny));
+    a0 = l * N + t;
+    xp0 = ((uint32_t)((ix + 1) & (pw.nx - 1)) * pw.ny + j) * N + t;
+    xm0 = ((uint32_t)((ix - 1) & (pw.nx - 1)) * pw.ny + j) * N + t;
+    ym0 = ((uint32_t)ix * pw.ny + ((j - 1) & (pw.ny - 1))) * N + t;
+    yp1 = ((uint32_t)ix * pw.ny + ((j + 2) & (pw.ny - 1))) * N + t;
+  }
+  static constexpr int d(int w, int i) { return w * N + TPL * i; }
+  // The same positions with offsets that the loads of the current stage
+  // depend on (see rf::opaque_zero).
+  __device__ PairPos here() const {
+    PairPos r = *this;
+    const uint32_t z = rf::opaque_zero();
+    r.a0 += z;
+    r.xp0 += z;
+    r.xm0 += z;
+    r.ym0 += z;
+    r.yp1 += z;
+    return r;
+  }
+  // field value at point (w, i)
+  template <typename X> __device__ X at(const void *f, int w, int i) const { return rf::lde<X>(f, a0, d(w, i)); }
+  template <typename X> __device__ void put(void *f, int w, int i, X v) const { rf::ste<X>(f, a0, v, d(w, i)); }
+  // neighbour of point (w, i) one step along axis D in direction S (+1 / -1)
+  template <typename X, int D, int S> __device__ X nb(const void *f, int w, int i) const {
+    constexpr int E = N / TPL;
+    if constexpr (D == 0) {
+      return rf::lde<X>(f, S > 0 ? xp0 : xm0, d(w, i));
+    } else if constexpr (D == 1) {
+      if (S > 0) return w ? rf::lde<X>(f, yp1, TPL * i) : rf::lde<X>(f, a0, d(1, i));
+      return w ? rf::lde<X>(f, a0, d(0, i)) : rf::lde<X>(f, ym0, TPL * i);
+    } else {
+      // z +- 1; only the last point of the line wraps forward, the first backward
+      if (S > 0) {
+        if (i < E - 1) return rf::lde<X>(f, a0, d(w, i) + 1);
+        return rf::lde<X>(f, t == TPL - 1 ? a0 - t : a0 + TPL * i + 1, w * N);
+      }
+      if (i > 0) return rf::lde<X>(f, a0, d(w, i) - 1);
+      return rf::lde<X>(f, t == 0 ? a0 + (N - 1) : a0 - 1, w * N);
     }
   }
+};
+
+// v_d = Im(psi* (psi_{+d} - psi_{-d})/2h) / (M (rho + eps)), clamped when enabled.
+template <typename T, int D, class Q>
+__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T r) {
+  const cplx<T> s = q.template at<cplx<T>>(psi, w, i);
+  const cplx<T> a = q.template nb<cplx<T>, D, 1>(psi, w, i), b = q.template nb<cplx<T>, D, -1>(psi, w, i);
+  const T dx = (a.x - b.x) * pw.inv2h, dy = (a.y - b.y) * pw.inv2h;
+  T v = (s.x * dy - s.y * dx) * (pw.invM / (r + pw.eps_v));
+  if (pw.clamp_v) v = clampv(v, pw.vmax);
+  return v;
 }
 
-// Load NIN half-spectrum inputs (rows 2p, 2p+1) as packed complex lines f*TILE + pr.
-template <typename T, int N, int TILE>
-__device__ inline void load_merge(cplx<T> *L, const Ptrs8 &ins, int NIN, int64_t p0, int np, int P) {
-  constexpr int PITCH = Shape<N>::PITCH;
-  for (int i = 0; i < NIN; ++i) {
-    const cplx<T> *in = (const cplx<T> *)ins.p[i];
-    for (int m = threadIdx.x; m < np * N; m += 256) {
-      const int pr = m / N, idx = m - pr * N;
-      const cplx<T> *A = in + (p0 + pr) * 2 * P;
-      L[(i * TILE + pr) * PITCH + idx] = merge(A, A + P, idx, N);
-    }
-  }
+template <typename T, int D, class Q>
+__device__ __forceinline__ T grad_d(const void *f, const Q &q, int w, int i, const PW<T> &pw) {
+  return (q.template nb<T, D, 1>(f, w, i) - q.template nb<T, D, -1>(f, w, i)) * pw.inv2h;
 }
+
+// Pair kernels use the plan with the fewest points per thread: 4 (8 for N = 512,
+// where a line must still fit one wavefront).
+template <int N> constexpr int pair_em() { return N > 256 ? 8 : 4; }
+
+// Common prologue of the pair kernels.
+#define FUSED_PAIR_SETUP()                                                   \
+  using C = cplx<T>;                                                         \
+  using G = LineCfg<T, N, pair_em<N>()>;                                          \
+  constexpr int E = G::E, TPL = G::TPL, LP = G::LP;                          \
+  __shared__ C s_x[G::LINES * LP];                                           \
+  __shared__ C s_tw[2 * N];                                                  \
+  const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;                   \
+  const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;                  \
+  const bool act = p0 < npairs;                                              \
+  /* out-of-range lanes compute on pair 0 and store nothing */               \
+  const uint32_t p = act ? p0 : 0;                                           \
+  C *X = s_x + ln * LP;                                                      \
+  rf::load_tw<T, N, 256>(s_tw, g_tw);                                        \
+  __syncthreads();                                                           \
+  const PairPos<T, N, TPL> q(p, t, pw)
+
+// r2c of the pair held in v and store of both half spectra to field `out`.
+template <typename T, int N>
+__device__ __forceinline__ void r2c_store(rf::Regs<T, N, pair_em<N>()> &v, cplx<T> *X, const cplx<T> *tw, int t,
+                                          void *out, uint32_t p, int P, bool act) {
+  rf::rfft<T, N, pair_em<N>(), -1, rf::WaveSync>(v, X, tw, t);
+  fft::split_store<T, N, pair_em<N>(), rf::WaveSync>(v, X, t, out, p, P, act);
+}
+
+// Load + c2r of both lines of the pair from half-spectrum field `in`.
+template <typename T, int N>
+__device__ __forceinline__ void load_c2r(rf::Regs<T, N, pair_em<N>()> &v, cplx<T> *X, const cplx<T> *tw, int t,
+                                         const void *in, uint32_t p, int P) {
+  fft::load_merge<T, N, pair_em<N>()>(v, in, p, P, t);
+  rf::rfft<T, N, pair_em<N>(), +1, rf::WaveSync>(v, X, tw, t);
+}
+
+template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { return w ? c.y : c.x; }
 
 // -------------------------------------------------------------------- zA --
 template <typename T, int N, bool BF>
 __global__ void __launch_bounds__(256)
-    k_zA(const cplx<T> *__restrict__ psi, T *__restrict__ rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
+    k_zA(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
          const cplx<T> *__restrict__ g_tw) {
-  constexpr int NF = BF ? 5 : 1;
-  using Cfg = ZCfg<T, N, NF>;
-  constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
-  __shared__ cplx<T> L[NF * TILE * PITCH];
-  __shared__ cplx<T> s_tw[2 * N];
-  const int64_t p0 = (int64_t)blockIdx.x * TILE;
-  const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
-  fft::load_twiddles<T, N>(s_tw, g_tw);
-  for (int e = threadIdx.x; e < np * 2 * N; e += 256) {
-    const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, z = w - which * N;
-    const int64_t c = p0 * 2 * N + e;
-    const cplx<T> s = psi[c];
-    const T r = s.x * s.x + s.y * s.y;
-    rho[c] = r;
-    comp(L[(0 * TILE + pr) * PITCH + z], which) = r;
-    if constexpr (BF) {
-      T v[3];
-      velocity_at(psi, c, pw, r, v);
-      comp(L[(1 * TILE + pr) * PITCH + z], which) = v[0] * r;
-      comp(L[(2 * TILE + pr) * PITCH + z], which) = v[1] * r;
-      comp(L[(3 * TILE + pr) * PITCH + z], which) = v[2] * r;
-      comp(L[(4 * TILE + pr) * PITCH + z], which) = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * r;
+  FUSED_PAIR_SETUP();
+  C v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const C sa = q.template at<C>(psi, 0, i), sb = q.template at<C>(psi, 1, i);
+    v[i] = cmk<C>(sa.x * sa.x + sa.y * sa.y, sb.x * sb.x + sb.y * sb.y);
+  }
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      q.template put<T>(rho, 0, i, v[i].x);
+      q.template put<T>(rho, 1, i, v[i].y);
     }
   }
-  __syncthreads();
-  lines_fft<T, N, -1, 256>(L, s_tw, NF * TILE);
-  store_split<T, N, TILE>(L, outs, NF, p0, np, P);
+  r2c_store<T, N>(v, X, s_tw, t, outs.p[0], p, P, act);
+  if constexpr (BF) {
+    T v2[2][E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v2[0][i] = v2[1][i] = 0;
+    auto qd = [&](auto dtag) {
+      constexpr int D = decltype(dtag)::value;
+      const auto qq = q.here();
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        T o[2];
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          const C s = qq.template at<C>(psi, w, i);
+          const T r = s.x * s.x + s.y * s.y;
+          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, r);
+          v2[w][i] += vd * vd;
+          o[w] = vd * r;
+        }
+        v[i] = cmk<C>(o[0], o[1]);
+      }
+      r2c_store<T, N>(v, X, s_tw, t, outs.p[1 + D], p, P, act);
+    };
+    qd(std::integral_constant<int, 0>{});
+    qd(std::integral_constant<int, 1>{});
+    qd(std::integral_constant<int, 2>{});
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      v[i] = cmk<C>(v2[0][i] * q.template at<T>(rho, 0, i), v2[1][i] * q.template at<T>(rho, 1, i));
+    }
+    r2c_store<T, N>(v, X, s_tw, t, outs.p[4], p, P, act);
+  }
 }
 
 // ------------------------------------------------------------------ xmid --
@@ -180,10 +219,14 @@ struct MidProg {
   const void *sinx, *siny, *sinz;
 };
 
+// Coefficient s_t K_t(k) [i sin(k_d h)/h] times a, for the point at byte offset
+// ui + ob of a real half-spectrum field (x frequency el, y frequency o, z
+// frequency kz); ui is uniform across the wave, ob per thread.
 template <typename T>
-__device__ inline cplx<T> mid_coef(const MidTerm &t, const MidProg &pg, int64_t gi, int el, int o, int kz, cplx<T> a) {
+__device__ __forceinline__ cplx<T> mid_coef(const MidTerm &t, const MidProg &pg, size_t ui, uint32_t ob, int el, int o,
+                                            int kz, cplx<T> a) {
   T s = (T)t.scale;
-  if (t.kern >= 0) s *= ((const T *)pg.kern[t.kern])[gi];
+  if (t.kern >= 0) s *= rf::gld<T>((const char *)pg.kern[t.kern] + ui, ob);
   if (t.deriv >= 0) {
     s *= t.deriv == 0 ? ((const T *)pg.sinx)[el] : (t.deriv == 1 ? ((const T *)pg.siny)[o] : ((const T *)pg.sinz)[kz]);
     return cmk<cplx<T>>(-s * a.y, s * a.x);
@@ -191,73 +234,81 @@ __device__ inline cplx<T> mid_coef(const MidTerm &t, const MidProg &pg, int64_t 
   return cmk<cplx<T>>(s * a.x, s * a.y);
 }
 
+// x-middle geometry: 8 points per thread (several spectra are live at once),
+// 8 columns per block (256 threads at N = 256).
+template <typename T, int N> using MidCfg = ColCfg<T, N, 8, 8>;
+
 // Spectra are [nx][ny][P] after the y pass; the x line (ky = o, kz = c) has stride ny*P.
+// The program is read from device memory: indexed at run time, a by-value
+// kernel argument of this size would be copied to scratch by every thread.
 template <typename T, int N>
-__global__ void __launch_bounds__((MidCfg<T, N>::BS))
-    k_xmid(MidProg pg, int ny, int P, int ntile, const cplx<T> *__restrict__ g_tw) {
-  using Cfg = MidCfg<T, N>;
-  constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH, BS = Cfg::BS;
-  __shared__ cplx<T> A[TILE * PITCH];
-  __shared__ cplx<T> B[TILE * PITCH];
-  __shared__ cplx<T> Cc[TILE * PITCH];
-  __shared__ cplx<T> s_tw[2 * N];
+__global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__restrict__ pgp, int ny, int P, int ntile,
+                                                           const cplx<T> *__restrict__ g_tw) {
+  const MidProg &pg = *pgp;
+  using C = cplx<T>;
+  using G = MidCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
+  __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
+  __shared__ C s_tw[2 * N];
   // consecutive column tiles of one ky on one XCD, back to back (shared L2 lines)
   int b = blockIdx.x;
   const int nblk = gridDim.x;
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-  const int o = b / ntile, ct = b - o * ntile, c0 = ct * TILE;
-  const int ncol = (P - c0) < TILE ? (P - c0) : TILE;
-  const int64_t es = (int64_t)ny * P, base = (int64_t)o * P + c0;
-  fft::load_twiddles<T, N>(s_tw, g_tw);
+  const int o = b / ntile, c0 = (b - o * ntile) * COLS;
+  const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
+  const bool act = c0 + col < P;
+  const int kz = act ? c0 + col : P - 1;  // out-of-range lanes read the last column and store nothing
+  const uint32_t es = (uint32_t)ny * P;
+  const uint32_t e0 = (uint32_t)o * P + kz + t * es;  // element index of point t
+  const uint32_t obC = e0 * (uint32_t)sizeof(C), obR = e0 * (uint32_t)sizeof(T);
+  const size_t stC = (size_t)TPL * es * sizeof(C), stR = (size_t)TPL * es * sizeof(T);  // per point i
+  C *X = s_x + col * LP;
+  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  __syncthreads();
+  C acc[E];
   if (pg.multi >= 0)
-    for (int e = threadIdx.x; e < TILE * PITCH; e += BS) Cc[e] = cmk<cplx<T>>(0, 0);
-  for (int i = 0; i < pg.nin; ++i) {
-    const cplx<T> *in = (const cplx<T> *)pg.in[i];
-    for (int e = threadIdx.x; e < TILE * N; e += BS) {
-      const int col = e % TILE, el = e / TILE;
-      if (col < ncol) A[col * PITCH + el] = in[base + el * es + col];
-    }
-    __syncthreads();
-    lines_fft<T, N, -1, BS>(A, s_tw, TILE);
+#pragma unroll
+    for (int i = 0; i < E; ++i) acc[i] = cmk<C>(0, 0);
+  // the next input is loaded while the current one is transformed
+  C nxt[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) nxt[i] = rf::gld<C>((const char *)pg.in[0] + i * stC, obC);
+  for (int ii = 0; ii < pg.nin; ++ii) {
+    C v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = nxt[i];
+    if (ii + 1 < pg.nin)
+#pragma unroll
+      for (int i = 0; i < E; ++i) nxt[i] = rf::gld<C>((const char *)pg.in[ii + 1] + i * stC, obC);
+    rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, X, s_tw, t);
     if (pg.multi >= 0) {
       const MidOut &mo = pg.o[pg.multi];
-      for (int t = 0; t < mo.nterms; ++t) {
-        if (mo.t[t].in != i) continue;
-        for (int e = threadIdx.x; e < TILE * N; e += BS) {
-          const int col = e % TILE, el = e / TILE;
-          if (col >= ncol) continue;
-          const cplx<T> v = mid_coef<T>(mo.t[t], pg, base + el * es + col, el, o, c0 + col, A[col * PITCH + el]);
-          Cc[col * PITCH + el] = cadd(Cc[col * PITCH + el], v);
-        }
+      for (int q = 0; q < mo.nterms; ++q) {
+        if (mo.t[q].in != ii) continue;
+        const MidTerm tm = mo.t[q];
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+          acc[i] = cadd(acc[i], mid_coef<T>(tm, pg, i * stR, obR, t + TPL * i, o, kz, v[i]));
       }
     }
     for (int m = 0; m < pg.nout; ++m) {
-      if (m == pg.multi || pg.o[m].t[0].in != i) continue;
+      if (m == pg.multi || pg.o[m].t[0].in != ii) continue;
       const MidTerm tm = pg.o[m].t[0];
-      for (int e = threadIdx.x; e < TILE * N; e += BS) {
-        const int col = e % TILE, el = e / TILE;
-        if (col < ncol)
-          B[col * PITCH + el] = mid_coef<T>(tm, pg, base + el * es + col, el, o, c0 + col, A[col * PITCH + el]);
-      }
-      __syncthreads();
-      lines_fft<T, N, +1, BS>(B, s_tw, TILE);
-      cplx<T> *out = (cplx<T> *)pg.o[m].out;
-      for (int e = threadIdx.x; e < TILE * N; e += BS) {
-        const int col = e % TILE, el = e / TILE;
-        if (col < ncol) out[base + el * es + col] = B[col * PITCH + el];
-      }
-      __syncthreads();
+      C u[E];
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        u[i] = mid_coef<T>(tm, pg, i * stR, obR, t + TPL * i, o, kz, v[i]);
+      rf::rfft<T, N, G::EM, +1, rf::BlockSync>(u, X, s_tw, t);
+      if (act)
+#pragma unroll
+        for (int i = 0; i < E; ++i) rf::gst<C>((char *)pg.o[m].out + i * stC, obC, u[i]);
     }
-    __syncthreads();
   }
   if (pg.multi >= 0) {
-    __syncthreads();
-    lines_fft<T, N, +1, BS>(Cc, s_tw, TILE);
-    cplx<T> *out = (cplx<T> *)pg.o[pg.multi].out;
-    for (int e = threadIdx.x; e < TILE * N; e += BS) {
-      const int col = e % TILE, el = e / TILE;
-      if (col < ncol) out[base + el * es + col] = Cc[col * PITCH + el];
-    }
+    rf::rfft<T, N, G::EM, +1, rf::BlockSync>(acc, X, s_tw, t);
+    if (act)
+#pragma unroll
+      for (int i = 0; i < E; ++i) rf::gst<C>((char *)pg.o[pg.multi].out + i * stC, obC, acc[i]);
   }
 }
 
@@ -271,75 +322,129 @@ struct ZABArgs {
 
 template <typename T, int N, bool KC, bool BF>
 __global__ void __launch_bounds__(256) k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
-  constexpr int NIN = 1 + (KC ? 1 : 0) + (BF ? 5 : 0), NOUT = 1 + (KC ? 3 : 0);
-  constexpr int NL = NIN > NOUT ? NIN : NOUT;
-  using Cfg = ZCfg<T, N, NL>;
-  constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
+  FUSED_PAIR_SETUP();
   constexpr int IA = KC ? 2 : 1;  // first backflow input
-  __shared__ cplx<T> L[NL * TILE * PITCH];
-  __shared__ cplx<T> s_tw[2 * N];
-  const int64_t p0 = (int64_t)blockIdx.x * TILE;
-  const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
-  fft::load_twiddles<T, N>(s_tw, g_tw);
-  load_merge<T, N, TILE>(L, a.in, NIN, p0, np, P);
-  __syncthreads();
-  lines_fft<T, N, +1, 256>(L, s_tw, NIN * TILE);
-  const T *rho = (const T *)a.rho;
-  for (int e = threadIdx.x; e < np * 2 * N; e += 256) {
-    const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, z = w - which * N;
-    const int64_t c = p0 * 2 * N + e;
-    auto in_at = [&](int i) -> T { return comp(L[(i * TILE + pr) * PITCH + z], which); };
-    const T r = rho[c];
-    const T rb = in_at(0);
-    T vre = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
-    const T Q = (rb * pw.c2 + rb * rb * pw.c3) * r;
-    T g[3] = {0, 0, 0}, G[3] = {0, 0, 0};
-    if constexpr (KC || BF) {
-      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+  const void *rho = a.rho;
+  C v[E];
+  T vre[2][E];
+  if constexpr (BF) {
+    // A = U*rho; per axis: B_d = U*q_d -> W_d = v_d A - B_d (stored),
+    // BF real part sum_d (v_d^2 A - 2 v_d B_d) + C, imaginary part sum_d (g_d/(rho+eps)) W_d / 2
+    T A[2][E], tr[2][E], im[2][E];
+    load_c2r<T, N>(v, X, s_tw, t, a.in.p[IA], p, P);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) g[d] = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+    for (int i = 0; i < E; ++i) {
+      A[0][i] = v[i].x;
+      A[1][i] = v[i].y;
     }
-    if constexpr (KC) {
-      const T rt = in_at(1);
-      const T rst = rt * (-pw.inv_rho0s) + (T)1;
+    const void *psi = a.psi;
+    auto axis = [&](auto dtag) {
+      constexpr int D = decltype(dtag)::value;
+      load_c2r<T, N>(v, X, s_tw, t, a.in.p[IA + 1 + D], p, P);
+      void *wd = D == 0 ? a.wx : (D == 1 ? a.wy : a.wz);
+      T Wv[2][E];
+      const auto qq = q.here();
 #pragma unroll
-      for (int d = 0; d < 3; ++d) G[d] = g[d] * rst;
-      ((T *)a.rhot)[c] = rt;
+      for (int i = 0; i < E; ++i) {
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          const T Bd = part(v[i], w);
+          const T r = qq.template at<T>(rho, w, i);
+          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, r);
+          const T g = grad_d<T, D>(rho, qq, w, i, pw);
+          const T inv = (T)1 / (r + pw.div_eps);
+          const T W = vd * A[w][i] - Bd;
+          Wv[w][i] = W;
+          const T ti = vd * vd * A[w][i] + (T)-2 * vd * Bd;
+          const T ii = (T)0.5 * (g * inv) * W;
+          if constexpr (D == 0) {
+            tr[w][i] = ti;
+            im[w][i] = ii;
+          } else {
+            tr[w][i] += ti;
+            im[w][i] += ii;
+          }
+        }
+      }
+      if (act)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          q.template put<T>(wd, 0, i, Wv[0][i]);
+          q.template put<T>(wd, 1, i, Wv[1][i]);
+        }
+    };
+    axis(std::integral_constant<int, 0>{});
+    axis(std::integral_constant<int, 1>{});
+    axis(std::integral_constant<int, 2>{});
+    load_c2r<T, N>(v, X, s_tw, t, a.in.p[IA + 4], p, P);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        T re = (tr[w][i] + part(v[i], w)) * ((T)-0.5 * pw.mass);
+        if (pw.clamp_bf) re = clampv(re, pw.bfmax);
+        vre[w][i] = re * pw.cbf;
+      }
     }
-    if constexpr (BF) {
-      T v[3];
-      velocity_at((const cplx<T> *)a.psi, c, pw, r, v);
-      const T A = in_at(IA), Bx = in_at(IA + 1), By = in_at(IA + 2), Bz = in_at(IA + 3), Cc = in_at(IA + 4);
-      T t = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * A;
-      t += (T)-2 * v[0] * Bx;
-      t += (T)-2 * v[1] * By;
-      t += (T)-2 * v[2] * Bz;
-      t = t + Cc;
-      T re = t * ((T)-0.5 * pw.mass);
-      if (pw.clamp_bf) re = clampv(re, pw.bfmax);
-      vre += re * pw.cbf;
-      const T Wx = v[0] * A - Bx, Wy = v[1] * A - By, Wz = v[2] * A - Bz;
-      ((T *)a.wx)[c] = Wx;
-      ((T *)a.wy)[c] = Wy;
-      ((T *)a.wz)[c] = Wz;
-      const T inv = (T)1 / (r + pw.div_eps);
-      T im = (T)0.5 * (g[0] * inv) * Wx;
-      im += (T)0.5 * (g[1] * inv) * Wy;
-      im += (T)0.5 * (g[2] * inv) * Wz;
-      ((T *)a.vim)[c] = im;
+    if (act)
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        q.template put<T>(a.vim, 0, i, im[0][i]);
+        q.template put<T>(a.vim, 1, i, im[1][i]);
+      }
+  }
+  // local correlation: rhobar = Pi*rho
+  load_c2r<T, N>(v, X, s_tw, t, a.in.p[0], p, P);
+  const auto ql = q.here();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    T Q[2];
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const T rb = part(v[i], w);
+      const T loc = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
+      if constexpr (BF) vre[w][i] += loc;
+      else vre[w][i] = loc;
+      Q[w] = (rb * pw.c2 + rb * rb * pw.c3) * ql.template at<T>(rho, w, i);
     }
-    ((T *)a.vre)[c] = vre;
-    // outputs replace inputs at this (pair, z, line) only: every input of this point was read above
-    comp(L[(0 * TILE + pr) * PITCH + z], which) = Q;
-    if constexpr (KC) {
-      comp(L[(1 * TILE + pr) * PITCH + z], which) = G[0];
-      comp(L[(2 * TILE + pr) * PITCH + z], which) = G[1];
-      comp(L[(3 * TILE + pr) * PITCH + z], which) = G[2];
+    v[i] = cmk<C>(Q[0], Q[1]);
+  }
+  r2c_store<T, N>(v, X, s_tw, t, a.out.p[0], p, P, act);
+  if constexpr (KC) {
+    // rhot = F*rho (stored), rho_st = 1 - rhot/rho_0s, G_d = (D_d rho) rho_st
+    load_c2r<T, N>(v, X, s_tw, t, a.in.p[1], p, P);
+    T rst[2][E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+#pragma unroll
+      for (int w = 0; w < 2; ++w) rst[w][i] = part(v[i], w) * (-pw.inv_rho0s) + (T)1;
+    }
+    if (act)
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        q.template put<T>(a.rhot, 0, i, v[i].x);
+        q.template put<T>(a.rhot, 1, i, v[i].y);
+      }
+    auto gd = [&](auto dtag) {
+      constexpr int D = decltype(dtag)::value;
+      const auto qq = q.here();
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        v[i] = cmk<C>(grad_d<T, D>(rho, qq, 0, i, pw) * rst[0][i], grad_d<T, D>(rho, qq, 1, i, pw) * rst[1][i]);
+      }
+      r2c_store<T, N>(v, X, s_tw, t, a.out.p[1 + D], p, P, act);
+    };
+    gd(std::integral_constant<int, 0>{});
+    gd(std::integral_constant<int, 1>{});
+    gd(std::integral_constant<int, 2>{});
+  }
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      q.template put<T>(a.vre, 0, i, vre[0][i]);
+      q.template put<T>(a.vre, 1, i, vre[1][i]);
     }
   }
-  __syncthreads();
-  lines_fft<T, N, -1, 256>(L, s_tw, NOUT * TILE);
-  store_split<T, N, TILE>(L, a.out, NOUT, p0, np, P);
 }
 
 // ------------------------------------------------------------------- zBC --
@@ -351,48 +456,59 @@ struct ZBCArgs {
 };
 
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__(256) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
-  constexpr int NIN = 4;
-  using Cfg = ZCfg<T, N, NIN>;
-  constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
-  __shared__ cplx<T> L[NIN * TILE * PITCH];
-  __shared__ cplx<T> s_tw[2 * N];
-  const int64_t p0 = (int64_t)blockIdx.x * TILE;
-  const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
-  fft::load_twiddles<T, N>(s_tw, g_tw);
-  load_merge<T, N, TILE>(L, a.in, NIN, p0, np, P);
-  __syncthreads();
-  lines_fft<T, N, +1, 256>(L, s_tw, NIN * TILE);
-  const T *rho = (const T *)a.rho, *rhot = (const T *)a.rhot;
-  for (int e = threadIdx.x; e < np * 2 * N; e += 256) {
-    const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, z = w - which * N;
-    const int64_t c = p0 * 2 * N + e;
-    auto in_at = [&](int i) -> T { return comp(L[(i * TILE + pr) * PITCH + z], which); };
-    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
-    const T rst = rhot[c] * (-pw.inv_rho0s) + (T)1;
-    T H = 0, t3 = 0;
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+  FUSED_PAIR_SETUP();
+  const void *rho = a.rho, *rhot = a.rhot;
+  C v[E];
+  T H[2][E], t3[2][E];
+  auto axis = [&](auto dtag) {
+    constexpr int D = decltype(dtag)::value;
+    load_c2r<T, N>(v, X, s_tw, t, a.in.p[D], p, P);
+    const auto qq = q.here();
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const T g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
-      const T Pd = (rhot[nb.p[d]] - rhot[nb.m[d]]) * pw.inv2h;  // (dF_d * rho) = D_d (F * rho)
-      const T J = in_at(d);
-      H += g * J;
-      t3 += Pd * J;
-    }
-    ((T *)a.vre)[c] += in_at(3) * rst * pw.ckc - t3 * (pw.ckc * pw.k23);
-    if constexpr (BF) {
-      const T *W[3] = {(const T *)a.wx, (const T *)a.wy, (const T *)a.wz};
-      T im = ((T *)a.vim)[c];
+    for (int i = 0; i < E; ++i) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) im += (T)0.5 * ((W[d][nb.p[d]] - W[d][nb.m[d]]) * pw.inv2h);
-      if (pw.clamp_bf) im = clampv(im, pw.bfmax);
-      ((T *)a.vim)[c] = im * pw.cbf;
+      for (int w = 0; w < 2; ++w) {
+        const T g = grad_d<T, D>(rho, qq, w, i, pw);
+        const T Pd = grad_d<T, D>(rhot, qq, w, i, pw);  // (dF_d * rho) = D_d (F * rho)
+        const T J = part(v[i], w);
+        if constexpr (D == 0) {
+          H[w][i] = g * J;
+          t3[w][i] = Pd * J;
+        } else {
+          H[w][i] += g * J;
+          t3[w][i] += Pd * J;
+        }
+      }
     }
-    comp(L[(0 * TILE + pr) * PITCH + z], which) = H;
+  };
+  axis(std::integral_constant<int, 0>{});
+  axis(std::integral_constant<int, 1>{});
+  axis(std::integral_constant<int, 2>{});
+  load_c2r<T, N>(v, X, s_tw, t, a.in.p[3], p, P);
+  const auto qt = q.here();
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        const T rst = qt.template at<T>(rhot, w, i) * (-pw.inv_rho0s) + (T)1;
+        qt.template put<T>(a.vre, w, i,
+                          qt.template at<T>(a.vre, w, i) + (part(v[i], w) * rst * pw.ckc - t3[w][i] * (pw.ckc * pw.k23)));
+        if constexpr (BF) {
+          T im = qt.template at<T>(a.vim, w, i);
+          im += (T)0.5 * ((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h);
+          im += (T)0.5 * ((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h);
+          im += (T)0.5 * ((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h);
+          if (pw.clamp_bf) im = clampv(im, pw.bfmax);
+          qt.template put<T>(a.vim, w, i, im * pw.cbf);
+        }
+      }
+    }
   }
-  __syncthreads();
-  lines_fft<T, N, -1, 256>(L, s_tw, TILE);
-  store_split<T, N, TILE>(L, a.out, 1, p0, np, P);
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = cmk<C>(H[0][i], H[1][i]);
+  r2c_store<T, N>(v, X, s_tw, t, a.out.p[0], p, P, act);
 }
 
 // -------------------------------------------------------------------- zC --
@@ -413,48 +529,65 @@ struct ZCArgs {
 
 template <typename T, int N, bool STEP>
 __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
-  constexpr int NL = STEP ? 3 : 1;
-  using Cfg = ZCfg<T, N, NL>;
-  constexpr int TILE = Cfg::TILE, PITCH = Cfg::PITCH;
-  __shared__ cplx<T> L[NL * TILE * PITCH];
-  __shared__ cplx<T> s_tw[2 * N];
-  const int64_t p0 = (int64_t)blockIdx.x * TILE;
-  const int np = (int)((npairs - p0) < TILE ? (npairs - p0) : TILE);
-  fft::load_twiddles<T, N>(s_tw, g_tw);
-  load_merge<T, N, TILE>(L, a.in, 1, p0, np, P);
+  using C = cplx<T>;
+  using G = LineCfg<T, N, pair_em<N>()>;
+  constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
+  __shared__ C s_x[G::LINES * LP];
+  __shared__ C s_tw[2 * N];
+  const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
+  const bool act = p0 < npairs;
+  const uint32_t p = act ? p0 : 0;
+  C *X = s_x + ln * LP;
+  rf::load_tw<T, N, 256>(s_tw, g_tw);
   __syncthreads();
-  lines_fft<T, N, +1, 256>(L, s_tw, TILE);
-  const T *vre = (const T *)a.vre, *vim = (const T *)a.vim;
-  for (int e = threadIdx.x; e < np * 2 * N; e += 256) {
-    const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, z = w - which * N;
-    const int64_t c = p0 * 2 * N + e;
-    const T re = vre[c] + comp(L[pr * PITCH + z], which);
-    const T im = vim ? vim[c] : (T)0;
-    if constexpr (!STEP) {
-      cplx<T> *V = (cplx<T> *)a.V;
-      cplx<T> v = V[c];
-      v.x += re;
-      v.y += im;
-      V[c] = v;
-    } else {
-      cplx<T> ve = cmk<cplx<T>>(re - (T)a.mu + (a.vext ? ((const T *)a.vext)[c] : (T)0), im);
-      if (a.vsave) ((cplx<T> *)a.vsave)[c] = ve;
-      if (a.vavg) {
-        const cplx<T> va = ((const cplx<T> *)a.vavg)[c];
-        ve = cmk<cplx<T>>((va.x + ve.x) * (T)0.5, (va.y + ve.y) * (T)0.5);
-      }
-      const cplx<T> ph = fft::vphase<T>(ve, (T)a.dtr, (T)a.dti);
-      ((cplx<T> *)a.phase)[c] = ph;
-      L[((1 + which) * TILE + pr) * PITCH + z] = cmul(((const cplx<T> *)a.psi_src)[c], ph);
+  C v[E];
+  load_c2r<T, N>(v, X, s_tw, t, a.in.p[0], p, P);
+  const uint32_t cb = p * 2u * N + (uint32_t)t;
+  if constexpr (!STEP) {
+    const uint32_t c0 = cb + rf::opaque_zero();
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          const int d = w * N + TPL * i;
+          C u = rf::lde<C>(a.V, c0, d);
+          u.x += rf::lde<T>(a.vre, c0, d) + part(v[i], w);
+          u.y += a.vim ? rf::lde<T>(a.vim, c0, d) : (T)0;
+          rf::ste<C>(a.V, c0, u, d);
+        }
     }
-  }
-  if constexpr (STEP) {
-    __syncthreads();
-    lines_fft<T, N, -1, 256>(L + TILE * PITCH, s_tw, 2 * TILE);
-    cplx<T> *dst = (cplx<T> *)a.psi_dst + p0 * 2 * N;
-    for (int e = threadIdx.x; e < np * 2 * N; e += 256) {
-      const int pr = e / (2 * N), w = e - pr * 2 * N, which = w / N, z = w - which * N;
-      dst[e] = L[((1 + which) * TILE + pr) * PITCH + z];
+  } else {
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const uint32_t c0 = cb + rf::opaque_zero();
+      C u[E], ph[E], vs[E];
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int d = w * N + TPL * i;
+        const T re = rf::lde<T>(a.vre, c0, d) + part(v[i], w);
+        const T im = a.vim ? rf::lde<T>(a.vim, c0, d) : (T)0;
+        C ve = cmk<C>(re - (T)a.mu + (a.vext ? rf::lde<T>(a.vext, c0, d) : (T)0), im);
+        if (a.vsave) vs[i] = ve;
+        if (a.vavg) {
+          const C va = rf::lde<C>(a.vavg, c0, d);
+          ve = cmk<C>((va.x + ve.x) * (T)0.5, (va.y + ve.y) * (T)0.5);
+        }
+        ph[i] = fft::vphase<T>(ve, (T)a.dtr, (T)a.dti);
+        u[i] = cmul(rf::lde<C>(a.psi_src, c0, d), ph[i]);
+      }
+      if (act) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          rf::ste<C>(a.phase, c0, ph[i], w * N + TPL * i);
+          if (a.vsave) rf::ste<C>(a.vsave, c0, vs[i], w * N + TPL * i);
+        }
+      }
+      rf::rfft<T, N, pair_em<N>(), -1, rf::WaveSync>(u, X, s_tw, t);
+      if (act)
+#pragma unroll
+        for (int i = 0; i < E; ++i) rf::ste<C>(a.psi_dst, c0, u[i], w * N + TPL * i);
     }
   }
 }

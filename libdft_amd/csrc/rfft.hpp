@@ -1,0 +1,214 @@
+// Register-resident line FFTs for gfx950.
+//
+// A line of N points is held by TPL = N / E threads, E points each, in the
+// decimated layout
+//
+//     register i of thread t  <->  point t + TPL * i        (i < E, t < TPL).
+//
+// A Stockham autosort transform with radices R_0..R_{S-1} (each dividing E)
+// runs stage s as E / R_s radix-R_s butterflies per thread, entirely in VGPRs.
+// Between stages the points are exchanged through LDS (S - 1 exchanges per
+// transform: one for N = 64..256, two for N = 512); the last stage leaves its
+// outputs in the same decimated layout, in natural frequency order.  So a
+// pass loads a line from HBM straight into registers, transforms, applies its
+// k-space or pointwise work in registers and stores: no LDS round trip for the
+// data itself, and the forward and inverse transforms share one layout.
+//
+// Exchanges are synchronised either per wavefront (all TPL threads of a line
+// in one wave: z passes, where lines are contiguous and t-fastest lanes are
+// already coalesced) or per block (column-tile passes along x and y, where a
+// column-fastest lane order gives 128-byte row segments and a line's threads
+// span several waves).
+#pragma once
+#include "fft.hpp"
+
+namespace dft {
+namespace rf {
+
+// Radix plans.  EMAX bounds the points per thread: 16 for passes that hold one
+// field per thread (more work per exchange), 8 for fused stages that keep
+// several fields live at once (half the registers per field, one more
+// exchange for N = 128 and 256).
+template <int N, int EMAX> struct Radix;
+template <int EMAX> struct Radix<4, EMAX> { static constexpr int S = 1, R0 = 4, R1 = 1, R2 = 1; };
+template <int EMAX> struct Radix<8, EMAX> { static constexpr int S = 1, R0 = 8, R1 = 1, R2 = 1; };
+template <> struct Radix<16, 16> { static constexpr int S = 1, R0 = 16, R1 = 1, R2 = 1; };
+template <> struct Radix<16, 8> { static constexpr int S = 2, R0 = 4, R1 = 4, R2 = 1; };
+template <int EMAX> struct Radix<32, EMAX> { static constexpr int S = 2, R0 = 8, R1 = 4, R2 = 1; };
+template <int EMAX> struct Radix<64, EMAX> { static constexpr int S = 2, R0 = 8, R1 = 8, R2 = 1; };
+template <> struct Radix<128, 16> { static constexpr int S = 2, R0 = 16, R1 = 8, R2 = 1; };
+template <> struct Radix<128, 8> { static constexpr int S = 3, R0 = 8, R1 = 8, R2 = 2; };
+template <> struct Radix<256, 16> { static constexpr int S = 2, R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Radix<256, 8> { static constexpr int S = 3, R0 = 8, R1 = 8, R2 = 4; };
+template <int EMAX> struct Radix<512, EMAX> { static constexpr int S = 3, R0 = 8, R1 = 8, R2 = 8; };
+// 4 points per thread (N <= 256): the fused stages with the most live fields.
+template <> struct Radix<4, 4> { static constexpr int S = 1, R0 = 4, R1 = 1, R2 = 1, R3 = 1; };
+template <> struct Radix<8, 4> { static constexpr int S = 2, R0 = 4, R1 = 2, R2 = 1, R3 = 1; };
+template <> struct Radix<16, 4> { static constexpr int S = 2, R0 = 4, R1 = 4, R2 = 1, R3 = 1; };
+template <> struct Radix<32, 4> { static constexpr int S = 3, R0 = 4, R1 = 4, R2 = 2, R3 = 1; };
+template <> struct Radix<64, 4> { static constexpr int S = 3, R0 = 4, R1 = 4, R2 = 4, R3 = 1; };
+template <> struct Radix<128, 4> { static constexpr int S = 4, R0 = 4, R1 = 4, R2 = 4, R3 = 2; };
+template <> struct Radix<256, 4> { static constexpr int S = 4, R0 = 4, R1 = 4, R2 = 4, R3 = 4; };
+
+template <typename P, typename = void> struct R3of { static constexpr int v = 1; };
+template <typename P> struct R3of<P, decltype((void)P::R3)> { static constexpr int v = P::R3; };
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int N, int EMAX = 16> struct RP {
+  using P = Radix<N, EMAX>;
+  static constexpr int S = P::S, R0 = P::R0, R1 = P::R1, R2 = P::R2, R3 = R3of<P>::v;
+  static constexpr int E = cmax(cmax(R0, R1), cmax(R2, R3));
+  static constexpr int TPL = N / E;
+  // LDS exchange buffer per line (complex elements), one pad slot every 16.
+  static constexpr int LP = N + N / 16 + 2;
+  static_assert(E % R0 == 0 && E % R1 == 0 && E % R2 == 0 && E % R3 == 0, "radices must divide E");
+  static_assert(TPL <= 64, "a line must fit one wavefront");
+};
+
+// The registers of one thread's share of a line.
+template <typename T, int N, int EM> using Regs = cplx<T>[RP<N, EM>::E];
+
+__device__ __forceinline__ int pad(int p) { return p + (p >> 4); }
+
+struct BlockSync {
+  __device__ __forceinline__ static void sync() { __syncthreads(); }
+};
+// All threads of a line are lanes of one wavefront: LDS operations of a wave
+// complete in issue order, so only the compiler has to be kept from moving
+// them across the exchange.
+struct WaveSync {
+  __device__ __forceinline__ static void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+// a * w^(DIR) with w = tw[idx] (forward table; its FP32 rounding residual at
+// tw[N + idx]).
+template <typename T, int N, int DIR>
+__device__ __forceinline__ cplx<T> twmul(cplx<T> a, const cplx<T> *tw, int idx) {
+  cplx<T> w = tw[idx];
+  if (DIR > 0) w.y = -w.y;
+  if constexpr (sizeof(T) == 4) {
+    cplx<T> wl = tw[N + idx];
+    if (DIR > 0) wl.y = -wl.y;
+    return fft::cmul_hl(a, w, wl);
+  } else {
+    return cmul(a, w);
+  }
+}
+
+// One radix-R Stockham stage; NS = product of the earlier radices.
+template <typename T, int N, int EM, int R, int NS, bool LAST, int DIR, class SYNC>
+__device__ __forceinline__ void rstage(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T> *tw, int t) {
+  using C = cplx<T>;
+  constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, NB = E / R;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + TPL * b;
+    C u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = v[b + r * NB];
+    if constexpr (NS > 1) {
+      const int k = j & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) u[r] = twmul<T, N, DIR>(u[r], tw, k * r * (N / (NS * R)));
+    }
+    fft::dft<R, DIR>(u);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * NB] = u[r];
+    } else {
+      const int d = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) X[pad(d + r * NS)] = u[r];
+    }
+  }
+  if constexpr (!LAST) {
+    SYNC::sync();
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = X[pad(t + TPL * i)];
+    SYNC::sync();
+  }
+}
+
+// Unnormalised transform of one line (DIR -1 forward, +1 inverse) in the
+// decimated layout of plan RP<N, EM>.  X: this line's LDS exchange buffer
+// (RP::LP elements); tw: twiddle table (2N entries).  Every thread of the sync
+// group must call it.
+template <typename T, int N, int EM, int DIR, class SYNC>
+__device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T> *tw, int t) {
+  using P = RP<N, EM>;
+  if constexpr (P::S == 1) {
+    rstage<T, N, EM, P::R0, 1, true, DIR, SYNC>(v, X, tw, t);
+  } else if constexpr (P::S == 2) {
+    rstage<T, N, EM, P::R0, 1, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R1, P::R0, true, DIR, SYNC>(v, X, tw, t);
+  } else if constexpr (P::S == 3) {
+    rstage<T, N, EM, P::R0, 1, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R1, P::R0, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R2, P::R0 * P::R1, true, DIR, SYNC>(v, X, tw, t);
+  } else {
+    rstage<T, N, EM, P::R0, 1, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R1, P::R0, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R2, P::R0 * P::R1, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R3, P::R0 * P::R1 * P::R2, true, DIR, SYNC>(v, X, tw, t);
+  }
+}
+
+// Copy the 2N-entry twiddle table to LDS (callers synchronise before use).
+template <typename T, int N, int BS> __device__ __forceinline__ void load_tw(cplx<T> *s_tw, const cplx<T> *g_tw) {
+  for (int i = threadIdx.x; i < 2 * N; i += BS) s_tw[i] = g_tw[i];
+}
+
+// Global accesses as base pointer + 32-bit per-thread byte offset + constant.
+// Every field is below 4 GiB, so the per-thread part is one VGPR; a base that
+// is uniform across the wave (kernel argument, optionally advanced by a
+// uniform stride) stays in SGPRs and the constant folds into the
+// instruction's immediate offset, so a thread addresses all its points
+// through one or two VGPRs instead of one 64-bit address per point.
+template <typename X> __device__ __forceinline__ X gld(const void *base, uint32_t off, int imm = 0) {
+  return *(const X *)((const char *)base + off + imm);
+}
+template <typename X> __device__ __forceinline__ void gst(void *base, uint32_t off, X v, int imm = 0) {
+  *(X *)((char *)base + off + imm) = v;
+}
+// A zero the compiler cannot see through, produced where it is called.  Loads
+// from memory that the kernel has not yet written are treated as invariant
+// and would otherwise be hoisted to the kernel entry (all stages' loads live
+// at once); adding this to a stage's offsets keeps its loads in that stage.
+__device__ __forceinline__ uint32_t opaque_zero() {
+  uint32_t z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+
+// Element-indexed forms: element `e` (per thread) plus constant element delta `d`.
+template <typename X> __device__ __forceinline__ X lde(const void *f, uint32_t e, int d = 0) {
+  return gld<X>(f, e * (uint32_t)sizeof(X), d * (int)sizeof(X));
+}
+template <typename X> __device__ __forceinline__ void ste(void *f, uint32_t e, X v, int d = 0) {
+  gst<X>(f, e * (uint32_t)sizeof(X), v, d * (int)sizeof(X));
+}
+
+// Column-tile geometry for passes along a strided axis: COLS columns per
+// block, column-fastest lanes (plan with up to 16 points per thread).
+template <typename T, int N, int EMAX = 16, int MINCOLS = 8> struct ColCfg {
+  static constexpr int EM = EMAX;
+  static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, LP = RP<N, EM>::LP, S = RP<N, EM>::S;
+  static constexpr int COLS = 256 / TPL < MINCOLS ? MINCOLS : 256 / TPL;
+  static constexpr int BS = COLS * TPL;
+};
+
+// Line geometry for passes along the contiguous axis: one line (or line pair)
+// per TPL consecutive lanes, 256 threads per block.
+template <typename T, int N, int EMAX = 16> struct LineCfg {
+  static constexpr int EM = EMAX;
+  static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, LP = RP<N, EM>::LP, S = RP<N, EM>::S;
+  static constexpr int BS = 256;
+  static constexpr int LINES = BS / TPL;
+};
+
+}  // namespace rf
+}  // namespace dft
