@@ -133,8 +133,8 @@ void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::Combine
 }
 
 template <typename T, int N, int DIR>
-void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cplx<T> *pot, T dtr, T dti, T scale,
-                 double *partial, hipStream_t s) {
+void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cplx<T> *pot, double dtr, double dti,
+                 T scale, double *partial, hipStream_t s) {
   using G = rf::LineCfg<T, N>;
   const int64_t nlines = (int64_t)p.nx * p.ny;
   dim3 grid((unsigned)((nlines + G::LINES - 1) / G::LINES));
@@ -147,7 +147,8 @@ void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cpl
 }
 
 template <typename T, int N>
-void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dti, hipStream_t s) {
+void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, double cx, double cy, double cz, double dtr, double dti,
+                 hipStream_t s) {
   using G = rf::ColCfg<T, N>;
   const int ntile = (p.nz + G::COLS - 1) / G::COLS;
   dim3 grid((unsigned)(p.ny * ntile));
@@ -158,14 +159,17 @@ void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, T cx, T cy, T cz, T dtr, T dt
   DFT_PROF_END(s, "fft_x_kinetic_c2c", 2.0 * (double)p.n() * sizeof(cplx<T>));
 }
 
+// Inverse z pass of a step: transform, scale, and multiply by e^{-iV dt/2}
+// with V = veff, the potential the step's first half used.
 template <typename T, int N>
-void launch_zc2c_phase_inv(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, T scale, hipStream_t s) {
+void launch_zc2c_phase_inv(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *veff, double dtr, double dti, T scale,
+                           hipStream_t s) {
   using G = rf::LineCfg<T, N>;
   const int64_t nlines = (int64_t)p.nx * p.ny;
   dim3 grid((unsigned)((nlines + G::LINES - 1) / G::LINES));
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, +1, true>), grid, dim3(G::BS), 0, s, psi, psi, phase, nlines,
-                     p.twz, (T)0, (T)0, scale, (double *)nullptr);
+  hipLaunchKernelGGL((fft::k_zc2c_pot<T, N, +1>), grid, dim3(G::BS), 0, s, psi, psi, veff, nlines, p.twz, dtr, dti,
+                     scale, (double *)nullptr);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_z_c2c_phase_inv", 3.0 * (double)p.n() * sizeof(cplx<T>));
 }
@@ -234,13 +238,13 @@ void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, do
   by.out[0] = psi;
   const double L2x = std::pow(2.0 * M_PI / (p.nx * p.step), 2), L2y = std::pow(2.0 * M_PI / (p.ny * p.step), 2),
                L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
-  const T cx = (T)(L2x / (2.0 * mass)), cy = (T)(L2y / (2.0 * mass)), cz = (T)(L2z / (2.0 * mass));
-  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, psi, psi, pot, (T)dtr, (T)dti, (T)1, nullptr, s)));
+  const double cx = L2x / (2.0 * mass), cy = L2y / (2.0 * mass), cz = L2z / (2.0 * mass);
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, psi, psi, pot, dtr, dti, (T)1, nullptr, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, (T)1, s)));
-  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, (T)dtr, (T)dti, s)));
+  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
-  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, psi, psi, pot, (T)dtr, (T)dti, inv, partial, s)));
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, psi, psi, pot, dtr, dti, inv, partial, s)));
 }
 
 template <typename T>
@@ -265,19 +269,19 @@ void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double 
   by.out[0] = psi;
   const double L2x = std::pow(2.0 * M_PI / (p.nx * p.step), 2), L2y = std::pow(2.0 * M_PI / (p.ny * p.step), 2),
                L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
-  const T cx = (T)(L2x / (2.0 * mass)), cy = (T)(L2y / (2.0 * mass)), cz = (T)(L2z / (2.0 * mass));
+  const double cx = L2x / (2.0 * mass), cy = L2y / (2.0 * mass), cz = L2z / (2.0 * mass);
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, (T)1, s)));
-  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, (T)dtr, (T)dti, s)));
+  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
-  DFT_DISPATCH_N(p.nz, (launch_zc2c_phase_inv<T, N_>(p, psi, phase, inv, s)));
+  DFT_DISPATCH_N(p.nz, (launch_zc2c_phase_inv<T, N_>(p, psi, phase, dtr, dti, inv, s)));
 }
 
 template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s) {
   fft::Batch b{};
   b.in[0] = out;
   b.out[0] = out;
-  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, in, out, nullptr, (T)0, (T)0, (T)1, nullptr, s)));
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, in, out, nullptr, 0.0, 0.0, (T)1, nullptr, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, 1, 1, p.nz, (T)1, s)));
   DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, b, 1, 0, p.nz, (T)1, s)));
 }

@@ -226,14 +226,14 @@ __device__ __forceinline__ void block_sum_256(double acc, double *partial) {
   if (threadIdx.x == 0) partial[blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
 }
 
-// z pass over a full complex field, multiplying by e^{-iV dt/2} before (DIR<0)
-// or after (DIR>0, together with `scale`) the transform.  With PHASE the
-// `pot` array already holds e^{-iV dt/2}.  After the inverse pass each block
-// writes sum |psi|^2 to partial[block] (FP64).
-template <typename T, int N, int DIR, bool PHASE = false>
+// z pass over a full complex field, multiplying by e^{-iV dt/2} (V = pot,
+// complex) before (DIR<0) or after (DIR>0, together with `scale`) the
+// transform.  After the inverse pass each block writes sum |psi|^2 to
+// partial[block] (FP64) when `partial` is given.
+template <typename T, int N, int DIR>
 __global__ void __launch_bounds__(256)
     k_zc2c_pot(const cplx<T> *src, cplx<T> *dst, const cplx<T> *__restrict__ pot, int64_t nlines,
-               const cplx<T> *__restrict__ g_tw, T dtr, T dti, T scale, double *__restrict__ partial) {
+               const cplx<T> *__restrict__ g_tw, double dtr, double dti, T scale, double *__restrict__ partial) {
   using C = cplx<T>;
   using G = LineCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
@@ -251,8 +251,7 @@ __global__ void __launch_bounds__(256)
   if (DIR < 0 && pot) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const C ph = lde<C>(pot, e0, TPL * i);
-      v[i] = cmul(v[i], PHASE ? ph : vphase<T>(ph, dtr, dti));
+      v[i] = apply_vphase<T>(v[i], lde<C>(pot, e0, TPL * i), dtr, dti);
     }
   }
   __syncthreads();
@@ -262,10 +261,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       v[i] = cscale(v[i], scale);
-      if (pot) {
-        const C ph = lde<C>(pot, e0, TPL * i);
-        v[i] = cmul(v[i], PHASE ? ph : vphase<T>(ph, dtr, dti));
-      }
+      if (pot) v[i] = apply_vphase<T>(v[i], lde<C>(pot, e0, TPL * i), dtr, dti);
       acc += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
     }
     if (!act) acc = 0.0;
@@ -279,15 +275,21 @@ __global__ void __launch_bounds__(256)
 
 // Fused x pass of the propagator: forward x FFT, kinetic phase
 // exp(-i k^2 dt / 2M), inverse x FFT.  Field is [nx][ny][nz] complex.
+// The phase factorises as X(kx) YZ(ky, kz); both factors are evaluated in FP64
+// (X once per block, in LDS), multiplied in FP64, and in FP32 applied as the
+// rounded value plus its residual, so that its modulus is exact: a phase of
+// the same mode is applied every step, and a rounded one would change that
+// mode's amplitude by the same factor every time.
 template <typename T, int N>
 __global__ void __launch_bounds__((ColCfg<T, N>::BS))
-    k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, T cx, T cy, T cz, T dtr,
-           T dti) {
+    k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, double cx, double cy, double cz,
+           double dtr, double dti) {
   using C = cplx<T>;
   using G = ColCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
+  __shared__ double2 s_kx[N];
   const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
   const bool act = c0 + col < nz;
@@ -299,18 +301,34 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
   C v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = gld<C>((const char *)psi + (size_t)i * st, ob);
+  // exp(-i K dt) = exp(K dti) (cos(K dtr) - i sin(K dtr)) for the x part
+  for (int e = threadIdx.x; e < N; e += G::BS) {
+    const int mx = e < N / 2 ? e : e - N;
+    const double K = cx * (double)(mx * mx);
+    double s, c;
+    sincos(K * dtr, &s, &c);
+    const double m = exp(K * dti);
+    s_kx[e] = make_double2(m * c, -m * s);
+  }
   __syncthreads();
   rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
-  // kinetic phase: K = cx*mx^2 + cy*my^2 + cz*mz^2 (signed frequency indices)
   const int my = o < ny / 2 ? o : o - ny;
-  const int kz = cc, mz = kz < nz / 2 ? kz : kz - nz;
-  const double Kyz = (double)cy * (double)(my * my) + (double)cz * (double)(mz * mz);
+  const int mz = cc < nz / 2 ? cc : cc - nz;
+  const double Kyz = cy * (double)(my * my) + cz * (double)(mz * mz);
+  double sy, cyz;
+  sincos(Kyz * dtr, &sy, &cyz);
+  const double myz = exp(Kyz * dti);
+  const double yr = myz * cyz, yi = -myz * sy;
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    const int el = t + TPL * i, mx = el < N / 2 ? el : el - N;
-    const double K = (double)cx * (double)(mx * mx) + Kyz;
-    // exp(-i K dt) = exp(K dti) (cos(K dtr) - i sin(K dtr))
-    v[i] = cmul(v[i], unit_phase<T>(K * (double)dti, -K * (double)dtr));
+    const double2 x = s_kx[t + TPL * i];
+    const double pr = x.x * yr - x.y * yi, pi = x.x * yi + x.y * yr;
+    if constexpr (sizeof(T) == 4) {
+      const float hr = (float)pr, hi = (float)pi;
+      v[i] = cmul_hl(v[i], make_float2(hr, hi), make_float2((float)(pr - hr), (float)(pi - hi)));
+    } else {
+      v[i] = cmul(v[i], cmk<C>(pr, pi));
+    }
   }
   rf::rfft<T, N, G::EM, +1, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
   if (act) {

@@ -41,7 +41,8 @@ int64_t strang_partials(int nx, int ny, int nz, int prec);
 template <typename T>
 void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, hipStream_t s);
 // Rest of a Strang step after the fused z pass: y, x (with kinetic phase), y,
-// and the inverse z pass multiplying by `phase` = e^{-iV dt/2}.
+// and the inverse z pass multiplying by e^{-iV dt/2} with V = `phase` (the
+// potential the first half step used).
 template <typename T>
 void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double dtr, double dti, double mass,
                  hipStream_t s);

@@ -385,7 +385,7 @@ struct dft_ot_device {
   void *plan;                // Plan3D<float> or Plan3D<double>
   void *slot[NSLOT];
   void *kern[4];             // real half spectra (LJ, SA, F, U)
-  void *phase;               // complex: e^{-iV dt/2} of the current step (allocated on first step)
+  void *phase;               // complex: V_eff of the current step, whose e^{-iV dt/2} both z passes apply (allocated on first step)
   void *vsave;               // complex: predictor potential V_n of a predict-correct step
   // x-middle programs of the fused schedule in device memory, and the host
   // copies they were uploaded from (re-uploaded only when one changes)

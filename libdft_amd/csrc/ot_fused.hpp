@@ -521,7 +521,7 @@ struct ZCArgs {
   double mu;
   const void *vavg;      // complex, may be null: V_eff = (vavg + V_eff)/2
   void *vsave;           // complex, may be null: stores V_eff before averaging
-  void *phase;           // complex out: e^{-i V_eff dt/2}
+  void *phase;           // complex out: V_eff of the step (the inverse z pass applies e^{-i V_eff dt/2})
   const void *psi_src;
   void *psi_dst;         // z forward transform of psi_src * phase
   double dtr, dti;
@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, con
 #pragma unroll
     for (int w = 0; w < 2; ++w) {
       const uint32_t c0 = cb + rf::opaque_zero();
-      C u[E], ph[E], vs[E];
+      C u[E], ph[E], vs[E];  // ph: the potential of this half step (re-applied by the inverse z pass)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int d = w * N + TPL * i;
@@ -574,8 +574,8 @@ __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, con
           const C va = rf::lde<C>(a.vavg, c0, d);
           ve = cmk<C>((va.x + ve.x) * (T)0.5, (va.y + ve.y) * (T)0.5);
         }
-        ph[i] = fft::vphase<T>(ve, (T)a.dtr, (T)a.dti);
-        u[i] = cmul(rf::lde<C>(a.psi_src, c0, d), ph[i]);
+        ph[i] = ve;
+        u[i] = fft::apply_vphase<T>(rf::lde<C>(a.psi_src, c0, d), ve, a.dtr, a.dti);
       }
       if (act) {
 #pragma unroll
