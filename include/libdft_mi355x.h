@@ -286,6 +286,22 @@ void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density)
 /* E[rho] of the same parts, FP64 reduction. */
 REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density);
 
+/* ---- adjoint (extensions; no counterpart in the reference) ---- */
+/* Reverse pass of dft_ot_potential (PLAIN and KC models) at the state gwf:
+   rho_bar = (dV_OT/drho)^T v_bar, the transpose of the potential's Jacobian
+   applied to v_bar. */
+void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf);
+/* Gradient of the total energy E[psi_K] = E_kin + E_OT + h^3 sum rho V_ext
+   after nsteps steps of dft_ot_propagate(otf, ., V_ext, mu, time) from gwf
+   (unchanged), with V_ext(r) = amp exp(-|r - r0|^2 / (2 width^2)) on the grid
+   r_i = (i - n/2) step - r0, with respect to (amp, width): grad[0..1].  An
+   adjoint sweep that keeps every `every`-th state and recomputes each segment
+   in the reverse pass (memory: nsteps/every + every states).  PLAIN and KC
+   models; returns 0, or -1 (message on stderr) for backflow models, invalid
+   arguments or lack of device memory. */
+int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
+                        REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad);
+
 /* ---- measurement (extensions; no counterpart in the reference) ---- */
 /* Per-launch HIP-event timing of the library's kernels on its stream. */
 void dft_profile_enable(int on);

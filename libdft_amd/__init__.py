@@ -119,6 +119,8 @@ _SIGS = {
     "grid_bench_copy": (None, ["rg", "rg"]),
     "dft_ot_propagate": (None, ["otf", "wf", "rg", "d", "z"]),
     "dft_ot_propagate_pc": (None, ["otf", "wf", "wf", "rg", "d", "z"]),
+    "dft_ot_potential_vjp": (None, ["otf", "rg", "rg", "wf"]),
+    "dft_ot_adjoint_well": ("i", ["otf", "wf", "d", "d", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
 }
 for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "chempot_pressurized",
            "compressibility", "sound_speed", "chempot3"):
@@ -266,6 +268,16 @@ class Functional:
 
     def bulk(self, name, *args):
         return getattr(lib(), "dft_ot_bulk_" + name)(self.h, *[float(a) for a in args])
+
+    def adjoint_well(self, amp, width, r0, mu, dt, nsteps, every):
+        """dft_ot_adjoint_well: (E[psi_K], [dE/damp, dE/dwidth]) for nsteps steps from self.wave."""
+        E = C.c_double(0.0)
+        g = (C.c_double * 2)()
+        rc = lib().dft_ot_adjoint_well(self.h, self.wave.h, amp, width, r0[0], r0[1], r0[2], mu, Complex.of(dt),
+                                       nsteps, every, C.cast(C.byref(E), C.c_void_p), C.cast(g, C.c_void_p))
+        if rc != 0:
+            raise RuntimeError("dft_ot_adjoint_well failed")
+        return E.value, [g[0], g[1]]
 
     def free(self):
         if self.h:

@@ -286,6 +286,19 @@ template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T
   DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, b, 1, 0, p.nz, (T)1, s)));
 }
 
+// Inverse of c2c_fwd, normalised by 1/N (out of place).
+template <typename T> void c2c_inv(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s) {
+  fft::Batch b{}, bx{};
+  bx.in[0] = in;
+  bx.out[0] = out;
+  b.in[0] = out;
+  b.out[0] = out;
+  DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, +1>(p, bx, 1, 0, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, b, 1, 1, p.nz, (T)1, s)));
+  const T inv = (T)(1.0 / (double)p.n());
+  DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, out, out, nullptr, 0.0, 0.0, inv, nullptr, s)));
+}
+
 #define DFT_INST(T)                                                                                          \
   template struct Plan3D<T>;                                                                                 \
   template void r2c3d<T>(const Plan3D<T> &, int, const T *const *, cplx<T> *const *, hipStream_t);           \
@@ -294,6 +307,7 @@ template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T
   template void strang<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double, double *,   \
                           hipStream_t);                                                                      \
   template void c2c_fwd<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
+  template void c2c_inv<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
   template void ypass<T>(const Plan3D<T> &, int, cplx<T> *const *, int, int, hipStream_t);                  \
   template void strang_tail<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double,        \
                                hipStream_t);

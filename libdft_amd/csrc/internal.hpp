@@ -48,6 +48,8 @@ void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double 
                  hipStream_t s);
 // Full complex forward transform (out of place).
 template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s);
+// Its inverse, normalised by 1/N (out of place).
+template <typename T> void c2c_inv(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s);
 
 // ----------------------------------------------------------- reductions ----
 // Neumaier sum of n FP64 partials (device pointer), synchronising.

@@ -361,6 +361,255 @@ __global__ void k_grad_final(T *__restrict__ grad, const T *__restrict__ vre, co
   }
 }
 
+
+// ------------------------------------------------- adjoint (reverse mode) --
+// Reverse pass through V_OT(rho) for the PLAIN and KC models: given Vbar, the
+// gradient of a scalar with respect to the real potential, accumulate
+// R = (dV/drho)^T Vbar.  Each forward stage is transposed in reverse order:
+// convolutions with symmetric kernels are self-transposed, the periodic
+// central difference D_d transposes to -D_d, and products pass the adjoint
+// to each factor.  Forward (what the unfused potential path computes):
+//   rb = Pi*rho, rt = F*rho, rst = 1 - rt/rho0s, g_d = D_d rho, G_d = g_d rst,
+//   J_d = F*G_d, P_d = D_d rt, T1 = sum_d D_d J_d, H = sum_d g_d J_d,
+//   V = L*rho + c2/2 rb^2 + c3/3 rb^3 + Pi*[(c2 rb + c3 rb^2) rho]
+//       + ckc [rst T1 - k23 sum_d P_d J_d + k23 F*H].
+struct VjpA {
+  const void *vbar, *rho, *rb, *qb, *lv, *hp, *rt, *t1, *J[3], *Pd[3];
+  void *rp, *rbb, *t1b, *pb[3], *jb[3], *gb[3], *rsb;
+};
+
+// Pointwise adjoints of the local terms and of the KC products.
+// qb = Pi*Vbar, lv = L*Vbar, hp = F*Vbar.
+template <typename T, bool KC> __global__ void k_vjp_a(VjpA a, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const T vb = ((const T *)a.vbar)[c], r = ((const T *)a.rho)[c], rb = ((const T *)a.rb)[c];
+    const T qb = ((const T *)a.qb)[c];
+    const T dl = pw.c2 * rb + pw.c3 * rb * rb;  // d(local V)/d rb, and Q/rho
+    ((T *)a.rbb)[c] = vb * dl + qb * r * (pw.c2 + (T)2 * pw.c3 * rb);
+    ((T *)a.rp)[c] = ((const T *)a.lv)[c] + qb * dl;
+    if constexpr (KC) {
+      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+      const T *rho = (const T *)a.rho;
+      const T rst = ((const T *)a.rt)[c] * (-pw.inv_rho0s) + (T)1;
+      const T ck = pw.ckc, ckk = pw.ckc * pw.k23;
+      const T Hb = ckk * ((const T *)a.hp)[c];
+      ((T *)a.t1b)[c] = ck * vb * rst;
+      ((T *)a.rsb)[c] = ck * vb * ((const T *)a.t1)[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+        const T J = ((const T *)a.J[d])[c], P = ((const T *)a.Pd[d])[c];
+        ((T *)a.pb[d])[c] = -ckk * vb * J;
+        ((T *)a.jb[d])[c] = -ckk * vb * P + Hb * g;
+        ((T *)a.gb[d])[c] = Hb * J;
+      }
+    }
+  }
+}
+
+// Jbar_d -= D_d T1bar (adjoint of T1 = sum_d D_d J_d).
+template <typename T> __global__ void k_vjp_b(VjpA a, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *t1b = (const T *)a.t1b;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ((T *)a.jb[d])[c] -= (t1b[nb.p[d]] - t1b[nb.m[d]]) * pw.inv2h;
+  }
+}
+
+// With Gbar_d = F*Jbar_d (in a.J): gbar_d += Gbar_d rst, rstbar += sum_d Gbar_d g_d,
+// rtbar = -rstbar / rho0s (written to a.rsb).
+template <typename T> __global__ void k_vjp_c(VjpA a, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *rho = (const T *)a.rho;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    const T rst = ((const T *)a.rt)[c] * (-pw.inv_rho0s) + (T)1;
+    T rsb = ((const T *)a.rsb)[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+      const T Gb = ((const T *)a.J[d])[c];
+      ((T *)a.gb[d])[c] += Gb * rst;
+      rsb += Gb * g;
+    }
+    ((T *)a.rsb)[c] = -rsb * pw.inv_rho0s;
+  }
+}
+
+// rtbar -= sum_d D_d Pbar_d (adjoint of P_d = D_d rt); Rpart -= sum_d D_d gbar_d (g_d = D_d rho).
+template <typename T> __global__ void k_vjp_d(VjpA a, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    T rt = ((const T *)a.rsb)[c], rp = ((const T *)a.rp)[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T *pb = (const T *)a.pb[d], *gb = (const T *)a.gb[d];
+      rt -= (pb[nb.p[d]] - pb[nb.m[d]]) * pw.inv2h;
+      rp -= (gb[nb.p[d]] - gb[nb.m[d]]) * pw.inv2h;
+    }
+    ((T *)a.rsb)[c] = rt;
+    ((T *)a.rp)[c] = rp;
+  }
+}
+
+// R = rp + conv; out = R (rho adjoint), or psibar += 2 R psi when psi is given.
+template <typename T>
+__global__ void k_vjp_out(T *out, cplx<T> *psib, const cplx<T> *psi, const T *rp, const T *conv, int64_t n) {
+  GRID_STRIDE(c, n) {
+    const T R = rp[c] + (conv ? conv[c] : (T)0);
+    if (psib) {
+      cplx<T> b = psib[c];
+      b.x += (T)2 * R * psi[c].x;
+      b.y += (T)2 * R * psi[c].y;
+      psib[c] = b;
+    } else {
+      out[c] = R;
+    }
+  }
+}
+
+// External potential of a gaussian well, amp exp(-|r - r0|^2 / 2 w^2), with
+// r_i = (i - n/2) h (the grid convention of the reference drivers), and its
+// derivatives with respect to amp and w.
+struct Well {
+  double amp, width, x0, y0, z0, h;
+  int nx, ny, nz;
+};
+__device__ inline void well_at(const Well &w, int64_t c, double &v, double &da, double &dw) {
+  const int k = (int)(c % w.nz), j = (int)((c / w.nz) % w.ny), i = (int)(c / ((int64_t)w.nz * w.ny));
+  const double x = (i - w.nx / 2) * w.h - w.x0, y = (j - w.ny / 2) * w.h - w.y0, z = (k - w.nz / 2) * w.h - w.z0;
+  const double r2 = x * x + y * y + z * z;
+  const double e = exp(-r2 / (2.0 * w.width * w.width));
+  v = w.amp * e;
+  da = e;
+  dw = w.amp * e * r2 / (w.width * w.width * w.width);
+}
+
+// Real field of the well (for the forward steps).
+template <typename T> __global__ void k_well(T *out, Well w, int64_t n) {
+  GRID_STRIDE(c, n) {
+    double v, da, dw;
+    well_at(w, c, v, da, dw);
+    out[c] = (T)v;
+  }
+}
+
+// Sum of two FP64 values over the block into pa[block], pb[block].
+__device__ inline void block_sum2(double a, double b, double *pa, double *pb) {
+  __shared__ double s[2][kBlock];
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + w];
+      s[1][threadIdx.x] += s[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = s[0][0];
+    pb[blockIdx.x] = s[1][0];
+  }
+}
+
+// veff = Re V_OT + V_ext - mu.
+template <typename T> __global__ void k_adj_veff(T *veff, const cplx<T> *vot, Well w, double mu, int64_t n) {
+  GRID_STRIDE(c, n) {
+    double v, da, dw;
+    well_at(w, c, v, da, dw);
+    veff[c] = (T)((double)vot[c].x + v - mu);
+  }
+}
+
+// e^{-i V dt/2} in FP64 for real V.
+__device__ inline double2 phase_of(double v, double dtr, double dti) {
+  double s, c;
+  sincos(-0.5 * v * dtr, &s, &c);
+  const double m = exp(0.5 * v * dti);
+  return make_double2(m * c, m * s);
+}
+
+// out = phase(veff) * in
+template <typename T>
+__global__ void k_adj_phase(cplx<T> *out, const cplx<T> *in, const T *veff, double dtr, double dti, int64_t n) {
+  GRID_STRIDE(c, n) {
+    const double2 f = phase_of((double)veff[c], dtr, dti);
+    const double ar = in[c].x, ai = in[c].y;
+    out[c] = cmk<cplx<T>>((T)(f.x * ar - f.y * ai), (T)(f.x * ai + f.y * ar));
+  }
+}
+
+// Adjoint of psi' = phi u2: u2bar = conj(phi) psibar', phibar = conj(u2) psibar'.
+template <typename T>
+__global__ void k_adj_out(cplx<T> *u2b, cplx<T> *phib, const cplx<T> *psibn, const cplx<T> *u2, const T *veff,
+                          double dtr, double dti, int64_t n) {
+  GRID_STRIDE(c, n) {
+    const double2 f = phase_of((double)veff[c], dtr, dti);
+    const double br = psibn[c].x, bi = psibn[c].y, ur = u2[c].x, ui = u2[c].y;
+    u2b[c] = cmk<cplx<T>>((T)(f.x * br + f.y * bi), (T)(f.x * bi - f.y * br));
+    phib[c] = cmk<cplx<T>>((T)(ur * br + ui * bi), (T)(ur * bi - ui * br));
+  }
+}
+
+// Adjoint of u1 = phi psi and of phi = e^{-i V dt/2}: psibar = conj(phi) u1bar,
+// phibar += conj(psi) u1bar, Vbar = Re(conj(dphi/dV) phibar) with
+// dphi/dV = -i dt/2 phi; per block: sum Vbar dV/damp and sum Vbar dV/dwidth.
+template <typename T>
+__global__ void k_adj_in(cplx<T> *psib, T *vbar, const cplx<T> *u1b, const cplx<T> *psi, const cplx<T> *phib,
+                         const T *veff, double dtr, double dti, Well w, double *pa, double *pw_, int64_t n) {
+  double sa = 0.0, sw = 0.0;
+  GRID_STRIDE(c, n) {
+    const double2 f = phase_of((double)veff[c], dtr, dti);
+    const double br = u1b[c].x, bi = u1b[c].y, pr = psi[c].x, pi = psi[c].y;
+    psib[c] = cmk<cplx<T>>((T)(f.x * br + f.y * bi), (T)(f.x * bi - f.y * br));
+    const double qr = phib[c].x + (pr * br + pi * bi), qi = phib[c].y + (pr * bi - pi * br);
+    // dphi/dV = (-i dt / 2) phi = ((dti f.x + dtr f.y) + i (dti f.y - dtr f.x)) / 2
+    const double dr = 0.5 * (dti * f.x + dtr * f.y), di = 0.5 * (dti * f.y - dtr * f.x);
+    const double vb = dr * qr + di * qi;
+    vbar[c] = (T)vb;
+    double v, da, dw;
+    well_at(w, c, v, da, dw);
+    sa += vb * da;
+    sw += vb * dw;
+  }
+  block_sum2(sa, sw, pa, pw_);
+}
+
+// psibar = 2 h^3 [kin + (grad + V_ext) psi], the gradient of the total energy
+// (kin = F^-1 K F psi); per block: sum rho dV/damp, sum rho dV/dwidth.
+template <typename T>
+__global__ void k_adj_seed(cplx<T> *psib, const cplx<T> *kin, const cplx<T> *psi, const T *grad, Well w, double h3,
+                           double *pa, double *pw_, int64_t n) {
+  double sa = 0.0, sw = 0.0;
+  GRID_STRIDE(c, n) {
+    double v, da, dw;
+    well_at(w, c, v, da, dw);
+    const double pr = psi[c].x, pi = psi[c].y, g = (double)grad[c] + v;
+    psib[c] = cmk<cplx<T>>((T)(2.0 * h3 * ((double)kin[c].x + g * pr)), (T)(2.0 * h3 * ((double)kin[c].y + g * pi)));
+    const double r = pr * pr + pi * pi;
+    sa += r * da;
+    sw += r * dw;
+  }
+  block_sum2(sa, sw, pa, pw_);
+}
+
+// spec *= K (kinetic symbol cx mx^2 + cy my^2 + cz mz^2)
+template <typename T>
+__global__ void k_kin_mul(cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
+    const int mx = i < nx / 2 ? i : i - nx, my = j < ny / 2 ? j : j - ny, mz = k < nz / 2 ? k : k - nz;
+    const double K = cx * (double)mx * mx + cy * (double)my * my + cz * (double)mz * mz;
+    spec[c] = cmk<cplx<T>>((T)(K * spec[c].x), (T)(K * spec[c].y));
+  }
+}
+
 }  // namespace dft
 
 // ============================================================ device state ==
@@ -593,12 +842,13 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     }
     if (KC) t[nt++] = {2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass)};
     fft::Combine c = cmb(nt, t);
-    T *out[1] = {S<T>(d, RB)};
+    // (TMP1 as a real field: RB keeps rhobar for the adjoint pass)
+    T *out[1] = {S<T>(d, TMP1)};
     cplx<T> *tmp[1] = {CS<T>(d, TMP0)};
     c2r3d<T>(p, 3, in, 1, &c, kt, tmp, out, s);
   }
   // P4
-  LAUNCH_PW("ot_accumulate_potential", 5 + ((LOC || KC) ? 1 : 0) + (BF ? 1 : 0), (k_final<T>), (cplx<T> *)V->dev, S<T>(d, VRE), (LOC || KC) ? S<T>(d, RB) : (T *)nullptr,
+  LAUNCH_PW("ot_accumulate_potential", 5 + ((LOC || KC) ? 1 : 0) + (BF ? 1 : 0), (k_final<T>), (cplx<T> *)V->dev, S<T>(d, VRE), (LOC || KC) ? S<T>(d, TMP1) : (T *)nullptr,
             BF ? S<T>(d, VIM) : (T *)nullptr, d->n);
   if (write_back_w && BF) {
     // the reference leaves v_d A - B_d in the velocity inputs (src/ot.c:923-931)
@@ -1177,6 +1427,272 @@ void check_r(dft_ot_functional *otf, rgrid *g, const char *fn) {
   }
 }
 
+
+// -------------------------------------------------------- adjoint sweep --
+// Work space of the reverse pass: real fields, complex fields and half spectra.
+template <typename T> struct AdjWork {
+  int64_t n = 0, nh = 0;
+  T *r[24] = {};
+  cplx<T> *c[4] = {};
+  cplx<T> *spec[6] = {};
+  double *pa = nullptr, *pw = nullptr;
+  bool alloc(int64_t n_, int64_t nh_) {
+    n = n_;
+    nh = nh_;
+    for (auto &x : r)
+      if (!(x = (T *)dmalloc(sizeof(T) * n))) return false;
+    for (auto &x : c)
+      if (!(x = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n))) return false;
+    for (auto &x : spec)
+      if (!(x = (cplx<T> *)dmalloc(sizeof(cplx<T>) * nh))) return false;
+    pa = (double *)dmalloc(sizeof(double) * red_blocks(n));
+    pw = (double *)dmalloc(sizeof(double) * red_blocks(n));
+    return pa && pw;
+  }
+  void release() {
+    for (auto &x : r) dfree(x);
+    for (auto &x : c) dfree(x);
+    for (auto &x : spec) dfree(x);
+    dfree(pa);
+    dfree(pw);
+  }
+};
+
+// Real fields of the work space.
+enum { AQB, ALV, AHP, ARP, ARBB, AT1B, APB, AJB = APB + 3, AGB = AJB + 3, ARSB = AGB + 3, AGBAR, ACONV = AGBAR + 3,
+       AVBAR, AVEFF, ARHO, AGRAD };
+
+// out = sum_m K_m * in_m over real fields (r2c of the inputs, one c2r output per combine).
+template <typename T>
+void adj_conv(dft_ot_device *d, AdjWork<T> &w, int nin, const T *const *in, int nout, const fft::Combine *c,
+              T *const *out) {
+  Plan3D<T> &p = plan_of<T>(d);
+  hipStream_t s = lib_stream();
+  cplx<T> *si[3] = {w.spec[0], w.spec[1], w.spec[2]};
+  r2c3d<T>(p, nin, in, si, s);
+  const cplx<T> *ci[3] = {w.spec[0], w.spec[1], w.spec[2]};
+  cplx<T> *tmp[3] = {w.spec[3], w.spec[4], w.spec[5]};
+  c2r3d<T>(p, nin, ci, nout, c, kerntab<T>(d), tmp, out, s);
+}
+
+// R = (dV_OT/drho)^T vbar at the state whose intermediates potential_impl left
+// in the slots (rho: the density it used).  psib += 2 R psi.
+template <typename T>
+void vjp_ot(dft_ot_functional *otf, const T *vbar, const T *rho, cplx<T> *psib, const cplx<T> *psi, T *rout,
+            AdjWork<T> &w) {
+  dft_ot_device *d = otf->device;
+  const double h3N = d->step * d->step * d->step / (double)d->n;
+  const bool KC = (otf->model & DFT_OT_KC) != 0;
+  PW<T> pw = make_pw<T>(otf, 1.0, otf->div_epsilon, false);
+  {
+    const T *in[1] = {vbar};
+    fft::Combine c[3] = {cmb1(0, K_SA, -1, h3N), cmb1(0, K_LJ, -1, h3N), cmb1(0, K_F, -1, h3N)};
+    T *out[3] = {w.r[AQB], w.r[ALV], w.r[AHP]};
+    adj_conv<T>(d, w, 1, in, KC ? 3 : 2, c, out);
+  }
+  VjpA a{};
+  a.vbar = vbar;
+  a.rho = rho;
+  a.rb = S<T>(d, RB);
+  a.qb = w.r[AQB];
+  a.lv = w.r[ALV];
+  a.hp = w.r[AHP];
+  a.rt = S<T>(d, RT);
+  a.t1 = S<T>(d, SA_);
+  for (int k = 0; k < 3; ++k) {
+    a.J[k] = S<T>(d, SBX + k);
+    a.Pd[k] = S<T>(d, PX + k);
+    a.pb[k] = w.r[APB + k];
+    a.jb[k] = w.r[AJB + k];
+    a.gb[k] = w.r[AGB + k];
+  }
+  a.rp = w.r[ARP];
+  a.rbb = w.r[ARBB];
+  a.t1b = w.r[AT1B];
+  a.rsb = w.r[ARSB];
+  if (KC) {
+    LAUNCH_PW("ot_adjoint_a", 22, (k_vjp_a<T, true>), a, pw);
+    LAUNCH_PW("ot_adjoint_b", 7, (k_vjp_b<T>), a, pw);
+    {
+      const T *in[3] = {w.r[AJB], w.r[AJB + 1], w.r[AJB + 2]};
+      fft::Combine c[3] = {cmb1(0, K_F, -1, h3N), cmb1(1, K_F, -1, h3N), cmb1(2, K_F, -1, h3N)};
+      T *out[3] = {w.r[AGBAR], w.r[AGBAR + 1], w.r[AGBAR + 2]};
+      adj_conv<T>(d, w, 3, in, 3, c, out);
+    }
+    VjpA ac = a;
+    for (int k = 0; k < 3; ++k) ac.J[k] = w.r[AGBAR + k];
+    LAUNCH_PW("ot_adjoint_c", 12, (k_vjp_c<T>), ac, pw);
+    LAUNCH_PW("ot_adjoint_d", 10, (k_vjp_d<T>), a, pw);
+    const T *in[2] = {w.r[ARBB], w.r[ARSB]};
+    fft::Term t[2] = {{0, K_SA, -1, h3N}, {1, K_F, -1, h3N}};
+    fft::Combine c = cmb(2, t);
+    T *out[1] = {w.r[ACONV]};
+    adj_conv<T>(d, w, 2, in, 1, &c, out);
+  } else {
+    LAUNCH_PW("ot_adjoint_a", 6, (k_vjp_a<T, false>), a, pw);
+    const T *in[1] = {w.r[ARBB]};
+    fft::Combine c = cmb1(0, K_SA, -1, h3N);
+    T *out[1] = {w.r[ACONV]};
+    adj_conv<T>(d, w, 1, in, 1, &c, out);
+  }
+  LAUNCH_PW("ot_adjoint_out", 4, (k_vjp_out<T>), rout, psib, psi, (const T *)w.r[ARP], (const T *)w.r[ACONV], d->n);
+}
+
+// V_OT of psi into the complex field w.c[3] (the unfused path, leaving its
+// intermediates in the slots) and veff = Re V_OT + V_ext - mu.
+template <typename T>
+void adj_potential(dft_ot_functional *otf, const cplx<T> *psi, double mass, const Well &wl, double mu, AdjWork<T> &w) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  DFT_HIP_CHECK(hipMemsetAsync(w.c[3], 0, sizeof(cplx<T>) * d->n, s));
+  cgrid V{};
+  V.nx = d->nx;
+  V.ny = d->ny;
+  V.nz = d->nz;
+  V.step = d->step;
+  V.dev = w.c[3];
+  V.dev_elems = d->n;
+  V.prec = d->prec;
+  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, false};
+  potential_impl<T>(otf, &V, psi, mass, w.r[ARHO], nullptr, nullptr, nullptr, parts, false);
+  hipLaunchKernelGGL(k_adj_veff<T>, dim3(grid_for(d->n)), dim3(kBlock), 0, s, w.r[AVEFF], (const cplx<T> *)w.c[3], wl,
+                     mu, d->n);
+  DFT_LAUNCH_CHECK();
+}
+
+// One step backwards: psib holds dJ/dpsi_{n+1} on entry and dJ/dpsi_n on exit;
+// th[0..1] accumulate dJ/d(amp, width) through V_ext of this step.
+template <typename T>
+void adj_step(dft_ot_functional *otf, const cplx<T> *psi, cplx<T> *psib, double mass, const Well &wl, double mu,
+              double dtr, double dti, AdjWork<T> &w, double th[2]) {
+  dft_ot_device *d = otf->device;
+  Plan3D<T> &p = plan_of<T>(d);
+  hipStream_t s = lib_stream();
+  const dim3 g(grid_for(d->n)), b(kBlock);
+  adj_potential<T>(otf, psi, mass, wl, mu, w);
+  // forward: u1 = phi psi, u2 = T u1
+  hipLaunchKernelGGL(k_adj_phase<T>, g, b, 0, s, w.c[0], psi, (const T *)w.r[AVEFF], dtr, dti, d->n);
+  DFT_LAUNCH_CHECK();
+  strang<T>(p, w.c[0], nullptr, dtr, dti, mass, nullptr, s);
+  // reverse: u2bar, phibar; u1bar = T^dagger u2bar (T at -conj(dt)); psibar, Vbar
+  hipLaunchKernelGGL(k_adj_out<T>, g, b, 0, s, w.c[1], w.c[2], (const cplx<T> *)psib, (const cplx<T> *)w.c[0],
+                     (const T *)w.r[AVEFF], dtr, dti, d->n);
+  DFT_LAUNCH_CHECK();
+  strang<T>(p, w.c[1], nullptr, -dtr, dti, mass, nullptr, s);
+  hipLaunchKernelGGL(k_adj_in<T>, dim3((unsigned)red_blocks(d->n)), b, 0, s, psib, w.r[AVBAR],
+                     (const cplx<T> *)w.c[1], psi, (const cplx<T> *)w.c[2], (const T *)w.r[AVEFF], dtr, dti, wl, w.pa,
+                     w.pw, d->n);
+  DFT_LAUNCH_CHECK();
+  th[0] += sum_partials(w.pa, red_blocks(d->n), s);
+  th[1] += sum_partials(w.pw, red_blocks(d->n), s);
+  vjp_ot<T>(otf, w.r[AVBAR], w.r[ARHO], psib, psi, nullptr, w);
+}
+
+// Gradient of E[psi_K] with respect to (amp, width) of the well, for K single
+// steps of dft_ot_propagate from the state in gwf (left unchanged), by an
+// adjoint sweep: the forward pass keeps every `every`-th state, the reverse
+// pass recomputes each segment from its checkpoint.
+template <typename T>
+int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, double dtr, double dti, int64_t K,
+                      int64_t every, double *E, double grad[2]) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  const int64_t n = d->n;
+  const size_t bytes = sizeof(cplx<T>) * n;
+  const int64_t ncp = (K + every - 1) / every;
+  AdjWork<T> w;
+  std::vector<cplx<T> *> cp(ncp, nullptr), seg(every, nullptr);
+  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->P);
+  for (auto &x : cp) ok = ok && (x = (cplx<T> *)dmalloc(bytes)) != nullptr;
+  for (auto &x : seg) ok = ok && (x = (cplx<T> *)dmalloc(bytes)) != nullptr;
+  wf *cur = grid_wf_clone(gwf, (char *)"adjoint state");
+  rgrid *vext = rgrid_clone(otf->density, (char *)"adjoint well");
+  auto cleanup = [&]() {
+    DFT_HIP_CHECK(hipStreamSynchronize(s));
+    w.release();
+    for (auto x : cp) dfree(x);
+    for (auto x : seg) dfree(x);
+    if (cur) grid_wf_free(cur);
+    if (vext) rgrid_free(vext);
+  };
+  if (!ok || !cur || !vext) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_well: out of device memory\n");
+    cleanup();
+    return -1;
+  }
+  hipLaunchKernelGGL(k_well<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, (T *)vext->dev, wl, n);
+  DFT_LAUNCH_CHECK();
+  REAL_COMPLEX dt;
+  __real__ dt = dtr;
+  __imag__ dt = dti;
+  const double mass = gwf->mass;
+  // forward sweep
+  DFT_HIP_CHECK(hipMemcpyAsync(cur->grid->dev, gwf->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
+  for (int64_t it = 0; it < K; ++it) {
+    if (it % every == 0) DFT_HIP_CHECK(hipMemcpyAsync(cp[it / every], cur->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
+    dft_ot_propagate(otf, cur, vext, mu, dt);
+  }
+  // J = E_kin + E_OT + E_ext and its gradient with respect to psi_K
+  Plan3D<T> &p = plan_of<T>(d);
+  const cplx<T> *psiK = (const cplx<T> *)cur->grid->dev;
+  const double h3 = d->step * d->step * d->step;
+  ew_density<T>(psiK, w.r[ARHO], n, s);
+  gradient_impl<T>(otf, w.r[AGRAD], w.r[ARHO]);
+  c2c_fwd<T>(p, psiK, w.c[0], s);
+  {
+    const double cx = std::pow(2.0 * M_PI / (d->nx * d->step), 2) / (2.0 * mass);
+    const double cy = std::pow(2.0 * M_PI / (d->ny * d->step), 2) / (2.0 * mass);
+    const double cz = std::pow(2.0 * M_PI / (d->nz * d->step), 2) / (2.0 * mass);
+    hipLaunchKernelGGL(k_kin_mul<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, w.c[0], d->nx, d->ny, d->nz, cx, cy, cz);
+    DFT_LAUNCH_CHECK();
+  }
+  c2c_inv<T>(p, w.c[0], w.c[1], s);
+  cplx<T> *psib = (cplx<T> *)dmalloc(bytes);
+  if (!psib) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_well: out of device memory\n");
+    cleanup();
+    return -1;
+  }
+  hipLaunchKernelGGL(k_adj_seed<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, psib, (const cplx<T> *)w.c[1],
+                     psiK, (const T *)w.r[AGRAD], wl, h3, w.pa, w.pw, n);
+  DFT_LAUNCH_CHECK();
+  double th[2] = {h3 * sum_partials(w.pa, red_blocks(n), s), h3 * sum_partials(w.pw, red_blocks(n), s)};
+  const double eext = wl.amp * th[0];
+  *E = grid_wf_energy(cur, nullptr) + dft_ot_energy(otf, cur) + eext;
+  // reverse sweep, segment by segment
+  for (int64_t j = ncp - 1; j >= 0; --j) {
+    const int64_t n0 = j * every, n1 = std::min(K, n0 + every);
+    DFT_HIP_CHECK(hipMemcpyAsync(cur->grid->dev, cp[j], bytes, hipMemcpyDeviceToDevice, s));
+    for (int64_t it = n0; it < n1; ++it) {
+      DFT_HIP_CHECK(hipMemcpyAsync(seg[it - n0], cur->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
+      if (it + 1 < n1) dft_ot_propagate(otf, cur, vext, mu, dt);
+    }
+    for (int64_t it = n1 - 1; it >= n0; --it)
+      adj_step<T>(otf, seg[it - n0], psib, mass, wl, mu, dtr, dti, w, th);
+  }
+  grad[0] = th[0];
+  grad[1] = th[1];
+  dfree(psib);
+  cleanup();
+  return 0;
+}
+
+// Standalone reverse pass of V_OT at the density of gwf (for tests): rho_bar = (dV/drho)^T v_bar.
+template <typename T> void potential_vjp_impl(dft_ot_functional *otf, T *rho_bar, const T *v_bar, wf *gwf) {
+  dft_ot_device *d = otf->device;
+  AdjWork<T> w;
+  if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->P)) {
+    fprintf(stderr, "libdft: dft_ot_potential_vjp: out of device memory\n");
+    w.release();
+    return;
+  }
+  Well none{0.0, 1.0, 0.0, 0.0, 0.0, d->step, d->nx, d->ny, d->nz};
+  adj_potential<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, none, 0.0, w);
+  vjp_ot<T>(otf, v_bar, w.r[ARHO], nullptr, nullptr, rho_bar, w);
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  w.release();
+}
+
 }  // namespace
 
 // ================================================================== C ABI ==
@@ -1444,6 +1960,40 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
   DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)otf->workspace1->dev, nullptr, 1.0, (T *)density->dev,
                                                 true, parts, false));
   return rgrid_integral(otf->workspace1);
+}
+
+
+int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
+                        REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
+  check_wf(otf, gwf, "dft_ot_adjoint_well");
+  if (otf->model & DFT_OT_BACKFLOW) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_well: the adjoint covers the PLAIN and KC models (no backflow)\n");
+    return -1;
+  }
+  if (nsteps < 1 || every < 1 || width <= 0.0 || !energy || !grad) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_well: invalid arguments\n");
+    return -1;
+  }
+  dft_ot_device *d = otf->device;
+  Well wl{amp, width, x0, y0, z0, d->step, d->nx, d->ny, d->nz};
+  int rc = 0;
+  double E = 0.0, g[2] = {0.0, 0.0};
+  DFT_BY_PREC(d->prec, rc = adjoint_well_impl<T>(otf, gwf, wl, mu, __real__ time, __imag__ time, nsteps, every, &E, g));
+  *energy = E;
+  grad[0] = g[0];
+  grad[1] = g[1];
+  return rc;
+}
+
+void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf) {
+  check_wf(otf, gwf, "dft_ot_potential_vjp");
+  check_r(otf, rho_bar, "dft_ot_potential_vjp");
+  check_r(otf, v_bar, "dft_ot_potential_vjp");
+  if (otf->model & DFT_OT_BACKFLOW) {
+    fprintf(stderr, "libdft: dft_ot_potential_vjp: the adjoint covers the PLAIN and KC models (no backflow)\n");
+    abort();
+  }
+  DFT_BY_PREC(otf->device->prec, potential_vjp_impl<T>(otf, (T *)rho_bar->dev, (const T *)v_bar->dev, gwf));
 }
 
 }  // extern "C"
