@@ -102,7 +102,8 @@ class Run:
 
 
 def cpu_baseline(n, step, psi0, dt):
-    """One step of the same workload on the FP64 oracle (OpenMP over the host cores)."""
+    """One step of the same workload on the FP64 oracle (OpenMP over the host cores).
+    Returns the baseline record and the oracle's total energy after that step."""
     from oracle import pyoracle as po
     o = po.OracleOT(9, (n, n, n), step, 1, 1)   # kernel sampling cost is setup, not part of a step
     psi = psi0.copy()
@@ -110,11 +111,45 @@ def cpu_baseline(n, step, psi0, dt):
     t0 = time.perf_counter()
     o.single_step(psi, None, mu, dt)
     sec = time.perf_counter() - t0
+    e_oracle = o.observables(psi)["energy"]
     o.close()
-    return {"value": 1.0 / sec, "unit": "steps/s", "cores": po.num_threads(), "kind": "port",
-            "sample": "1 full-OT single step of this workload (%d^3, FP64) on the C/OpenMP oracle; "
-                      "kernels sampled once per cell (setup cost excluded, step cost unchanged); %.1f s"
-                      % (n, sec)}
+    return ({"value": 1.0 / sec, "unit": "steps/s", "cores": po.num_threads(), "kind": "port",
+             "sample": "1 full-OT single step of this workload (%d^3, FP64) on the C/OpenMP oracle; "
+                       "kernels sampled once per cell (setup cost excluded, step cost unchanged); %.1f s"
+                       % (n, sec)}, e_oracle)
+
+
+def copy_calibration(n, step, reps=3):
+    """A plain device copy of one FP32 field (n^3 reads + n^3 writes), for scaling HBM
+    traffic counters (FETCH_SIZE / WRITE_SIZE) read under a profiler: kernel 'k_copy_d'."""
+    lib = L.lib()
+    a = L.RField((n, n, n), step, L.GRID_FP32)
+    b = L.RField((n, n, n), step, L.GRID_FP32)
+    lib.rgrid_constant(a.h, 1.0)
+    for _ in range(reps):
+        lib.grid_bench_copy(b.h, a.h)
+    lib.grid_synchronize()
+    a.free(), b.free()
+    return {"kernel": "k_copy_d", "read_bytes": 4.0 * n ** 3, "write_bytes": 4.0 * n ** 3, "launches": reps}
+
+
+def accuracy(n, step, psi0, dt, k):
+    """FP32 fields + FP64 reductions against the FP64 path: k steps from the same
+    (FP32-representable) state; and one step against the FP64 oracle (kernels sampled
+    once per cell on both sides)."""
+    out = {}
+    res = {}
+    for prec in (L.GRID_FP64, L.GRID_FP32):
+        r = Run(n, step, prec, psi0)
+        r.steps(k, dt)
+        res[prec] = r.energy()
+        r.free()
+    (e64, n64), (e32, n32) = res[L.GRID_FP64], res[L.GRID_FP32]
+    out["energy_rel_err_fp32_vs_fp64"] = abs(e32 - e64) / abs(e64)
+    out["norm_rel_err_fp32_vs_fp64"] = abs(n32 - n64) / abs(n64)
+    out["energy_K"] = e64 * L.AUTOK
+    out["steps_compared"] = k
+    return out
 
 
 def main():
@@ -126,6 +161,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--acc-steps", type=int, default=50,
+                    help="steps of the FP32 vs FP64 energy comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -136,8 +173,10 @@ def main():
     step = 0.3 / L.AUTOANG
     dt = 2.0 / L.AUTOFS
     rho0 = 0.0218360 * L.AUTOANG ** 3
-    psi0 = inputs.vortex_line((n, n, n), step, rho0, noise=1e-3)
+    # the initial state is FP32-representable, so both precisions start from the same state
+    psi0 = inputs.vortex_line((n, n, n), step, rho0, noise=1e-3).astype(np.complex64).astype(complex)
     lib = L.lib()
+    calib = copy_calibration(n, step)
 
     # FP32 fields, FP64 reductions: the measured path
     log("fp32 setup %d^3" % n)
@@ -153,8 +192,6 @@ def main():
     rows = profile_table()
     # the same steps through the reference driver's four calls (unfused potential accumulate + propagate)
     tpat = r32.timed(max(5, args.steps // 5), dt, pattern=True)
-    nsteps32 = args.warmup + args.steps + args.profile_steps + max(5, args.steps // 5)
-    e32, n32 = r32.energy()
     r32.free()
 
     out = {"metric": METRIC, "value": 1.0 / t32, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -177,6 +214,7 @@ def main():
                             / args.profile_steps}
     out["profiled_ms_per_step"] = 1e3 * tprof
     out["driver_pattern_ms_per_step"] = 1e3 * tpat
+    out["copy_calibration"] = calib
     out["kernels"] = [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows[:12]]
 
     if not args.no_fp64:
@@ -184,20 +222,24 @@ def main():
         r64 = Run(n, step, L.GRID_FP64, psi0)
         r64.steps(args.warmup, dt)
         t64 = r64.timed(args.steps, dt)
-        r64.steps(args.profile_steps, dt)
-        r64.steps(max(5, args.steps // 5), dt, pattern=True)
-        e64, n64 = r64.energy()
         r64.free()
         out["fp64_steps_per_s"] = 1.0 / t64
         out["fp32_over_fp64_speedup"] = t64 / t32
-        out["energy_rel_err_fp32_vs_fp64"] = abs(e32 - e64) / abs(e64)
-        out["norm_rel_err_fp32_vs_fp64"] = abs(n32 - n64) / abs(n64)
-        out["energy_K"] = e64 * L.AUTOK
-        out["steps_compared"] = nsteps32
+        if args.acc_steps > 0:
+            log("accuracy: %d steps in FP32 and FP64" % args.acc_steps)
+            out.update(accuracy(n, step, psi0, dt, args.acc_steps))
 
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, one step)")
-        out["cpu_baseline"] = cpu_baseline(n, step, psi0, dt)
+        out["cpu_baseline"], e_oracle = cpu_baseline(n, step, psi0, dt)
+        # one FP32 step from the same state, kernels sampled once per cell as in the oracle run
+        w = L.Wave((n, n, n), step, L.GRID_FP32)
+        w.set(psi0)
+        f = L.Functional(9, w, 1, 1)
+        lib.dft_ot_propagate(f.h, w.h, None, f.bulk("dEdRho", f.p.rho0), L.Complex.of(dt))
+        e1 = lib.grid_wf_energy(w.h, None) + f.energy()
+        f.free(), w.free()
+        out["energy_rel_err_fp32_vs_oracle_1step"] = abs(e1 - e_oracle) / abs(e_oracle)
     print(json.dumps(out))
 
 

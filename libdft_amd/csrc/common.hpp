@@ -62,6 +62,11 @@ __device__ inline void dsincos(double x, double *s, double *c) { sincos(x, s, c)
 
 constexpr int kBlock = 256;
 
+// Row pitch (complex elements) of a half spectrum [nx][ny][nz/2+1]: the row
+// holds nz/2+1 values and is padded to a multiple of 16 (128 bytes in FP32),
+// so that column tiles of the x and y passes read whole cache lines.
+inline int half_pitch(int nz) { return ((nz / 2 + 1) + 15) & ~15; }
+
 // Run BODY with `T` bound to the storage type of a grid precision.
 #define DFT_BY_PREC(prec, ...)  \
   do {                          \

@@ -44,6 +44,7 @@ template <typename T> void Plan3D<T>::init(int nx_, int ny_, int nz_, double ste
   nz = nz_;
   step = step_;
   P = nz / 2 + 1;
+  PP = half_pitch(nz);
   twx = make_twiddles<T>(nx);
   twy = make_twiddles<T>(ny);
   twz = make_twiddles<T>(nz);
@@ -71,7 +72,7 @@ void launch_zr2c(const Plan3D<T> &p, const fft::Batch &b, int nb, hipStream_t s)
   const int64_t npairs = (int64_t)p.nx * p.ny / 2;
   dim3 grid((unsigned)((npairs + G::LINES - 1) / G::LINES), nb);
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zr2c<T, N>), grid, dim3(G::BS), 0, s, b, npairs, p.P, p.twz);
+  hipLaunchKernelGGL((fft::k_zr2c<T, N>), grid, dim3(G::BS), 0, s, b, npairs, p.PP, p.twz);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_z_r2c", (double)nb * (p.n() * sizeof(T) + p.nhalf() * sizeof(cplx<T>)));
 }
@@ -82,25 +83,27 @@ void launch_zc2r(const Plan3D<T> &p, const fft::Batch &b, int nb, T scale, hipSt
   const int64_t npairs = (int64_t)p.nx * p.ny / 2;
   dim3 grid((unsigned)((npairs + G::LINES - 1) / G::LINES), nb);
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_zc2r<T, N>), grid, dim3(G::BS), 0, s, b, npairs, p.P, p.twz, scale);
+  hipLaunchKernelGGL((fft::k_zc2r<T, N>), grid, dim3(G::BS), 0, s, b, npairs, p.PP, p.twz, scale);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_z_c2r", (double)nb * (p.n() * sizeof(T) + p.nhalf() * sizeof(cplx<T>)));
 }
 
-// Strided pass along y (axis 1) or x (axis 0) over columns [0, cols).
+// Strided pass along y (axis 1) or x (axis 0) over columns [0, cols) of rows
+// of `pitch` elements.
 template <typename T, int N, int DIR>
-void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, int cols, T scale, hipStream_t s) {
+void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, int cols, int pitch, T scale,
+                    hipStream_t s) {
   using G = rf::ColCfg<T, N>;
   const int ntile = (cols + G::COLS - 1) / G::COLS;
   int64_t es, os;
   int outer;
   if (axis == 1) {
-    es = cols;
-    os = (int64_t)p.ny * cols;
+    es = pitch;
+    os = (int64_t)p.ny * pitch;
     outer = p.nx;
   } else {
-    es = (int64_t)p.ny * cols;
-    os = cols;
+    es = (int64_t)p.ny * pitch;
+    os = pitch;
     outer = p.ny;
   }
   dim3 grid((unsigned)(outer * ntile), nb);
@@ -126,7 +129,7 @@ void launch_xcombine(const Plan3D<T> &p, const fft::Batch &b, const fft::Combine
       per += sizeof(cplx<T>) + (cs.c[i].t[t].kern >= 0 ? sizeof(T) : 0);
   }
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fft::k_xcombine_inv<T, N>), grid, dim3(G::BS), 0, s, b, cs, kt, p.ny, p.P, ntile,
+  hipLaunchKernelGGL((fft::k_xcombine_inv<T, N>), grid, dim3(G::BS), 0, s, b, cs, kt, p.ny, p.P, p.PP, ntile,
                      p.twx);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_x_combine_c2c", per * (double)p.nhalf());
@@ -199,8 +202,8 @@ void r2c3d(const Plan3D<T> &p, int nb, const T *const *in, cplx<T> *const *out, 
       by.out[i] = out[b0 + i];
     }
     DFT_DISPATCH_N(p.nz, (launch_zr2c<T, N_>(p, bz, cnt, s)));
-    DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, cnt, 1, p.P, (T)1, s)));
-    DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, by, cnt, 0, p.P, (T)1, s)));
+    DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, cnt, 1, p.P, p.PP, (T)1, s)));
+    DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, by, cnt, 0, p.P, p.PP, (T)1, s)));
   }
 }
 
@@ -225,7 +228,7 @@ void c2r3d(const Plan3D<T> &p, int nin, const cplx<T> *const *in, int nout, cons
       cs.c[i] = cmb[b0 + i];
     }
     DFT_DISPATCH_N(p.nx, (launch_xcombine<T, N_>(p, bx, cs, kt, cnt, s)));
-    DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, cnt, 1, p.P, (T)1, s)));
+    DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, cnt, 1, p.P, p.PP, (T)1, s)));
     DFT_DISPATCH_N(p.nz, (launch_zc2r<T, N_>(p, bz, cnt, (T)1, s)));
   }
 }
@@ -240,23 +243,23 @@ void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, do
                L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
   const double cx = L2x / (2.0 * mass), cy = L2y / (2.0 * mass), cz = L2z / (2.0 * mass);
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, psi, psi, pot, dtr, dti, (T)1, nullptr, s)));
-  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
   DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, s)));
-  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, psi, psi, pot, dtr, dti, inv, partial, s)));
 }
 
 template <typename T>
-void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, hipStream_t s) {
+void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, int pitch, hipStream_t s) {
   for (int b0 = 0; b0 < nb; b0 += fft::kMaxBatch) {
     const int cnt = nb - b0 < fft::kMaxBatch ? nb - b0 : fft::kMaxBatch;
     fft::Batch b{};
     for (int i = 0; i < cnt; ++i) b.in[i] = b.out[i] = bufs[b0 + i];
     if (dir < 0) {
-      DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, cnt, 1, cols, (T)1, s)));
+      DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, cnt, 1, cols, pitch, (T)1, s)));
     } else {
-      DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, b, cnt, 1, cols, (T)1, s)));
+      DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, b, cnt, 1, cols, pitch, (T)1, s)));
     }
   }
 }
@@ -270,9 +273,9 @@ void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double 
   const double L2x = std::pow(2.0 * M_PI / (p.nx * p.step), 2), L2y = std::pow(2.0 * M_PI / (p.ny * p.step), 2),
                L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
   const double cx = L2x / (2.0 * mass), cy = L2y / (2.0 * mass), cz = L2z / (2.0 * mass);
-  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
   DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, s)));
-  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
   DFT_DISPATCH_N(p.nz, (launch_zc2c_phase_inv<T, N_>(p, psi, phase, dtr, dti, inv, s)));
 }
@@ -282,8 +285,8 @@ template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T
   b.in[0] = out;
   b.out[0] = out;
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, in, out, nullptr, 0.0, 0.0, (T)1, nullptr, s)));
-  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, 1, 1, p.nz, (T)1, s)));
-  DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, b, 1, 0, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, b, 1, 1, p.nz, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, -1>(p, b, 1, 0, p.nz, p.nz, (T)1, s)));
 }
 
 // Inverse of c2c_fwd, normalised by 1/N (out of place).
@@ -293,8 +296,8 @@ template <typename T> void c2c_inv(const Plan3D<T> &p, const cplx<T> *in, cplx<T
   bx.out[0] = out;
   b.in[0] = out;
   b.out[0] = out;
-  DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, +1>(p, bx, 1, 0, p.nz, (T)1, s)));
-  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, b, 1, 1, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.nx, (launch_strided<T, N_, +1>(p, bx, 1, 0, p.nz, p.nz, (T)1, s)));
+  DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, b, 1, 1, p.nz, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, out, out, nullptr, 0.0, 0.0, inv, nullptr, s)));
 }
@@ -308,7 +311,7 @@ template <typename T> void c2c_inv(const Plan3D<T> &p, const cplx<T> *in, cplx<T
                           hipStream_t);                                                                      \
   template void c2c_fwd<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
   template void c2c_inv<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
-  template void ypass<T>(const Plan3D<T> &, int, cplx<T> *const *, int, int, hipStream_t);                  \
+  template void ypass<T>(const Plan3D<T> &, int, cplx<T> *const *, int, int, int, hipStream_t);                  \
   template void strang_tail<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double,        \
                                hipStream_t);
 DFT_INST(float)

@@ -168,10 +168,11 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
 }
 
 // ------------------------------- inverse x pass with fused k-space combine --
-// Spectra are [nx][ny][P]; x line (ky, kz) at ky*P + kx*ny*P + kz.
+// Spectra are [nx][ny][PP] (P values per row); x line (ky, kz) at ky*PP + kx*ny*PP + kz.
 template <typename T, int N>
 __global__ void __launch_bounds__((ColCfg<T, N>::BS))
-    k_xcombine_inv(Batch b, CombineSet cs, KernTab kt, int ny, int P, int ntile, const cplx<T> *__restrict__ g_tw) {
+    k_xcombine_inv(Batch b, CombineSet cs, KernTab kt, int ny, int P, int PP, int ntile,
+                   const cplx<T> *__restrict__ g_tw) {
   using C = cplx<T>;
   using G = ColCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
@@ -183,8 +184,8 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
   const bool act = c0 + col < P;
   const int cc = act ? c0 + col : P - 1;
-  const uint32_t es = (uint32_t)ny * P;
-  const uint32_t e0 = (uint32_t)o * P + cc + t * es;  // element index of point t
+  const uint32_t es = (uint32_t)ny * PP;
+  const uint32_t e0 = (uint32_t)o * PP + cc + t * es;  // element index of point t
   rf::load_tw<T, N, G::BS>(s_tw, g_tw);
   const T *sx = (const T *)kt.sinx, *sy = (const T *)kt.siny, *sz = (const T *)kt.sinz;
   C v[E];

@@ -284,6 +284,7 @@ namespace {
 
 int64_t npts(INT nx, INT ny, INT nz) { return (int64_t)nx * ny * nz; }
 int64_t half_elems(INT nx, INT ny, INT nz) { return (int64_t)nx * ny * (nz / 2 + 1); }
+int64_t half_storage(INT nx, INT ny, INT nz) { return (int64_t)nx * ny * dft::half_pitch((int)nz); }
 
 bool dims_ok(INT nx, INT ny, INT nz) {
   return fft_size_ok(nx) && fft_size_ok(ny) && fft_size_ok(nz);
@@ -335,7 +336,7 @@ rgrid *rgrid_alloc(INT nx, INT ny, INT nz, REAL step, REAL (*)(rgrid *, INT, INT
   g->step = step;
   g->prec = g_prec;
   // room for a real field or a real half spectrum
-  g->dev_elems = npts(nx, ny, nz) > half_elems(nx, ny, nz) ? npts(nx, ny, nz) : half_elems(nx, ny, nz);
+  g->dev_elems = npts(nx, ny, nz) > half_storage(nx, ny, nz) ? npts(nx, ny, nz) : half_storage(nx, ny, nz);
   g->dev = dmalloc(esize(g->prec) * g->dev_elems);
   if (!g->dev) {
     free(g);
@@ -479,18 +480,29 @@ void rgrid_copy_from_host(rgrid *g, const REAL *src) {
 }
 
 // For a spectral grid (a functional's kernel field) this copies the real half
-// spectrum, nx*ny*(nz/2+1) values.
+// spectrum, nx*ny*(nz/2+1) values (rows are stored with a padded pitch).
 void rgrid_copy_to_host(const rgrid *g, REAL *dst) {
-  const int64_t n = g->spectral ? half_elems(g->nx, g->ny, g->nz) : npts(g->nx, g->ny, g->nz);
   hipStream_t s = lib_stream();
-  if (g->prec == GRID_FP64) {
-    DFT_HIP_CHECK(hipMemcpyAsync(dst, g->dev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    DFT_HIP_CHECK(hipStreamSynchronize(s));
+  const size_t es = esize(g->prec);
+  int64_t n;
+  std::vector<char> t;
+  void *host = dst;
+  if (g->prec != GRID_FP64) {
+    t.resize(es * (g->spectral ? half_elems(g->nx, g->ny, g->nz) : npts(g->nx, g->ny, g->nz)));
+    host = t.data();
+  }
+  if (g->spectral) {
+    const int64_t rows = (int64_t)g->nx * g->ny, P = g->nz / 2 + 1, PP = dft::half_pitch((int)g->nz);
+    n = rows * P;
+    DFT_HIP_CHECK(hipMemcpy2DAsync(host, es * P, g->dev, es * PP, es * P, rows, hipMemcpyDeviceToHost, s));
   } else {
-    std::vector<float> t(n);
-    DFT_HIP_CHECK(hipMemcpyAsync(t.data(), g->dev, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    DFT_HIP_CHECK(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < n; ++i) dst[i] = t[i];
+    n = npts(g->nx, g->ny, g->nz);
+    DFT_HIP_CHECK(hipMemcpyAsync(host, g->dev, es * n, hipMemcpyDeviceToHost, s));
+  }
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+  if (g->prec != GRID_FP64) {
+    const float *f = (const float *)t.data();
+    for (int64_t i = 0; i < n; ++i) dst[i] = f[i];
   }
 }
 

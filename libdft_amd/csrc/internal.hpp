@@ -13,13 +13,15 @@ void dfree(void *p);
 // ------------------------------------------------------------------ FFT ----
 template <typename T> struct Plan3D {
   int nx = 0, ny = 0, nz = 0, P = 0;
+  int PP = 0;  // row pitch of the half spectra (half_pitch(nz) >= P)
   double step = 0;
   cplx<T> *twx = nullptr, *twy = nullptr, *twz = nullptr;
   T *sinx = nullptr, *siny = nullptr, *sinz = nullptr;
   void init(int nx_, int ny_, int nz_, double step_);
   void destroy();
   int64_t n() const { return (int64_t)nx * ny * nz; }
-  int64_t nhalf() const { return (int64_t)nx * ny * P; }
+  int64_t nhalf() const { return (int64_t)nx * ny * P; }      // values (traffic counts)
+  int64_t nhalf_alloc() const { return (int64_t)nx * ny * PP; }  // storage
 };
 
 bool fft_size_ok(int64_t n);
@@ -37,9 +39,10 @@ template <typename T>
 void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, double dti, double mass,
             double *partial, hipStream_t s);
 int64_t strang_partials(int nx, int ny, int nz, int prec);
-// y pass (dir -1 forward, +1 inverse) over nb fields of `cols` columns, in place.
+// y pass (dir -1 forward, +1 inverse) over nb fields of `cols` columns with
+// row pitch `pitch`, in place.
 template <typename T>
-void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, hipStream_t s);
+void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, int pitch, hipStream_t s);
 // Rest of a Strang step after the fused z pass: y, x (with kinetic phase), y,
 // and the inverse z pass multiplying by e^{-iV dt/2} with V = `phase` (the
 // potential the first half step used).

@@ -629,6 +629,7 @@ enum Slot {
 struct dft_ot_device {
   int prec;
   int nx, ny, nz, P;
+  int PP;                    // row pitch of the half spectra (half_pitch(nz))
   int64_t n, slot_elems;
   double step;
   void *plan;                // Plan3D<float> or Plan3D<double>
@@ -889,11 +890,12 @@ template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho,
 }
 
 template <typename T, int N>
-void launch_xmid(const fused::MidProg *pg, int ny, int P, const cplx<T> *tw, hipStream_t s, double bytes) {
+void launch_xmid(const fused::MidProg *pg, int ny, int P, int PP, const cplx<T> *tw, hipStream_t s, double bytes) {
   using Cfg = fused::MidCfg<T, N>;
   const int ntile = (P + Cfg::COLS - 1) / Cfg::COLS;
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fused::k_xmid<T, N>), dim3((unsigned)(ny * ntile)), dim3(Cfg::BS), 0, s, pg, ny, P, ntile, tw);
+  hipLaunchKernelGGL((fused::k_xmid<T, N>), dim3((unsigned)(ny * ntile)), dim3(Cfg::BS), 0, s, pg, ny, P, PP, ntile,
+                     tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_xmid_fwd_kernel_inv", bytes);
 }
@@ -993,7 +995,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   const bool KC = (otf->model & DFT_OT_KC) != 0, BF = (otf->model & DFT_OT_BACKFLOW) != 0;
   const PW<T> pw = make_pw<T>(otf, wf_mass, otf->div_epsilon, true);
   const int64_t npairs = (int64_t)d->nx * d->ny / 2, nh = (int64_t)d->nx * d->ny * d->P, n = d->n;
-  const int P = d->P;
+  const int P = d->P, PP = d->PP;
   const double R = (double)n * sizeof(T), H = (double)nh * sizeof(cplx<T>);
   fft::KernTab kt = kerntab<T>(d);
   auto setk = [&](fused::MidProg &pg) {
@@ -1007,11 +1009,11 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   const int nA = BF ? 5 : 1;
   fused::Ptrs8 oA{};
   for (int i = 0; i < nA; ++i) oA.p[i] = d->slot[SP0 + i];
-  DFT_DISPATCH_N(d->nz, (launch_zA<T, N_>(BF, psi, rho, oA, npairs, P, pw, p.twz, s, 3 * R + nA * H)));
+  DFT_DISPATCH_N(d->nz, (launch_zA<T, N_>(BF, psi, rho, oA, npairs, PP, pw, p.twz, s, 3 * R + nA * H)));
   {
     cplx<T> *b[5];
     for (int i = 0; i < nA; ++i) b[i] = CS<T>(d, SP0 + i);
-    ypass<T>(p, nA, b, -1, P, s);
+    ypass<T>(p, nA, b, -1, P, PP, s);
   }
   // x middle: Pi*rho, [F*rho], [U*rho, U*q_x, U*q_y, U*q_z, U*q4]
   int nB = 0;
@@ -1034,10 +1036,10 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     }
     setk(pg);
     nB = pg.nout;
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 0, pg, s), d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 0, pg, s), d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[8];
     for (int i = 0; i < nB; ++i) b[i] = CS<T>(d, TMP0 + i);
-    ypass<T>(p, nB, b, +1, P, s);
+    ypass<T>(p, nB, b, +1, P, PP, s);
   }
   // ---- stage B: z^-1, local / KC / BF pointwise, z of {Q, G_d}
   const int nQ = KC ? 4 : 1;
@@ -1057,9 +1059,9 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     a.wy = d->slot[WY];
     a.wz = d->slot[WZ];
     const double bytes = nB * H + R + (BF ? 2 * R : 0) + nQ * H + R + (KC ? R : 0) + (BF ? 4 * R : 0);
-    DFT_DISPATCH_N(d->nz, (launch_zAB<T, N_>(KC, BF, a, npairs, P, pw, p.twz, s, bytes)));
+    DFT_DISPATCH_N(d->nz, (launch_zAB<T, N_>(KC, BF, a, npairs, PP, pw, p.twz, s, bytes)));
     cplx<T> *b[4] = {CS<T>(d, Q4), CS<T>(d, QX), CS<T>(d, QY), CS<T>(d, QZ)};
-    ypass<T>(p, nQ, b, -1, P, s);
+    ypass<T>(p, nQ, b, -1, P, PP, s);
   }
   // ---- KC: J_d = F*G_d, T1 = F*sum_d D_d G_d; then H, terms 1 and 3, BF divergence
   if (KC) {
@@ -1079,9 +1081,9 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.o[3].out = d->slot[TMP3];
     pg.multi = 3;
     setk(pg);
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 1, pg, s), d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 1, pg, s), d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[4] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2), CS<T>(d, TMP3)};
-    ypass<T>(p, 4, b, +1, P, s);
+    ypass<T>(p, 4, b, +1, P, PP, s);
     fused::ZBCArgs a{};
     for (int i = 0; i < 4; ++i) a.in.p[i] = d->slot[TMP0 + i];
     a.out.p[0] = d->slot[SH];
@@ -1093,9 +1095,9 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     a.vre = d->slot[VRE];
     a.vim = d->slot[VIM];
     const double bytes = 4 * H + 3 * R + (BF ? 4 * R : 0) + H + R + (BF ? R : 0);
-    DFT_DISPATCH_N(d->nz, (launch_zBC<T, N_>(BF, a, npairs, P, pw, p.twz, s, bytes)));
+    DFT_DISPATCH_N(d->nz, (launch_zBC<T, N_>(BF, a, npairs, PP, pw, p.twz, s, bytes)));
     cplx<T> *hb[1] = {CS<T>(d, SH)};
-    ypass<T>(p, 1, hb, -1, P, s);
+    ypass<T>(p, 1, hb, -1, P, PP, s);
   } else if (BF) {
     StageCIn ci{rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                 S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
@@ -1117,9 +1119,9 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.o[0].t[2] = mterm(2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass));
     pg.o[0].out = d->slot[TMP4];
     setk(pg);
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 2, pg, s), d->ny, P, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 2, pg, s), d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[1] = {CS<T>(d, TMP4)};
-    ypass<T>(p, 1, b, +1, P, s);
+    ypass<T>(p, 1, b, +1, P, PP, s);
     fused::ZCArgs a{};
     a.in.p[0] = d->slot[TMP4];
     a.vre = d->slot[VRE];
@@ -1140,7 +1142,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
       a.dti = st->dti;
       bytes += (st->vext ? R : 0) + (st->vavg ? 2 * R : 0) + (st->vsave ? 2 * R : 0) + 6 * R;
     }
-    DFT_DISPATCH_N(d->nz, (launch_zC<T, N_>(st != nullptr, a, npairs, P, p.twz, s, bytes)));
+    DFT_DISPATCH_N(d->nz, (launch_zC<T, N_>(st != nullptr, a, npairs, PP, p.twz, s, bytes)));
   }
 }
 
@@ -1317,7 +1319,7 @@ template <typename T> void gradient_impl(dft_ot_functional *otf, T *grad, const 
 template <typename T> bool build_kernels(dft_ot_functional *otf, int min_sub, int max_sub) {
   dft_ot_device *d = otf->device;
   hipStream_t s = lib_stream();
-  const int64_t nh = (int64_t)d->nx * d->ny * d->P;
+  const int64_t nh = (int64_t)d->nx * d->ny * d->PP;
   Plan3D<double> p64;
   p64.init(d->nx, d->ny, d->nz, d->step);
   double *field = (double *)dmalloc(sizeof(double) * d->n);
@@ -1391,7 +1393,7 @@ rgrid *spectral_handle(dft_ot_functional *otf, int w, const char *id) {
   g->nz = d->nz;
   g->step = d->step;
   g->dev = d->kern[w];
-  g->dev_elems = (int64_t)d->nx * d->ny * d->P;
+  g->dev_elems = (int64_t)d->nx * d->ny * d->PP;
   g->prec = d->prec;
   g->spectral = 1;
   strncpy(g->id, id, sizeof(g->id) - 1);
@@ -1602,7 +1604,7 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
   const int64_t ncp = (K + every - 1) / every;
   AdjWork<T> w;
   std::vector<cplx<T> *> cp(ncp, nullptr), seg(every, nullptr);
-  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->P);
+  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP);
   for (auto &x : cp) ok = ok && (x = (cplx<T> *)dmalloc(bytes)) != nullptr;
   for (auto &x : seg) ok = ok && (x = (cplx<T> *)dmalloc(bytes)) != nullptr;
   wf *cur = grid_wf_clone(gwf, (char *)"adjoint state");
@@ -1681,7 +1683,7 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
 template <typename T> void potential_vjp_impl(dft_ot_functional *otf, T *rho_bar, const T *v_bar, wf *gwf) {
   dft_ot_device *d = otf->device;
   AdjWork<T> w;
-  if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->P)) {
+  if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->PP)) {
     fprintf(stderr, "libdft: dft_ot_potential_vjp: out of device memory\n");
     w.release();
     return;
@@ -1774,11 +1776,12 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
   d->ny = (int)g->ny;
   d->nz = (int)g->nz;
   d->P = d->nz / 2 + 1;
+  d->PP = half_pitch(d->nz);
   d->n = (int64_t)d->nx * d->ny * d->nz;
   d->step = g->step;
-  d->slot_elems = (int64_t)d->nx * d->ny * 2 * d->P;
+  d->slot_elems = (int64_t)d->nx * d->ny * 2 * d->PP;
   const size_t es = g->prec == GRID_FP32 ? sizeof(float) : sizeof(double);
-  const int64_t nh = (int64_t)d->nx * d->ny * d->P;
+  const int64_t nh = (int64_t)d->nx * d->ny * d->PP;
   bool ok = true;
   for (int i = 0; i < NSLOT && ok; ++i) ok = (d->slot[i] = dmalloc(es * d->slot_elems)) != nullptr;
   for (int i = 0; i < 4 && ok; ++i) ok = (d->kern[i] = dmalloc(es * nh)) != nullptr;

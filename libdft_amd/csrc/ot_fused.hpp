@@ -238,12 +238,13 @@ __device__ __forceinline__ cplx<T> mid_coef(const MidTerm &t, const MidProg &pg,
 // 8 columns per block (256 threads at N = 256).
 template <typename T, int N> using MidCfg = ColCfg<T, N, 8, 8>;
 
-// Spectra are [nx][ny][P] after the y pass; the x line (ky = o, kz = c) has stride ny*P.
+// Spectra are [nx][ny][PP] (P values per row) after the y pass; the x line
+// (ky = o, kz = c) has stride ny*PP.
 // The program is read from device memory: indexed at run time, a by-value
 // kernel argument of this size would be copied to scratch by every thread.
 template <typename T, int N>
-__global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__restrict__ pgp, int ny, int P, int ntile,
-                                                           const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__restrict__ pgp, int ny, int P, int PP,
+                                                           int ntile, const cplx<T> *__restrict__ g_tw) {
   const MidProg &pg = *pgp;
   using C = cplx<T>;
   using G = MidCfg<T, N>;
@@ -258,8 +259,8 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__re
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
   const bool act = c0 + col < P;
   const int kz = act ? c0 + col : P - 1;  // out-of-range lanes read the last column and store nothing
-  const uint32_t es = (uint32_t)ny * P;
-  const uint32_t e0 = (uint32_t)o * P + kz + t * es;  // element index of point t
+  const uint32_t es = (uint32_t)ny * PP;
+  const uint32_t e0 = (uint32_t)o * PP + kz + t * es;  // element index of point t
   const uint32_t obC = e0 * (uint32_t)sizeof(C), obR = e0 * (uint32_t)sizeof(T);
   const size_t stC = (size_t)TPL * es * sizeof(C), stR = (size_t)TPL * es * sizeof(T);  // per point i
   C *X = s_x + col * LP;
