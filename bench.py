@@ -31,6 +31,17 @@ from tests import inputs  # noqa: E402
 
 METRIC = "OT split-op steps/s at 256³ (FP32 vs FP64 GPU, ≤1e-6 rel energy) + HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of the 256^3 FP32 step's kernels, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) of this script, written by --traffic (the command is in the file)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ot_step_256_fp32.md")
+# device symbol (as the profiler names it) of each profiled launch label, FP32 256^3
+SYMBOL = {"ot_xmid_fwd_kernel_inv": "dft::fused::k_xmid<float, 256>",
+          "ot_zAB_c2r_local_kc_bf_r2c": "dft::fused::k_zAB<float, 256, true, true>",
+          "ot_zBC_c2r_kc_terms_r2c": "dft::fused::k_zBC<float, 256, true>",
+          "ot_zA_density_r2c": "dft::fused::k_zA<float, 256, true>",
+          "ot_zC_c2r_phase_z_fwd": "dft::fused::k_zC<float, 256, true>",
+          "fft_z_c2c_phase_inv": "dft::fft::k_zc2c_pot<float, 256, 1>",
+          "fft_x_kinetic_c2c": "dft::fft::k_xkin<float, 256>"}
 
 
 def log(msg):
@@ -133,6 +144,65 @@ def copy_calibration(n, step, reps=3):
     return {"kernel": "k_copy_d", "read_bytes": 4.0 * n ** 3, "write_bytes": 4.0 * n ** 3, "launches": reps}
 
 
+def _kernel_symbol(name):
+    """Kernel name from a profiler CSV without its argument list or 'void ' prefix."""
+    name = name.split("(")[0].strip()
+    return name[5:] if name.startswith("void ") else name
+
+
+def summarize_traffic(fetch_csv, write_csv):
+    """Markdown table of HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    runs of this script: each counter (KiB per dispatch) is scaled by what it reads for the
+    copy kernel k_copy_d, which reads and writes one 256^3 FP32 field (64 MiB each way)."""
+    import csv
+    import collections
+
+    def load(path, counter):
+        agg = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                agg[_kernel_symbol(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return agg
+
+    f, w = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_kib = 4.0 * 256 ** 3 / 1024
+    sf = copy_kib / np.mean(f["k_copy_d"])
+    sw = copy_kib / np.mean(w["k_copy_d"])
+    lines = ["copy kernel k_copy_d (64 MiB read + 64 MiB written): FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB "
+             "per dispatch -> scale fetch x%.4f, write x%.4f" % (np.mean(f["k_copy_d"]), np.mean(w["k_copy_d"]), sf, sw),
+             "", "| kernel | dispatches | fetch MB | write MB | HBM MB per dispatch |", "|---|---|---|---|---|"]
+    for k in sorted(f, key=lambda k: -np.sum(f[k])):
+        if k not in w or k == "k_copy_d":
+            continue
+        fb, wb = np.mean(f[k]) * sf * 1024, np.mean(w[k]) * sw * 1024
+        lines.append("| `%s` | %d | %.1f | %.1f | %.1f |" % (k, len(f[k]), fb / 1e6, wb / 1e6, (fb + wb) / 1e6))
+    return "\n".join(lines)
+
+
+def summarize_kernel_stats(stats_csv, top=16):
+    """Markdown table of a rocprofv3 --kernel-trace --stats kernel_stats.csv."""
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = ["| kernel | calls | avg us | total ms | % |", "|---|---|---|---|---|"]
+    for r in rows[:top]:
+        lines.append("| `%s` | %s | %.1f | %.2f | %s |" % (_kernel_symbol(r["Name"]), r["Calls"],
+                                                         float(r["AverageNs"]) / 1e3,
+                                                         float(r["TotalDurationNs"]) / 1e6, r["Percentage"]))
+    return "\n".join(lines)
+
+
+def measured_traffic(label, n):
+    """HBM bytes per dispatch of a profiled launch label from TRAFFIC_FILE (256^3 FP32 only)."""
+    sym = SYMBOL.get(label)
+    if n != 256 or sym is None or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == "`%s`" % sym:
+            return float(cells[4]) * 1e6
+    return None
+
+
 def accuracy(n, step, psi0, dt, k):
     """FP32 fields + FP64 reductions against the FP64 path: k steps from the same
     (FP32-representable) state; and one step against the FP64 oracle (kernels sampled
@@ -163,7 +233,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--acc-steps", type=int, default=50,
                     help="steps of the FP32 vs FP64 energy comparison (0: skip)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the HBM traffic table from two rocprofv3 --pmc counter_collection.csv files and exit")
+    ap.add_argument("--kernel-stats", metavar="STATS_CSV",
+                    help="print a rocprofv3 kernel_stats.csv as a table and exit")
     args = ap.parse_args()
+    if args.traffic:
+        print(summarize_traffic(*args.traffic))
+        return
+    if args.kernel_stats:
+        print(summarize_kernel_stats(args.kernel_stats))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
@@ -204,7 +284,8 @@ def main():
     # roofline of the dominant kernel (largest total time in the profiled steps)
     dom = rows[0]
     out["roofline"] = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                       "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS,
+                       "traffic": measured_traffic(dom["kernel"], n),
                        "avg_us": dom["avg_us"], "bytes_per_launch": dom["bytes_per_launch"]}
     # whole-step roofline against the canonical 134 R per full-OT step (R = 4N bytes)
     step_bytes = 134 * 4.0 * n ** 3

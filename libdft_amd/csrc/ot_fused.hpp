@@ -252,9 +252,7 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__re
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
   // consecutive column tiles of one ky on one XCD, back to back (shared L2 lines)
-  int b = blockIdx.x;
-  const int nblk = gridDim.x;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int b = rf::xcd_block();
   const int o = b / ntile, c0 = (b - o * ntile) * COLS;
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
   const bool act = c0 + col < P;

@@ -184,6 +184,15 @@ __device__ __forceinline__ uint32_t opaque_zero() {
   return z;
 }
 
+// Logical block index with consecutive indices on one XCD: blocks are
+// dispatched round-robin over the 8 XCDs, each with its own L2, so with this
+// order the column tiles of one transform line (consecutive indices) share an
+// L2.  Identity when the grid is not a multiple of 8.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, n = gridDim.x;
+  return (n & 7) == 0 ? (b & 7) * (n >> 3) + (b >> 3) : b;
+}
+
 // Element-indexed forms: element `e` (per thread) plus constant element delta `d`.
 template <typename X> __device__ __forceinline__ X lde(const void *f, uint32_t e, int d = 0) {
   return gld<X>(f, e * (uint32_t)sizeof(X), d * (int)sizeof(X));
