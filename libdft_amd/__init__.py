@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdft_mi355x.so")
+# LIBDFT_MI355X_SO names another build of the library (e.g. one compiled with other tuning macros)
+LIB_PATH = os.environ.get("LIBDFT_MI355X_SO") or os.path.join(_HERE, "libdft_mi355x.so")
 _LIB = None
 
 GRID_FP32, GRID_FP64 = 1, 2

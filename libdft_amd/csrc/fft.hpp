@@ -173,7 +173,7 @@ struct CombineSet {
   Combine c[kMaxBatch];
 };
 struct KernTab {
-  const void *k[8];    // real half-spectrum arrays [nx][ny][P]
+  const void *k[8];    // real half-spectrum arrays [nx][ny][PP]
   const void *sinx, *siny, *sinz;  // sin(2 pi m / n) / h tables
 };
 

@@ -33,13 +33,13 @@ __device__ __forceinline__ void split_store(const Regs<T, N, EM> &v, cplx<T> *X,
   using C = cplx<T>;
   constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL;
 #pragma unroll
-  for (int i = 0; i < E; ++i) X[rf::pad(t + TPL * i)] = v[i];
+  for (int i = 0; i < E; ++i) X[rf::LayOf<T, N, EM, SYNC>::pad(t + TPL * i)] = v[i];
   SYNC::sync();
   const uint32_t ea = p * 2u * (uint32_t)P + (uint32_t)t, eb = ea + (uint32_t)P;  // rows 2p, 2p+1 at point t
   // points k = t + TPL*i < N/2 for i < E/2; k = N/2 is point i = E/2 of thread 0
   C zm[E / 2 + 1];
 #pragma unroll
-  for (int i = 0; i <= E / 2; ++i) zm[i] = X[rf::pad((N - t - TPL * i) & (N - 1))];
+  for (int i = 0; i <= E / 2; ++i) zm[i] = X[rf::LayOf<T, N, EM, SYNC>::pad((N - t - TPL * i) & (N - 1))];
   SYNC::sync();
   if (act) {
 #pragma unroll
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, co
   const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = p0 < npairs;
   const uint32_t p = act ? p0 : 0;  // out-of-range lanes read pair 0 and store nothing
-  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
   const uint32_t e0 = p * 2u * N + (uint32_t)t;
   C v[E];
 #pragma unroll
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = p0 < npairs;
   const uint32_t p = act ? p0 : 0;
-  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
   C v[E];
   load_merge<T, N, G::EM>(v, in, p, P, t);
   __syncthreads();
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
   const bool act = c0 + col < P;
   const int cc = act ? c0 + col : P - 1;  // out-of-range lanes read the last column and store nothing
-  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   const uint32_t ob = (uint32_t)((o * os + cc + t * es) * (int64_t)sizeof(C));
   const uint32_t st = (uint32_t)(TPL * es * (int64_t)sizeof(C));
   C v[E];
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
   const int cc = act ? c0 + col : P - 1;
   const uint32_t es = (uint32_t)ny * PP;
   const uint32_t e0 = (uint32_t)o * PP + cc + t * es;  // element index of point t
-  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   const T *sx = (const T *)kt.sinx, *sy = (const T *)kt.siny, *sz = (const T *)kt.sinz;
   C v[E];
 #pragma unroll
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t l0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = l0 < nlines;
   const uint32_t l = act ? l0 : 0;
-  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
   const uint32_t e0 = l * (uint32_t)N + (uint32_t)t;
   C v[E];
 #pragma unroll
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
   const uint32_t es = (uint32_t)ny * nz;
   const uint32_t ob = ((uint32_t)o * nz + cc + t * es) * (uint32_t)sizeof(C);
   const uint32_t st = (uint32_t)TPL * es * (uint32_t)sizeof(C);
-  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   C v[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = gld<C>((const char *)psi + (size_t)i * st, ob);

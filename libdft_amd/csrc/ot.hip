@@ -889,13 +889,31 @@ template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho,
   DFT_PROF_END(s, "ot_zA_density_r2c", bytes);
 }
 
-template <typename T, int N>
-void launch_xmid(const fused::MidProg *pg, int ny, int P, int PP, const cplx<T> *tw, hipStream_t s, double bytes) {
+template <typename T, int N, int NK, bool DV>
+void launch_xmid1(const fused::MidProg *pg, int ny, int P, int PP, const cplx<T> *tw, hipStream_t s) {
   using Cfg = fused::MidCfg<T, N>;
-  const int ntile = (P + Cfg::COLS - 1) / Cfg::COLS;
+  hipLaunchKernelGGL((fused::k_xmid<T, N, NK, DV>), dim3((unsigned)fused::mid_blocks<Cfg::COLS>(ny, P)),
+                     dim3(Cfg::BS), 0, s, pg, ny, P, PP, tw);
+}
+
+// x-middle pass of program pg (device copy) that reads nk distinct kernels and
+// has derivative terms or not (host copy hp).
+template <typename T, int N>
+void launch_xmid(const fused::MidProg *pg, const fused::MidProg &hp, int ny, int P, int PP, const cplx<T> *tw,
+                 hipStream_t s, double bytes) {
+  bool deriv = false;
+  for (int m = 0; m < hp.nout; ++m)
+    for (int q = 0; q < hp.o[m].nterms; ++q) deriv = deriv || hp.o[m].t[q].deriv >= 0;
   DFT_PROF_BEGIN(s);
-  hipLaunchKernelGGL((fused::k_xmid<T, N>), dim3((unsigned)(ny * ntile)), dim3(Cfg::BS), 0, s, pg, ny, P, PP, ntile,
-                     tw);
+  if (deriv && hp.nkern == 1) launch_xmid1<T, N, 1, true>(pg, ny, P, PP, tw, s);
+  else if (!deriv && hp.nkern == 1) launch_xmid1<T, N, 1, false>(pg, ny, P, PP, tw, s);
+  else if (!deriv && hp.nkern == 2) launch_xmid1<T, N, 2, false>(pg, ny, P, PP, tw, s);
+  else if (!deriv && hp.nkern == 3) launch_xmid1<T, N, 3, false>(pg, ny, P, PP, tw, s);
+  else {
+    fprintf(stderr, "libdft: x-middle program with %d kernels%s is not instantiated\n", hp.nkern,
+            deriv ? " and derivative terms" : "");
+    abort();
+  }
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_xmid_fwd_kernel_inv", bytes);
 }
@@ -977,13 +995,10 @@ fused::MidTerm mterm(int in, int kern, int deriv, double scale) {
   return t;
 }
 
-// Algorithmic traffic of an x-middle program.
+// Algorithmic traffic of an x-middle program: inputs and outputs, and each
+// distinct kernel spectrum once.
 template <typename T> double mid_bytes(const fused::MidProg &pg, int64_t nh) {
-  double b = (pg.nin + pg.nout) * (double)nh * sizeof(cplx<T>);
-  for (int m = 0; m < pg.nout; ++m)
-    for (int t = 0; t < pg.o[m].nterms; ++t)
-      if (pg.o[m].t[t].kern >= 0) b += (double)nh * sizeof(T);
-  return b;
+  return (pg.nin + pg.nout) * (double)nh * sizeof(cplx<T>) + pg.nkern * (double)nh * sizeof(T);
 }
 
 template <typename T>
@@ -998,8 +1013,24 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   const int P = d->P, PP = d->PP;
   const double R = (double)n * sizeof(T), H = (double)nh * sizeof(cplx<T>);
   fft::KernTab kt = kerntab<T>(d);
+  // terms name kernels by K_*; the program lists the distinct ones and the terms their slots
   auto setk = [&](fused::MidProg &pg) {
-    for (int i = 0; i < 4; ++i) pg.kern[i] = kt.k[i];
+    int slot[4] = {-1, -1, -1, -1};
+    pg.nkern = 0;
+    for (int m = 0; m < pg.nout; ++m)
+      for (int q = 0; q < pg.o[m].nterms; ++q) {
+        int &k = pg.o[m].t[q].kern;
+        if (k < 0) continue;
+        if (slot[k] < 0) {
+          if (pg.nkern == fused::kMidMaxKern) {
+            fprintf(stderr, "libdft: x-middle program uses more than %d kernels\n", fused::kMidMaxKern);
+            abort();
+          }
+          slot[k] = pg.nkern;
+          pg.kern[pg.nkern++] = kt.k[k];
+        }
+        k = slot[k];
+      }
     pg.sinx = kt.sinx;
     pg.siny = kt.siny;
     pg.sinz = kt.sinz;
@@ -1036,7 +1067,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     }
     setk(pg);
     nB = pg.nout;
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 0, pg, s), d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 0, pg, s), pg, d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[8];
     for (int i = 0; i < nB; ++i) b[i] = CS<T>(d, TMP0 + i);
     ypass<T>(p, nB, b, +1, P, PP, s);
@@ -1081,7 +1112,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.o[3].out = d->slot[TMP3];
     pg.multi = 3;
     setk(pg);
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 1, pg, s), d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 1, pg, s), pg, d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[4] = {CS<T>(d, TMP0), CS<T>(d, TMP1), CS<T>(d, TMP2), CS<T>(d, TMP3)};
     ypass<T>(p, 4, b, +1, P, PP, s);
     fused::ZBCArgs a{};
@@ -1119,7 +1150,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.o[0].t[2] = mterm(2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass));
     pg.o[0].out = d->slot[TMP4];
     setk(pg);
-    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 2, pg, s), d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
+    DFT_DISPATCH_N(d->nx, (launch_xmid<T, N_>(mid_prog(d, 2, pg, s), pg, d->ny, P, PP, p.twx, s, mid_bytes<T>(pg, nh))));
     cplx<T> *b[1] = {CS<T>(d, TMP4)};
     ypass<T>(p, 1, b, +1, P, PP, s);
     fused::ZCArgs a{};

@@ -115,7 +115,7 @@ template <int N> constexpr int pair_em() { return N > 256 ? 8 : 4; }
 // Common prologue of the pair kernels.
 #define FUSED_PAIR_SETUP()                                                   \
   using C = cplx<T>;                                                         \
-  using G = LineCfg<T, N, pair_em<N>()>;                                          \
+  using G = LineCfg<T, N, pair_em<N>()>;                                     \
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;                          \
   __shared__ C s_x[G::LINES * LP];                                           \
   __shared__ C s_tw[2 * N];                                                  \
@@ -125,7 +125,7 @@ template <int N> constexpr int pair_em() { return N > 256 ? 8 : 4; }
   /* out-of-range lanes compute on pair 0 and store nothing */               \
   const uint32_t p = act ? p0 : 0;                                           \
   C *X = s_x + ln * LP;                                                      \
-  rf::load_tw<T, N, 256>(s_tw, g_tw);                                        \
+  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);                                  \
   __syncthreads();                                                           \
   const PairPos<T, N, TPL> q(p, t, pw)
 
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256)
 
 // ------------------------------------------------------------------ xmid --
 struct MidTerm {
-  int in, kern, deriv, pad;
+  int in, kern, deriv, pad;  // kern: slot of MidProg::kern (or -1); deriv: axis of i sin(k h)/h (or -1)
   double scale;
 };
 struct MidOut {
@@ -210,68 +210,105 @@ struct MidOut {
   MidTerm t[3];
   void *out;
 };
-constexpr int kMidMaxIn = 6, kMidMaxOut = 8;
+constexpr int kMidMaxIn = 6, kMidMaxOut = 8, kMidMaxKern = 3;
 struct MidProg {
-  int nin, nout, multi, pad;
+  int nin, nout, multi, nkern;
   const void *in[kMidMaxIn];
   MidOut o[kMidMaxOut];
-  const void *kern[4];
+  const void *kern[kMidMaxKern];  // the distinct kernel spectra the terms use
   const void *sinx, *siny, *sinz;
 };
 
-// Coefficient s_t K_t(k) [i sin(k_d h)/h] times a, for the point at byte offset
-// ui + ob of a real half-spectrum field (x frequency el, y frequency o, z
-// frequency kz); ui is uniform across the wave, ob per thread.
-template <typename T>
-__device__ __forceinline__ cplx<T> mid_coef(const MidTerm &t, const MidProg &pg, size_t ui, uint32_t ob, int el, int o,
-                                            int kz, cplx<T> a) {
-  T s = (T)t.scale;
-  if (t.kern >= 0) s *= rf::gld<T>((const char *)pg.kern[t.kern] + ui, ob);
-  if (t.deriv >= 0) {
-    s *= t.deriv == 0 ? ((const T *)pg.sinx)[el] : (t.deriv == 1 ? ((const T *)pg.siny)[o] : ((const T *)pg.sinz)[kz]);
-    return cmk<cplx<T>>(-s * a.y, s * a.x);
-  }
-  return cmk<cplx<T>>(s * a.x, s * a.y);
+// x-middle geometry: 8 points per thread (several spectra are live at once);
+// DFT_XMID_COLS columns per block.
+#ifndef DFT_XMID_COLS
+#define DFT_XMID_COLS 8
+#endif
+template <typename T, int N> using MidCfg = ColCfg<T, N, 8, DFT_XMID_COLS>;
+
+// Blocks of the x-middle pass: ny * ntile tiles of COLS consecutive kz < P-1
+// at one ky, then ceil(ny / COLS) tiles of the Nyquist column kz = P-1 over
+// COLS consecutive ky.
+template <int COLS> __host__ __device__ inline int mid_tiles(int P) { return (P - 1 + COLS - 1) / COLS; }
+template <int COLS> __host__ __device__ inline int mid_blocks(int ny, int P) {
+  return ny * mid_tiles<COLS>(P) + (ny + COLS - 1) / COLS;
 }
 
-// x-middle geometry: 8 points per thread (several spectra are live at once),
-// 8 columns per block (256 threads at N = 256).
-template <typename T, int N> using MidCfg = ColCfg<T, N, 8, 8>;
-
 // Spectra are [nx][ny][PP] (P values per row) after the y pass; the x line
-// (ky = o, kz = c) has stride ny*PP.
+// (ky = o, kz) has stride ny*PP.  Each distinct kernel spectrum is read once
+// per point into registers (NK of them) and shared by the terms that use it.
 // The program is read from device memory: indexed at run time, a by-value
 // kernel argument of this size would be copied to scratch by every thread.
-template <typename T, int N>
+template <typename T, int N, int NK, bool DERIV>
 __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__restrict__ pgp, int ny, int P, int PP,
-                                                           int ntile, const cplx<T> *__restrict__ g_tw) {
+                                                           const cplx<T> *__restrict__ g_tw) {
   const MidProg &pg = *pgp;
   using C = cplx<T>;
   using G = MidCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
-  // consecutive column tiles of one ky on one XCD, back to back (shared L2 lines)
+  // consecutive tiles (one ky) on one XCD, back to back (shared L2 lines)
   const int b = rf::xcd_block();
-  const int o = b / ntile, c0 = (b - o * ntile) * COLS;
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
-  const bool act = c0 + col < P;
-  const int kz = act ? c0 + col : P - 1;  // out-of-range lanes read the last column and store nothing
+  const int ntile = mid_tiles<COLS>(P);
+  int o, kz;
+  bool act;
+  if (b < ny * ntile) {
+    o = b / ntile;
+    const int c = (b - o * ntile) * COLS + col;
+    act = c < P - 1;
+    kz = act ? c : P - 2;  // out-of-range lanes read an in-range column and store nothing
+  } else {
+    const int r = (b - ny * ntile) * COLS + col;
+    act = r < ny;
+    o = act ? r : ny - 1;
+    kz = P - 1;
+  }
   const uint32_t es = (uint32_t)ny * PP;
   const uint32_t e0 = (uint32_t)o * PP + kz + t * es;  // element index of point t
   const uint32_t obC = e0 * (uint32_t)sizeof(C), obR = e0 * (uint32_t)sizeof(T);
   const size_t stC = (size_t)TPL * es * sizeof(C), stR = (size_t)TPL * es * sizeof(T);  // per point i
   C *X = s_x + col * LP;
-  rf::load_tw<T, N, G::BS>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
+  // the first input and the kernels are in flight while the twiddles land
+  C nxt[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) nxt[i] = rf::gld<C>((const char *)pg.in[0] + i * stC, obC);
+  T kv[NK][E];
+#pragma unroll
+  for (int k = 0; k < NK; ++k)
+#pragma unroll
+    for (int i = 0; i < E; ++i) kv[k][i] = rf::gld<T>((const char *)pg.kern[k] + i * stR, obR);
+  T sx[DERIV ? E : 1], sy = 0, sz = 0;
+  if constexpr (DERIV) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) sx[i] = ((const T *)pg.sinx)[t + TPL * i];
+    sy = ((const T *)pg.siny)[o];
+    sz = ((const T *)pg.sinz)[kz];
+  }
+  // coefficient scale * K(k) [* i sin(k_d h)/h] times a, point i
+  auto coef = [&](const MidTerm &tm, int i, C a) {
+    T s = (T)tm.scale;
+    if (NK > 0 && tm.kern >= 0) {
+      T k = kv[0][i];
+#pragma unroll
+      for (int j = 1; j < NK; ++j)
+        if (tm.kern == j) k = kv[j][i];
+      s *= k;
+    }
+    if (DERIV && tm.deriv >= 0) {
+      s *= tm.deriv == 0 ? sx[DERIV ? i : 0] : (tm.deriv == 1 ? sy : sz);
+      return cmk<C>(-s * a.y, s * a.x);
+    }
+    return cmk<C>(s * a.x, s * a.y);
+  };
   __syncthreads();
   C acc[E];
   if (pg.multi >= 0)
 #pragma unroll
     for (int i = 0; i < E; ++i) acc[i] = cmk<C>(0, 0);
   // the next input is loaded while the current one is transformed
-  C nxt[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i) nxt[i] = rf::gld<C>((const char *)pg.in[0] + i * stC, obC);
   for (int ii = 0; ii < pg.nin; ++ii) {
     C v[E];
 #pragma unroll
@@ -286,8 +323,7 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__re
         if (mo.t[q].in != ii) continue;
         const MidTerm tm = mo.t[q];
 #pragma unroll
-        for (int i = 0; i < E; ++i)
-          acc[i] = cadd(acc[i], mid_coef<T>(tm, pg, i * stR, obR, t + TPL * i, o, kz, v[i]));
+        for (int i = 0; i < E; ++i) acc[i] = cadd(acc[i], coef(tm, i, v[i]));
       }
     }
     for (int m = 0; m < pg.nout; ++m) {
@@ -295,8 +331,7 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__re
       const MidTerm tm = pg.o[m].t[0];
       C u[E];
 #pragma unroll
-      for (int i = 0; i < E; ++i)
-        u[i] = mid_coef<T>(tm, pg, i * stR, obR, t + TPL * i, o, kz, v[i]);
+      for (int i = 0; i < E; ++i) u[i] = coef(tm, i, v[i]);
       rf::rfft<T, N, G::EM, +1, rf::BlockSync>(u, X, s_tw, t);
       if (act)
 #pragma unroll
@@ -538,7 +573,7 @@ __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, con
   const bool act = p0 < npairs;
   const uint32_t p = act ? p0 : 0;
   C *X = s_x + ln * LP;
-  rf::load_tw<T, N, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
   __syncthreads();
   C v[E];
   load_c2r<T, N>(v, X, s_tw, t, a.in.p[0], p, P);

@@ -20,6 +20,8 @@
 // column-fastest lane order gives 128-byte row segments and a line's threads
 // span several waves).
 #pragma once
+#include <type_traits>
+
 #include "fft.hpp"
 
 namespace dft {
@@ -59,16 +61,12 @@ template <int N, int EMAX = 16> struct RP {
   static constexpr int S = P::S, R0 = P::R0, R1 = P::R1, R2 = P::R2, R3 = R3of<P>::v;
   static constexpr int E = cmax(cmax(R0, R1), cmax(R2, R3));
   static constexpr int TPL = N / E;
-  // LDS exchange buffer per line (complex elements), one pad slot every 16.
-  static constexpr int LP = N + N / 16 + 2;
   static_assert(E % R0 == 0 && E % R1 == 0 && E % R2 == 0 && E % R3 == 0, "radices must divide E");
   static_assert(TPL <= 64, "a line must fit one wavefront");
 };
 
 // The registers of one thread's share of a line.
 template <typename T, int N, int EM> using Regs = cplx<T>[RP<N, EM>::E];
-
-__device__ __forceinline__ int pad(int p) { return p + (p >> 4); }
 
 struct BlockSync {
   __device__ __forceinline__ static void sync() { __syncthreads(); }
@@ -83,6 +81,47 @@ struct WaveSync {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 };
+
+// LDS layout of one line's exchange buffer: point p sits at element pad(p),
+// LP elements per line.  LDS serves 128 bytes per cycle over 32 four-byte
+// banks, i.e. 16 lanes of 8-byte (FP32 complex) or 8 lanes of 16-byte (FP64
+// complex) accesses; the layout is chosen per plan so that both the stage
+// writes (points strided by the earlier radices) and the reads (consecutive
+// points) of those lanes fall in distinct banks.
+//  - wave-synchronised lines (a line is one wavefront's lanes, t fastest),
+//    N >= 64 with 4 or 8 points per thread, or N >= 256: XOR swizzle inside
+//    each 16-element block, p ^ (5 * (p / 16) mod 16), no padding;
+//  - column tiles (lanes column-fastest, one line per column), N >= 64: one
+//    pad element every 8, and a line pitch that offsets neighbouring columns
+//    by an odd number of elements (by 2 mod 4 for 8 points per thread and
+//    N >= 256 in FP32);
+//  - otherwise one pad element every 16.
+template <int N, int EM, bool WAVE, int W> struct Lay {
+  static constexpr bool XOR = WAVE && N >= 64 && (EM <= 8 || N >= 256);
+  static constexpr bool ADD3 = !WAVE && N >= 64;
+  static constexpr int MAXP = XOR ? N - 1 : (ADD3 ? (N - 1) + (N - 1) / 8 : (N - 1) + (N - 1) / 16);
+  static constexpr int LP = XOR ? N
+                            : WAVE ? N + N / 16 + 2
+                                   : (ADD3 && RP<N, EM>::E == 8 && N >= 256 && W == 8 ? N + N / 8 + 2
+                                                                                       : ((MAXP + 1) | 1));
+  static_assert(LP > MAXP, "line pitch too small for the layout");
+  __device__ __forceinline__ static int pad(int p) {
+    if constexpr (XOR) return p ^ (((p >> 4) * 5) & 15);
+    else if constexpr (ADD3) return p + (p >> 3);
+    else return p + (p >> 4);
+  }
+};
+template <class SYNC> constexpr bool kWave = std::is_same<SYNC, WaveSync>::value;
+template <typename T, int N, int EM, class SYNC> using LayOf = Lay<N, EM, kWave<SYNC>, (int)sizeof(cplx<T>)>;
+
+// Inter-stage twiddles in LDS, stage by stage: stage s >= 1 (radix R_s, NS_s =
+// R_0 ... R_{s-1}) uses w^(k r N / (NS_s R_s)) for r = 1..R_s-1, k < NS_s, at
+// entry (NS_s - R_0) + (r - 1) NS_s + k, so the lanes of a line (consecutive k)
+// read consecutive entries.  N - R_0 entries in all; the FP32 rounding
+// residuals follow at offset N.
+template <int N, int EM> __device__ __forceinline__ int tw_entry(int NS, int r, int k) {
+  return (NS - RP<N, EM>::R0) + (r - 1) * NS + k;
+}
 
 // a * w^(DIR) with w = tw[idx] (forward table; its FP32 rounding residual at
 // tw[N + idx]).
@@ -113,7 +152,7 @@ __device__ __forceinline__ void rstage(Regs<T, N, EM> &v, cplx<T> *X, const cplx
     if constexpr (NS > 1) {
       const int k = j & (NS - 1);
 #pragma unroll
-      for (int r = 1; r < R; ++r) u[r] = twmul<T, N, DIR>(u[r], tw, k * r * (N / (NS * R)));
+      for (int r = 1; r < R; ++r) u[r] = twmul<T, N, DIR>(u[r], tw, tw_entry<N, EM>(NS, r, k));
     }
     fft::dft<R, DIR>(u);
     if constexpr (LAST) {
@@ -122,20 +161,20 @@ __device__ __forceinline__ void rstage(Regs<T, N, EM> &v, cplx<T> *X, const cplx
     } else {
       const int d = (j / NS) * NS * R + (j & (NS - 1));
 #pragma unroll
-      for (int r = 0; r < R; ++r) X[pad(d + r * NS)] = u[r];
+      for (int r = 0; r < R; ++r) X[LayOf<T, N, EM, SYNC>::pad(d + r * NS)] = u[r];
     }
   }
   if constexpr (!LAST) {
     SYNC::sync();
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = X[pad(t + TPL * i)];
+    for (int i = 0; i < E; ++i) v[i] = X[LayOf<T, N, EM, SYNC>::pad(t + TPL * i)];
     SYNC::sync();
   }
 }
 
 // Unnormalised transform of one line (DIR -1 forward, +1 inverse) in the
 // decimated layout of plan RP<N, EM>.  X: this line's LDS exchange buffer
-// (RP::LP elements); tw: twiddle table (2N entries).  Every thread of the sync
+// (LayOf::LP elements); tw: twiddle table of load_tw.  Every thread of the sync
 // group must call it.
 template <typename T, int N, int EM, int DIR, class SYNC>
 __device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T> *tw, int t) {
@@ -157,9 +196,24 @@ __device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T
   }
 }
 
-// Copy the 2N-entry twiddle table to LDS (callers synchronise before use).
-template <typename T, int N, int BS> __device__ __forceinline__ void load_tw(cplx<T> *s_tw, const cplx<T> *g_tw) {
-  for (int i = threadIdx.x; i < 2 * N; i += BS) s_tw[i] = g_tw[i];
+// Gather the stage-ordered twiddles of plan RP<N, EM> (see tw_entry) from the
+// natural table g_tw (w^m at m, FP32 residual at N + m) into LDS; callers
+// synchronise before use.
+template <typename T, int N, int EM, int BS>
+__device__ __forceinline__ void load_tw(cplx<T> *s_tw, const cplx<T> *g_tw) {
+  using P = RP<N, EM>;
+  constexpr int NS1 = P::R0, NS2 = NS1 * P::R1, NS3 = NS2 * P::R2;
+  for (int e = threadIdx.x; e < N - P::R0; e += BS) {
+    const int q = e + P::R0;
+    int ns, R;
+    if (P::S >= 4 && q >= NS3) ns = NS3, R = P::R3;
+    else if (P::S >= 3 && q >= NS2) ns = NS2, R = P::R2;
+    else ns = NS1, R = P::R1;
+    const int o = q - ns, r = o / ns + 1, k = o - (r - 1) * ns;
+    const int m = k * r * (N / (ns * R));
+    s_tw[e] = g_tw[m];
+    if constexpr (sizeof(T) == 4) s_tw[N + e] = g_tw[N + m];
+  }
 }
 
 // Global accesses as base pointer + 32-bit per-thread byte offset + constant.
@@ -205,7 +259,8 @@ template <typename X> __device__ __forceinline__ void ste(void *f, uint32_t e, X
 // block, column-fastest lanes (plan with up to 16 points per thread).
 template <typename T, int N, int EMAX = 16, int MINCOLS = 8> struct ColCfg {
   static constexpr int EM = EMAX;
-  static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, LP = RP<N, EM>::LP, S = RP<N, EM>::S;
+  static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, S = RP<N, EM>::S;
+  static constexpr int LP = Lay<N, EM, false, (int)sizeof(cplx<T>)>::LP;
   static constexpr int COLS = 256 / TPL < MINCOLS ? MINCOLS : 256 / TPL;
   static constexpr int BS = COLS * TPL;
 };
@@ -214,7 +269,8 @@ template <typename T, int N, int EMAX = 16, int MINCOLS = 8> struct ColCfg {
 // per TPL consecutive lanes, 256 threads per block.
 template <typename T, int N, int EMAX = 16> struct LineCfg {
   static constexpr int EM = EMAX;
-  static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, LP = RP<N, EM>::LP, S = RP<N, EM>::S;
+  static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, S = RP<N, EM>::S;
+  static constexpr int LP = Lay<N, EM, true, (int)sizeof(cplx<T>)>::LP;
   static constexpr int BS = 256;
   static constexpr int LINES = BS / TPL;
 };
