@@ -219,12 +219,23 @@ struct MidProg {
   const void *sinx, *siny, *sinz;
 };
 
-// x-middle geometry: 8 points per thread (several spectra are live at once);
-// DFT_XMID_COLS columns per block.
+// x-middle geometry: DFT_XMID_EM points per thread (several spectra are live
+// at once), at least DFT_XMID_COLS columns per block.
+#ifndef DFT_XMID_EM
+#define DFT_XMID_EM 8
+#endif
 #ifndef DFT_XMID_COLS
 #define DFT_XMID_COLS 8
 #endif
-template <typename T, int N> using MidCfg = ColCfg<T, N, 8, DFT_XMID_COLS>;
+template <typename T, int N> using MidCfg = ColCfg<T, N, DFT_XMID_EM, DFT_XMID_COLS>;
+// minimum resident blocks per CU asked of the register allocator (0: none)
+#ifndef DFT_XMID_ORDER
+#define DFT_XMID_ORDER 0
+#endif
+#ifndef DFT_XMID_MINB
+#define DFT_XMID_MINB 0
+#endif
+template <typename T> constexpr int mid_minb() { return sizeof(T) == 4 ? DFT_XMID_MINB : 0; }
 
 // Blocks of the x-middle pass: ny * ntile tiles of COLS consecutive kz < P-1
 // at one ky, then ceil(ny / COLS) tiles of the Nyquist column kz = P-1 over
@@ -240,7 +251,7 @@ template <int COLS> __host__ __device__ inline int mid_blocks(int ny, int P) {
 // The program is read from device memory: indexed at run time, a by-value
 // kernel argument of this size would be copied to scratch by every thread.
 template <typename T, int N, int NK, bool DERIV>
-__global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__restrict__ pgp, int ny, int P, int PP,
+__global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(const MidProg *__restrict__ pgp, int ny, int P, int PP,
                                                            const cplx<T> *__restrict__ g_tw) {
   const MidProg &pg = *pgp;
   using C = cplx<T>;
@@ -248,15 +259,18 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__re
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
-  // consecutive tiles (one ky) on one XCD, back to back (shared L2 lines)
-  const int b = rf::xcd_block();
+  // DFT_XMID_ORDER 0: consecutive tiles (one ky) on one XCD, back to back
+  // (shared L2 lines); 1: ky fastest, so that the blocks in flight spread over
+  // whole x-planes
+  const int b = DFT_XMID_ORDER == 0 ? rf::xcd_block() : (int)blockIdx.x;
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
   const int ntile = mid_tiles<COLS>(P);
   int o, kz;
   bool act;
   if (b < ny * ntile) {
-    o = b / ntile;
-    const int c = (b - o * ntile) * COLS + col;
+    const int tile = DFT_XMID_ORDER == 0 ? b % ntile : b / ny;
+    o = DFT_XMID_ORDER == 0 ? b / ntile : b % ny;
+    const int c = tile * COLS + col;
     act = c < P - 1;
     kz = act ? c : P - 2;  // out-of-range lanes read an in-range column and store nothing
   } else {
@@ -347,6 +361,14 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS)) k_xmid(const MidProg *__re
 }
 
 // ------------------------------------------------------------------- zAB --
+// minimum resident blocks per CU asked of the register allocator for the FP32
+// zAB / zBC kernels (0: none)
+#ifndef DFT_ZAB_MINB
+#define DFT_ZAB_MINB 0
+#endif
+#ifndef DFT_ZBC_MINB
+#define DFT_ZBC_MINB 2
+#endif
 struct ZABArgs {
   Ptrs8 in;       // rhobar, [rhot], [A, Bx, By, Bz, C]
   Ptrs8 out;      // Q, [G_x, G_y, G_z]
@@ -355,7 +377,7 @@ struct ZABArgs {
 };
 
 template <typename T, int N, bool KC, bool BF>
-__global__ void __launch_bounds__(256) k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : 0) k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   constexpr int IA = KC ? 2 : 1;  // first backflow input
   const void *rho = a.rho;
@@ -490,7 +512,7 @@ struct ZBCArgs {
 };
 
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : 0) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   const void *rho = a.rho, *rhot = a.rhot;
   C v[E];
