@@ -35,7 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE, WRITE_SIZE) of this script, written by --traffic (the command is in the file)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ot_step_256_fp32.md")
 # device symbol (as the profiler names it) of each profiled launch label, FP32 256^3
-SYMBOL = {"ot_xmid_fwd_kernel_inv": "dft::fused::k_xmid<float, 256>",
+SYMBOL = {"ot_xconv_single": "dft::fused::k_xconv<float, 256, false, true>",
+          "ot_xconv_multi": "dft::fused::k_xconv<float, 256, false, false>",
+          "ot_xmid_fused": "dft::fused::k_xmid<float, 256, 1, true>",
+          "fft_y_c2c_fwd": "dft::fft::k_strided<float, 256, -1>",
+          "fft_y_c2c_inv": "dft::fft::k_strided<float, 256, 1>",
           "ot_zAB_c2r_local_kc_bf_r2c": "dft::fused::k_zAB<float, 256, true, true>",
           "ot_zBC_c2r_kc_terms_r2c": "dft::fused::k_zBC<float, 256, true>",
           "ot_zA_density_r2c": "dft::fused::k_zA<float, 256, true>",

@@ -111,7 +111,7 @@ void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, i
   DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_strided<T, N, DIR>), grid, dim3(G::BS), 0, s, b, cols, es, os, ntile, tw, scale);
   DFT_LAUNCH_CHECK();
-  DFT_PROF_END(s, axis == 1 ? "fft_y_c2c" : "fft_x_c2c",
+  DFT_PROF_END(s, axis == 1 ? (DIR < 0 ? "fft_y_c2c_fwd" : "fft_y_c2c_inv") : (DIR < 0 ? "fft_x_c2c_fwd" : "fft_x_c2c_inv"),
                2.0 * nb * (double)p.nx * p.ny * cols * sizeof(cplx<T>));
 }
 

@@ -360,6 +360,107 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
   }
 }
 
+// ----------------------------------------------------------------- xconv --
+// The x-middle program one output per block: out_m = F_x^-1 sum_t c_t F_x in_t
+// over the terms of output m.  An input shared by several outputs is read once
+// per output; the outputs of one column tile are consecutive logical blocks
+// on one XCD, so those re-reads mostly hit that XCD's L2.  Holding one line
+// (16 points per thread, 16 columns = 128-byte row segments per block) plus
+// an accumulator keeps the register count low enough for four waves per SIMD.
+template <typename T, int N> using XConvCfg = ColCfg<T, N, 16, 16>;
+
+template <typename T, int N, bool DERIV, bool ONE>
+__global__ void __launch_bounds__((XConvCfg<T, N>::BS)) k_xconv(const MidProg *__restrict__ pgp, int m0, int nl, int ny,
+                                                              int P, int PP, const cplx<T> *__restrict__ g_tw) {
+  const MidProg &pg = *pgp;
+  using C = cplx<T>;
+  using G = XConvCfg<T, N>;
+  constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
+  __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
+  __shared__ C s_tw[2 * N];
+  // outputs m0 .. m0 + nl - 1 of the program
+  const int L = rf::xcd_block();
+  const int b = L / nl, m = m0 + (L - b * nl);
+  const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
+  const int ntile = mid_tiles<COLS>(P);
+  int o, kz;
+  bool act;
+  if (b < ny * ntile) {
+    o = b / ntile;
+    const int c = (b - o * ntile) * COLS + col;
+    act = c < P - 1;
+    kz = act ? c : P - 2;  // out-of-range lanes read an in-range column and store nothing
+  } else {
+    const int r = (b - ny * ntile) * COLS + col;
+    act = r < ny;
+    o = act ? r : ny - 1;
+    kz = P - 1;
+  }
+  const uint32_t es = (uint32_t)ny * PP;
+  const uint32_t e0 = (uint32_t)o * PP + kz + t * es;  // element index of point t
+  const uint32_t obC = e0 * (uint32_t)sizeof(C), obR = e0 * (uint32_t)sizeof(T);
+  const size_t stC = (size_t)TPL * es * sizeof(C), stR = (size_t)TPL * es * sizeof(T);  // per point i
+  C *X = s_x + col * LP;
+  const MidOut &mo = pg.o[m];
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
+  if constexpr (ONE) {
+    // single-term outputs: transform, scale and transform back in place
+    const MidTerm tm = mo.t[0];
+    C v[E];
+    T k[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = rf::gld<C>((const char *)pg.in[tm.in] + i * stC, obC);
+#pragma unroll
+    for (int i = 0; i < E; ++i) k[i] = rf::gld<T>((const char *)pg.kern[tm.kern] + i * stR, obR);
+    __syncthreads();
+    rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, X, s_tw, t);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const T sc = (T)tm.scale * k[i];
+      v[i] = cmk<C>(sc * v[i].x, sc * v[i].y);
+    }
+    rf::rfft<T, N, G::EM, +1, rf::BlockSync>(v, X, s_tw, t);
+    if (act)
+#pragma unroll
+      for (int i = 0; i < E; ++i) rf::gst<C>((char *)mo.out + i * stC, obC, v[i]);
+    return;
+  }
+  C acc[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) acc[i] = cmk<C>(0, 0);
+  for (int q = 0; q < mo.nterms; ++q) {
+    const MidTerm tm = mo.t[q];
+    C v[E];
+    T k[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = rf::gld<C>((const char *)pg.in[tm.in] + i * stC, obC);
+    if (tm.kern >= 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) k[i] = rf::gld<T>((const char *)pg.kern[tm.kern] + i * stR, obR);
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) k[i] = (T)1;
+    }
+    if (q == 0) __syncthreads();  // twiddles
+    rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, X, s_tw, t);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      T sc = (T)tm.scale * k[i];
+      if (DERIV && tm.deriv >= 0) {
+        sc *= tm.deriv == 0 ? ((const T *)pg.sinx)[t + TPL * i]
+                            : (tm.deriv == 1 ? ((const T *)pg.siny)[o] : ((const T *)pg.sinz)[kz]);
+        acc[i] = cadd(acc[i], cmk<C>(-sc * v[i].y, sc * v[i].x));
+      } else {
+        acc[i] = cadd(acc[i], cmk<C>(sc * v[i].x, sc * v[i].y));
+      }
+    }
+  }
+  rf::rfft<T, N, G::EM, +1, rf::BlockSync>(acc, X, s_tw, t);
+  if (act)
+#pragma unroll
+    for (int i = 0; i < E; ++i) rf::gst<C>((char *)mo.out + i * stC, obC, acc[i]);
+}
+
 // ------------------------------------------------------------------- zAB --
 // minimum resident blocks per CU asked of the register allocator for the FP32
 // zAB / zBC kernels (0: none)
