@@ -82,6 +82,16 @@ struct WaveSync {
   }
 };
 
+// A zero the compiler cannot see through, produced where it is called.  Loads
+// from memory that the kernel has not yet written are treated as invariant
+// and would otherwise be hoisted to the kernel entry (all stages' loads live
+// at once); adding this to a stage's offsets keeps its loads in that stage.
+__device__ __forceinline__ uint32_t opaque_zero() {
+  uint32_t z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+
 // LDS layout of one line's exchange buffer: point p sits at element pad(p),
 // LP elements per line.  LDS serves 128 bytes per cycle over 32 four-byte
 // banks, i.e. 16 lanes of 8-byte (FP32 complex) or 8 lanes of 16-byte (FP64
@@ -177,8 +187,12 @@ __device__ __forceinline__ void rstage(Regs<T, N, EM> &v, cplx<T> *X, const cplx
 // (LayOf::LP elements); tw: twiddle table of load_tw.  Every thread of the sync
 // group must call it.
 template <typename T, int N, int EM, int DIR, class SYNC>
-__device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T> *tw, int t) {
+__device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T> *tw_, int t) {
   using P = RP<N, EM>;
+  // Twiddles are re-read from LDS by every transform: the table is never
+  // written after set-up, so the compiler would otherwise keep the values of
+  // the first transform live in registers through all later ones.
+  const cplx<T> *tw = tw_ + opaque_zero();
   if constexpr (P::S == 1) {
     rstage<T, N, EM, P::R0, 1, true, DIR, SYNC>(v, X, tw, t);
   } else if constexpr (P::S == 2) {
@@ -227,15 +241,6 @@ template <typename X> __device__ __forceinline__ X gld(const void *base, uint32_
 }
 template <typename X> __device__ __forceinline__ void gst(void *base, uint32_t off, X v, int imm = 0) {
   *(X *)((char *)base + off + imm) = v;
-}
-// A zero the compiler cannot see through, produced where it is called.  Loads
-// from memory that the kernel has not yet written are treated as invariant
-// and would otherwise be hoisted to the kernel entry (all stages' loads live
-// at once); adding this to a stage's offsets keeps its loads in that stage.
-__device__ __forceinline__ uint32_t opaque_zero() {
-  uint32_t z;
-  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-  return z;
 }
 
 // Logical block index with consecutive indices on one XCD: blocks are
