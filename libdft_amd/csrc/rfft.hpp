@@ -239,8 +239,20 @@ __device__ __forceinline__ void load_tw(cplx<T> *s_tw, const cplx<T> *g_tw) {
 template <typename X> __device__ __forceinline__ X gld(const void *base, uint32_t off, int imm = 0) {
   return *(const X *)((const char *)base + off + imm);
 }
+// Stores are marked non-temporal (DFT_NT_STORE): every pass writes fields
+// that only a later kernel reads, so they need not displace the data (stencil
+// neighbours, re-read inputs) that the writing kernel itself still reads.
+#ifndef DFT_NT_STORE
+#define DFT_NT_STORE 1
+#endif
 template <typename X> __device__ __forceinline__ void gst(void *base, uint32_t off, X v, int imm = 0) {
-  *(X *)((char *)base + off + imm) = v;
+  X *p = (X *)((char *)base + off + imm);
+  if constexpr (DFT_NT_STORE && (sizeof(X) == 4 || sizeof(X) == 8 || sizeof(X) == 16)) {
+    using V = uint32_t __attribute__((ext_vector_type(sizeof(X) / 4)));
+    __builtin_nontemporal_store(__builtin_bit_cast(V, v), (V *)p);
+  } else {
+    *p = v;
+  }
 }
 
 // Logical block index with consecutive indices on one XCD: blocks are
