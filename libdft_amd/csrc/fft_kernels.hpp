@@ -8,8 +8,10 @@ namespace fft {
 
 using rf::ColCfg;
 using rf::gld;
+using rf::gld_once;
 using rf::gst;
 using rf::lde;
+using rf::lde_once;
 using rf::ste;
 using rf::LineCfg;
 using rf::Regs;
@@ -68,11 +70,11 @@ __device__ __forceinline__ void load_merge(Regs<T, N, EM> &v, const void *in, ui
   for (int i = 0; i < E; ++i) {
     C a, b;
     if (i < E / 2) {
-      a = lde<C>(in, fa, TPL * i);
-      b = lde<C>(in, fb, TPL * i);
+      a = lde_once<C>(in, fa, TPL * i);
+      b = lde_once<C>(in, fb, TPL * i);
     } else {
-      a = lde<C>(in, ma, -TPL * i);
-      b = lde<C>(in, mb, -TPL * i);
+      a = lde_once<C>(in, ma, -TPL * i);
+      b = lde_once<C>(in, mb, -TPL * i);
       if (i > E / 2 || t > 0) {  // m > N/2: conjugate of the mirrored point
         a.y = -a.y;
         b.y = -b.y;
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, co
   const uint32_t e0 = p * 2u * N + (uint32_t)t;
   C v[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) v[i] = cmk<C>(lde<T>(in, e0, TPL * i), lde<T>(in, e0, N + TPL * i));
+  for (int i = 0; i < E; ++i) v[i] = cmk<C>(lde_once<T>(in, e0, TPL * i), lde_once<T>(in, e0, N + TPL * i));
   __syncthreads();
   rf::rfft<T, N, G::EM, -1, rf::WaveSync>(v, s_x + ln * LP, s_tw, t);
   split_store<T, N, G::EM, rf::WaveSync>(v, s_x + ln * LP, t, out, p, P, act);
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t e0 = l * (uint32_t)N + (uint32_t)t;
   C v[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) v[i] = lde<C>(src, e0, TPL * i);
+  for (int i = 0; i < E; ++i) v[i] = lde_once<C>(src, e0, TPL * i);
   if (DIR < 0 && pot) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {

@@ -239,6 +239,21 @@ __device__ __forceinline__ void load_tw(cplx<T> *s_tw, const cplx<T> *g_tw) {
 template <typename X> __device__ __forceinline__ X gld(const void *base, uint32_t off, int imm = 0) {
   return *(const X *)((const char *)base + off + imm);
 }
+// A load of data that this kernel reads exactly once (DFT_NT_LOAD), marked
+// non-temporal.  Used where it measured faster (the spectra the z passes
+// read); the column passes keep ordinary loads, which measured faster there.
+#ifndef DFT_NT_LOAD
+#define DFT_NT_LOAD 1
+#endif
+template <typename X> __device__ __forceinline__ X gld_once(const void *base, uint32_t off, int imm = 0) {
+  const X *p = (const X *)((const char *)base + off + imm);
+  if constexpr (DFT_NT_LOAD && (sizeof(X) == 4 || sizeof(X) == 8 || sizeof(X) == 16)) {
+    using V = uint32_t __attribute__((ext_vector_type(sizeof(X) / 4)));
+    return __builtin_bit_cast(X, __builtin_nontemporal_load((const V *)p));
+  } else {
+    return *p;
+  }
+}
 // Stores are marked non-temporal (DFT_NT_STORE): every pass writes fields
 // that only a later kernel reads, so they need not displace the data (stencil
 // neighbours, re-read inputs) that the writing kernel itself still reads.
@@ -267,6 +282,9 @@ __device__ __forceinline__ int xcd_block() {
 // Element-indexed forms: element `e` (per thread) plus constant element delta `d`.
 template <typename X> __device__ __forceinline__ X lde(const void *f, uint32_t e, int d = 0) {
   return gld<X>(f, e * (uint32_t)sizeof(X), d * (int)sizeof(X));
+}
+template <typename X> __device__ __forceinline__ X lde_once(const void *f, uint32_t e, int d = 0) {
+  return gld_once<X>(f, e * (uint32_t)sizeof(X), d * (int)sizeof(X));
 }
 template <typename X> __device__ __forceinline__ void ste(void *f, uint32_t e, X v, int d = 0) {
   gst<X>(f, e * (uint32_t)sizeof(X), v, d * (int)sizeof(X));
