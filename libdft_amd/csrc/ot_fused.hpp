@@ -71,6 +71,10 @@ template <typename T, int N, int TPL> struct PairPos {
   }
   // field value at point (w, i)
   template <typename X> __device__ X at(const void *f, int w, int i) const { return rf::lde<X>(f, a0, d(w, i)); }
+  // the same for a field this kernel reads only here
+  template <typename X> __device__ X at_once(const void *f, int w, int i) const {
+    return rf::lde_once<X>(f, a0, d(w, i));
+  }
   template <typename X> __device__ void put(void *f, int w, int i, X v) const { rf::ste<X>(f, a0, v, d(w, i)); }
   // neighbour of point (w, i) one step along axis D in direction S (+1 / -1)
   template <typename X, int D, int S> __device__ X nb(const void *f, int w, int i) const {
@@ -651,9 +655,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : 0) k_zBC(
       for (int w = 0; w < 2; ++w) {
         const T rst = qt.template at<T>(rhot, w, i) * (-pw.inv_rho0s) + (T)1;
         qt.template put<T>(a.vre, w, i,
-                          qt.template at<T>(a.vre, w, i) + (part(v[i], w) * rst * pw.ckc - t3[w][i] * (pw.ckc * pw.k23)));
+                          qt.template at_once<T>(a.vre, w, i) + (part(v[i], w) * rst * pw.ckc - t3[w][i] * (pw.ckc * pw.k23)));
         if constexpr (BF) {
-          T im = qt.template at<T>(a.vim, w, i);
+          T im = qt.template at_once<T>(a.vim, w, i);
           im += (T)0.5 * ((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h);
           im += (T)0.5 * ((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h);
           im += (T)0.5 * ((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h);
@@ -723,16 +727,16 @@ __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, con
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         const int d = w * N + TPL * i;
-        const T re = rf::lde<T>(a.vre, c0, d) + part(v[i], w);
-        const T im = a.vim ? rf::lde<T>(a.vim, c0, d) : (T)0;
-        C ve = cmk<C>(re - (T)a.mu + (a.vext ? rf::lde<T>(a.vext, c0, d) : (T)0), im);
+        const T re = rf::lde_once<T>(a.vre, c0, d) + part(v[i], w);
+        const T im = a.vim ? rf::lde_once<T>(a.vim, c0, d) : (T)0;
+        C ve = cmk<C>(re - (T)a.mu + (a.vext ? rf::lde_once<T>(a.vext, c0, d) : (T)0), im);
         if (a.vsave) vs[i] = ve;
         if (a.vavg) {
-          const C va = rf::lde<C>(a.vavg, c0, d);
+          const C va = rf::lde_once<C>(a.vavg, c0, d);
           ve = cmk<C>((va.x + ve.x) * (T)0.5, (va.y + ve.y) * (T)0.5);
         }
         ph[i] = ve;
-        u[i] = fft::apply_vphase<T>(rf::lde<C>(a.psi_src, c0, d), ve, a.dtr, a.dti);
+        u[i] = fft::apply_vphase<T>(rf::lde_once<C>(a.psi_src, c0, d), ve, a.dtr, a.dti);
       }
       if (act) {
 #pragma unroll
