@@ -118,30 +118,39 @@ template <int R, int DIR, typename C> __device__ inline void dft(C *v) {
   bfly_levels<R, DIR, 2>(v);
 }
 
-// m (cos p, sin p) with m = e^a, evaluated in FP64 and rounded once to T.
-// A phase factor is applied to the same mode or point every step; evaluated
-// in FP32, its modulus error is systematic (~3e-8 here) and the norm drifts by
-// that much per step, while a correctly rounded FP64 value only leaves the
-// unbiased error of the final rounding.
-template <typename T> __device__ inline cplx<T> unit_phase(double a, double p) {
-  double s, c;
-  sincos(p, &s, &c);
-  const double m = exp(a);
-  return cmk<cplx<T>>((T)(m * c), (T)(m * s));
+// e^x for |x| < 2^-10 (fifth-order series, relative error below 1e-21), exp
+// otherwise.
+__device__ inline double exp_small(double x) {
+  if (fabs(x) < 0x1p-10)
+    return 1.0 + x * (1.0 + 0.5 * x * (1.0 + x * (1.0 / 3.0) * (1.0 + 0.25 * x * (1.0 + 0.2 * x))));
+  return exp(x);
 }
 
-// psi e^{-i V dt/2} for complex V and complex dt.  The factor is evaluated in
-// FP64 and, in FP32, applied as its rounded value plus residual, so that its
-// modulus is exact: in a bulk region V is nearly uniform, every point would
-// see the same rounding of |e^{-iV dt/2}| twice per step, and the norm would
-// drift by ~1e-8 per step.
+// sin and cos in FP64 accuracy: Taylor polynomials for |x| < 2^-4 (next terms
+// below 1e-21), the library sincos otherwise.  The split-operator phases are
+// small angles, so the polynomial is the common path.
+__device__ inline void sincos_small(double x, double *s, double *c) {
+  if (fabs(x) < 0x1p-4) {
+    const double x2 = x * x;
+    *s = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0)))));
+    *c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0)))));
+  } else {
+    sincos(x, s, c);
+  }
+}
+
+// psi e^{-i V dt/2} for complex V and complex dt, e^{a + i p}, with the factor
+// evaluated in FP64.  In FP32 it is applied as its rounded value plus
+// residual, so that its modulus is exact: in a bulk region V is nearly
+// uniform, every point would see the same rounding of |e^{-iV dt/2}| twice
+// per step, and an FP32-rounded factor drifts the norm by ~1e-8 per step.
 template <typename T>
 __device__ inline cplx<T> apply_vphase(cplx<T> psi, cplx<T> v, double dtr, double dti) {
   const double a = 0.5 * ((double)v.x * dti + (double)v.y * dtr);
   const double p = -0.5 * ((double)v.x * dtr - (double)v.y * dti);
   double s, c;
-  sincos(p, &s, &c);
-  const double m = exp(a), pr = m * c, pi = m * s;
+  sincos_small(p, &s, &c);
+  const double m = exp_small(a), pr = m * c, pi = m * s;
   if constexpr (sizeof(T) == 4) {
     const float hr = (float)pr, hi = (float)pi;
     return cmul_hl(psi, make_float2(hr, hi), make_float2((float)(pr - hr), (float)(pi - hi)));
