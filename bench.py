@@ -207,6 +207,40 @@ def measured_traffic(label, n):
     return None
 
 
+# step traffic in units of one real field R (4N B in FP32, 8N B in FP64):
+# 134 R for the full functional, 34 R for plain OT (algorithmic schedule counts)
+STEP_R = {0: 34, 9: 134}
+
+
+def size_sweep(steps=10, warmup=3):
+    """Steps/s and the whole-step HBM fraction at each workload size: plain OT at
+    64^3 (FP64, h = 0.5 A, noisy bulk), full OT at 128^3 (droplet, h = 0.4 A) in
+    FP32 and FP64, and full OT at 512^3 (bulk with a 17 A cavity, h = 0.4 A) in
+    FP32; real-time single steps (dt = 2 fs) except imaginary time at 64^3."""
+    rho0 = 0.0218360 * L.AUTOANG ** 3
+    cases = [
+        ("plain OT 64^3 imaginary time", 64, 0.5, L.GRID_FP64, 0,
+         lambda n, h: inputs.noisy_bulk((n, n, n), rho0, 0.01), -1j * 10.0 / L.AUTOFS),
+        ("full OT 128^3 droplet", 128, 0.4, L.GRID_FP32, 9, lambda n, h: inputs.droplet((n, n, n), h), 2.0 / L.AUTOFS),
+        ("full OT 128^3 droplet", 128, 0.4, L.GRID_FP64, 9, lambda n, h: inputs.droplet((n, n, n), h), 2.0 / L.AUTOFS),
+        ("full OT 512^3 cavity", 512, 0.4, L.GRID_FP32, 9, lambda n, h: inputs.cavity((n, n, n), h, rho0),
+         2.0 / L.AUTOFS),
+    ]
+    out = []
+    for name, n, h_ang, prec, model, make, dt in cases:
+        step = h_ang / L.AUTOANG
+        log("size sweep: %s (%s)" % (name, "FP32" if prec == L.GRID_FP32 else "FP64"))
+        r = Run(n, step, prec, make(n, step), model)
+        r.steps(warmup, dt)
+        t = r.timed(steps, dt)
+        r.free()
+        R = (4.0 if prec == L.GRID_FP32 else 8.0) * n ** 3
+        gbs = STEP_R[model] * R / t / 1e9
+        out.append({"workload": name, "grid": n, "dtype": "f32" if prec == L.GRID_FP32 else "f64",
+                    "steps_per_s": 1.0 / t, "step_GBps": gbs, "step_frac": gbs / HBM_PEAK_GBS})
+    return out
+
+
 def accuracy(n, step, psi0, dt, k):
     """FP32 fields + FP64 reductions against the FP64 path: k steps from the same
     (FP32-representable) state; and one step against the FP64 oracle (kernels sampled
@@ -237,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--acc-steps", type=int, default=50,
                     help="steps of the FP32 vs FP64 energy comparison (0: skip)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the per-size steps/s table")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the HBM traffic table from two rocprofv3 --pmc counter_collection.csv files and exit")
     ap.add_argument("--kernel-stats", metavar="STATS_CSV",
@@ -276,6 +311,16 @@ def main():
     rows = profile_table()
     # the same steps through the reference driver's four calls (unfused potential accumulate + propagate)
     tpat = r32.timed(max(5, args.steps // 5), dt, pattern=True)
+    # predict-correct steps (two potential evaluations per step)
+    wp = L.Wave((n, n, n), step, L.GRID_FP32)
+    kpc = max(5, args.steps // 5)
+    lib.grid_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(kpc):
+        lib.dft_ot_propagate_pc(r32.f.h, r32.w.h, wp.h, None, r32.mu, L.Complex.of(dt))
+    lib.grid_synchronize()
+    tpc = (time.perf_counter() - t0) / kpc
+    wp.free()
     r32.free()
 
     out = {"metric": METRIC, "value": 1.0 / t32, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -299,6 +344,7 @@ def main():
                             / args.profile_steps}
     out["profiled_ms_per_step"] = 1e3 * tprof
     out["driver_pattern_ms_per_step"] = 1e3 * tpat
+    out["predict_correct_steps_per_s"] = 1.0 / tpc
     out["copy_calibration"] = calib
     out["kernels"] = [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows[:12]]
 
@@ -313,6 +359,9 @@ def main():
         if args.acc_steps > 0:
             log("accuracy: %d steps in FP32 and FP64" % args.acc_steps)
             out.update(accuracy(n, step, psi0, dt, args.acc_steps))
+
+    if not args.no_sweep:
+        out["sizes"] = size_sweep()
 
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, one step)")

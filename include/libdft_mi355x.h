@@ -301,6 +301,16 @@ void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, 
    arguments or lack of device memory. */
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad);
+/* Released cavity: writes psi_0 = sqrt(rho0 (1 + tanh((r - radius)/width)) / 2)
+   (r = |r_i|, r_i = (i - n/2) step) into gwf, propagates it nsteps steps of
+   dft_ot_propagate(otf, ., no V_ext, mu, time) and returns the bubble radius
+   R_b = (3 V_b / 4 pi)^(1/3), V_b = h^3 sum (1 - rho / rho0), of the final state
+   and its derivative with respect to the initial cavity radius, from the same
+   checkpointed adjoint sweep as dft_ot_adjoint_well.  PLAIN and KC models;
+   returns 0, or -1 (message on stderr) for backflow models, invalid arguments
+   or lack of device memory. */
+int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
+                          REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad);
 
 /* ---- measurement (extensions; no counterpart in the reference) ---- */
 /* Per-launch HIP-event timing of the library's kernels on its stream. */

@@ -122,6 +122,7 @@ _SIGS = {
     "dft_ot_propagate_pc": (None, ["otf", "wf", "wf", "rg", "d", "z"]),
     "dft_ot_potential_vjp": (None, ["otf", "rg", "rg", "wf"]),
     "dft_ot_adjoint_well": ("i", ["otf", "wf", "d", "d", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
+    "dft_ot_adjoint_cavity": ("i", ["otf", "wf", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
 }
 for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "chempot_pressurized",
            "compressibility", "sound_speed", "chempot3"):
@@ -279,6 +280,16 @@ class Functional:
         if rc != 0:
             raise RuntimeError("dft_ot_adjoint_well failed")
         return E.value, [g[0], g[1]]
+
+    def adjoint_cavity(self, rho0, radius, width, mu, dt, nsteps, every):
+        """dft_ot_adjoint_cavity: (R_b[psi_K], dR_b/dradius); writes psi_0(radius) into self.wave."""
+        J = C.c_double(0.0)
+        g = C.c_double(0.0)
+        rc = lib().dft_ot_adjoint_cavity(self.h, self.wave.h, rho0, radius, width, mu, Complex.of(dt), nsteps, every,
+                                         C.cast(C.byref(J), C.c_void_p), C.cast(C.byref(g), C.c_void_p))
+        if rc != 0:
+            raise RuntimeError("dft_ot_adjoint_cavity failed")
+        return J.value, g.value
 
     def free(self):
         if self.h:

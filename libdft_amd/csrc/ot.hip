@@ -1671,33 +1671,34 @@ void adj_step(dft_ot_functional *otf, const cplx<T> *psi, cplx<T> *psib, double 
 
 // Gradient of E[psi_K] with respect to (amp, width) of the well, for K single
 // steps of dft_ot_propagate from the state in gwf (left unchanged), by an
-// adjoint sweep: the forward pass keeps every `every`-th state, the reverse
-// pass recomputes each segment from its checkpoint.
-template <typename T>
-int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, double dtr, double dti, int64_t K,
-                      int64_t every, double *E, double grad[2]) {
+// Checkpointed adjoint of K single steps of dft_ot_propagate from the state in
+// gwf (left unchanged), with the well wl as V_ext.  The forward pass keeps every
+// `every`-th state; the reverse pass recomputes each segment from its
+// checkpoint.  seed(psi_K, psib, th) sets psib = dJ/dpsi_K (the convention
+// dJ = Re sum conj(psib) dpsi) and may start th; on return psib holds dJ/dpsi_0
+// and th[0..1] has accumulated dJ/d(amp, width) through V_ext.
+template <typename T, class Seed>
+int adjoint_sweep(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, double dtr, double dti, int64_t K,
+                  int64_t every, AdjWork<T> &w, cplx<T> *psib, double th[2], Seed seed) {
   dft_ot_device *d = otf->device;
   hipStream_t s = lib_stream();
   const int64_t n = d->n;
   const size_t bytes = sizeof(cplx<T>) * n;
   const int64_t ncp = (K + every - 1) / every;
-  AdjWork<T> w;
   std::vector<cplx<T> *> cp(ncp, nullptr), seg(every, nullptr);
-  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP);
+  bool ok = true;
   for (auto &x : cp) ok = ok && (x = (cplx<T> *)dmalloc(bytes)) != nullptr;
   for (auto &x : seg) ok = ok && (x = (cplx<T> *)dmalloc(bytes)) != nullptr;
   wf *cur = grid_wf_clone(gwf, (char *)"adjoint state");
   rgrid *vext = rgrid_clone(otf->density, (char *)"adjoint well");
   auto cleanup = [&]() {
     DFT_HIP_CHECK(hipStreamSynchronize(s));
-    w.release();
     for (auto x : cp) dfree(x);
     for (auto x : seg) dfree(x);
     if (cur) grid_wf_free(cur);
     if (vext) rgrid_free(vext);
   };
   if (!ok || !cur || !vext) {
-    fprintf(stderr, "libdft: dft_ot_adjoint_well: out of device memory\n");
     cleanup();
     return -1;
   }
@@ -1713,33 +1714,7 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
     if (it % every == 0) DFT_HIP_CHECK(hipMemcpyAsync(cp[it / every], cur->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
     dft_ot_propagate(otf, cur, vext, mu, dt);
   }
-  // J = E_kin + E_OT + E_ext and its gradient with respect to psi_K
-  Plan3D<T> &p = plan_of<T>(d);
-  const cplx<T> *psiK = (const cplx<T> *)cur->grid->dev;
-  const double h3 = d->step * d->step * d->step;
-  ew_density<T>(psiK, w.r[ARHO], n, s);
-  gradient_impl<T>(otf, w.r[AGRAD], w.r[ARHO]);
-  c2c_fwd<T>(p, psiK, w.c[0], s);
-  {
-    const double cx = std::pow(2.0 * M_PI / (d->nx * d->step), 2) / (2.0 * mass);
-    const double cy = std::pow(2.0 * M_PI / (d->ny * d->step), 2) / (2.0 * mass);
-    const double cz = std::pow(2.0 * M_PI / (d->nz * d->step), 2) / (2.0 * mass);
-    hipLaunchKernelGGL(k_kin_mul<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, w.c[0], d->nx, d->ny, d->nz, cx, cy, cz);
-    DFT_LAUNCH_CHECK();
-  }
-  c2c_inv<T>(p, w.c[0], w.c[1], s);
-  cplx<T> *psib = (cplx<T> *)dmalloc(bytes);
-  if (!psib) {
-    fprintf(stderr, "libdft: dft_ot_adjoint_well: out of device memory\n");
-    cleanup();
-    return -1;
-  }
-  hipLaunchKernelGGL(k_adj_seed<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, psib, (const cplx<T> *)w.c[1],
-                     psiK, (const T *)w.r[AGRAD], wl, h3, w.pa, w.pw, n);
-  DFT_LAUNCH_CHECK();
-  double th[2] = {h3 * sum_partials(w.pa, red_blocks(n), s), h3 * sum_partials(w.pw, red_blocks(n), s)};
-  const double eext = wl.amp * th[0];
-  *E = grid_wf_energy(cur, nullptr) + dft_ot_energy(otf, cur) + eext;
+  seed(cur, psib, th);
   // reverse sweep, segment by segment
   for (int64_t j = ncp - 1; j >= 0; --j) {
     const int64_t n0 = j * every, n1 = std::min(K, n0 + every);
@@ -1751,11 +1726,134 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
     for (int64_t it = n1 - 1; it >= n0; --it)
       adj_step<T>(otf, seg[it - n0], psib, mass, wl, mu, dtr, dti, w, th);
   }
-  grad[0] = th[0];
-  grad[1] = th[1];
-  dfree(psib);
   cleanup();
   return 0;
+}
+
+// J = E_kin + E_OT + E_ext of psi_K and dJ/d(amp, width) of the well.
+template <typename T>
+int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, double dtr, double dti, int64_t K,
+                      int64_t every, double *E, double grad[2]) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  const int64_t n = d->n;
+  AdjWork<T> w;
+  cplx<T> *psib = nullptr;
+  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP) && (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
+  double th[2] = {0.0, 0.0};
+  auto seed = [&](wf *cur, cplx<T> *pb, double t2[2]) {
+    // the gradient of the total energy with respect to psi_K
+    Plan3D<T> &p = plan_of<T>(d);
+    const cplx<T> *psiK = (const cplx<T> *)cur->grid->dev;
+    const double h3 = d->step * d->step * d->step;
+    ew_density<T>(psiK, w.r[ARHO], n, s);
+    gradient_impl<T>(otf, w.r[AGRAD], w.r[ARHO]);
+    c2c_fwd<T>(p, psiK, w.c[0], s);
+    const double cx = std::pow(2.0 * M_PI / (d->nx * d->step), 2) / (2.0 * gwf->mass);
+    const double cy = std::pow(2.0 * M_PI / (d->ny * d->step), 2) / (2.0 * gwf->mass);
+    const double cz = std::pow(2.0 * M_PI / (d->nz * d->step), 2) / (2.0 * gwf->mass);
+    hipLaunchKernelGGL(k_kin_mul<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, w.c[0], d->nx, d->ny, d->nz, cx, cy, cz);
+    DFT_LAUNCH_CHECK();
+    c2c_inv<T>(p, w.c[0], w.c[1], s);
+    hipLaunchKernelGGL(k_adj_seed<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, pb, (const cplx<T> *)w.c[1],
+                       psiK, (const T *)w.r[AGRAD], wl, h3, w.pa, w.pw, n);
+    DFT_LAUNCH_CHECK();
+    t2[0] = h3 * sum_partials(w.pa, red_blocks(n), s);
+    t2[1] = h3 * sum_partials(w.pw, red_blocks(n), s);
+    *E = grid_wf_energy(cur, nullptr) + dft_ot_energy(otf, cur) + wl.amp * t2[0];
+  };
+  const int rc = ok ? adjoint_sweep<T>(otf, gwf, wl, mu, dtr, dti, K, every, w, psib, th, seed) : -1;
+  if (rc) fprintf(stderr, "libdft: dft_ot_adjoint_well: out of device memory\n");
+  grad[0] = th[0];
+  grad[1] = th[1];
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+  dfree(psib);
+  w.release();
+  return rc;
+}
+
+// psi_0 = sqrt(rho0 (1 + tanh((r - R0)/w)) / 2): bulk with a spherical cavity.
+template <typename T> __global__ void k_cavity(cplx<T> *psi, Well g, double rho0, double R0, double wd, int64_t n) {
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((int64_t)g.nz * g.ny));
+    const double x = (i - g.nx / 2) * g.h, y = (j - g.ny / 2) * g.h, z = (k - g.nz / 2) * g.h;
+    const double r = sqrt(x * x + y * y + z * z);
+    psi[c] = cmk<cplx<T>>((T)sqrt(0.5 * rho0 * (1.0 + tanh((r - R0) / wd))), (T)0);
+  }
+}
+
+// Per block: sum (1 - |psi|^2 / rho0) (the missing volume over h^3).
+template <typename T> __global__ void k_cavity_volume(const cplx<T> *psi, double rho0, double *pa, double *pb, int64_t n) {
+  double sa = 0.0;
+  GRID_STRIDE(c, n) {
+    const double pr = psi[c].x, pi = psi[c].y;
+    sa += 1.0 - (pr * pr + pi * pi) / rho0;
+  }
+  block_sum2(sa, 0.0, pa, pb);
+}
+
+// psib = f psi (f real).
+template <typename T> __global__ void k_scale_into(cplx<T> *psib, const cplx<T> *psi, double f, int64_t n) {
+  GRID_STRIDE(c, n) psib[c] = cmk<cplx<T>>((T)(f * (double)psi[c].x), (T)(f * (double)psi[c].y));
+}
+
+// Per block: sum Re(conj(psib) dpsi_0/dR0) with dpsi_0/dR0 = -psi_0 (1 - tanh s) / (2w), s = (r - R0)/w.
+template <typename T>
+__global__ void k_cavity_grad(const cplx<T> *psib, Well g, double rho0, double R0, double wd, double *pa, double *pb,
+                              int64_t n) {
+  double sa = 0.0;
+  GRID_STRIDE(c, n) {
+    const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((int64_t)g.nz * g.ny));
+    const double x = (i - g.nx / 2) * g.h, y = (j - g.ny / 2) * g.h, z = (k - g.nz / 2) * g.h;
+    const double th = tanh((sqrt(x * x + y * y + z * z) - R0) / wd);
+    const double p0 = sqrt(0.5 * rho0 * (1.0 + th));
+    sa += (double)psib[c].x * (-p0 * (1.0 - th) / (2.0 * wd));
+  }
+  block_sum2(sa, 0.0, pa, pb);
+}
+
+// Released cavity: psi_0(R0) written to gwf, K steps without V_ext, J = the
+// bubble radius (3 V_b / 4 pi)^(1/3), V_b = h^3 sum (1 - rho / rho0), and dJ/dR0.
+template <typename T>
+int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0, double wd, double mu, double dtr,
+                        double dti, int64_t K, int64_t every, double *J, double *grad) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  const int64_t n = d->n;
+  const double h3 = d->step * d->step * d->step;
+  const Well geo{0.0, 1.0, 0.0, 0.0, 0.0, d->step, d->nx, d->ny, d->nz};
+  hipLaunchKernelGGL(k_cavity<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, (cplx<T> *)gwf->grid->dev, geo, rho0, R0, wd,
+                     n);
+  DFT_LAUNCH_CHECK();
+  AdjWork<T> w;
+  cplx<T> *psib = nullptr;
+  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP) && (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
+  double th[2] = {0.0, 0.0};
+  auto seed = [&](wf *cur, cplx<T> *pb, double *) {
+    const cplx<T> *psiK = (const cplx<T> *)cur->grid->dev;
+    hipLaunchKernelGGL(k_cavity_volume<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, psiK, rho0, w.pa, w.pw,
+                       n);
+    DFT_LAUNCH_CHECK();
+    const double V = h3 * sum_partials(w.pa, red_blocks(n), s);
+    *J = std::cbrt(3.0 * V / (4.0 * M_PI));
+    // dJ/dpsi = (J / 3V) dV/dpsi, dV/dpsi = -2 h^3 psi / rho0
+    hipLaunchKernelGGL(k_scale_into<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, pb, psiK,
+                       (*J / (3.0 * V)) * (-2.0 * h3 / rho0), n);
+    DFT_LAUNCH_CHECK();
+  };
+  int rc = ok ? adjoint_sweep<T>(otf, gwf, geo, mu, dtr, dti, K, every, w, psib, th, seed) : -1;
+  if (rc) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_cavity: out of device memory\n");
+  } else {
+    hipLaunchKernelGGL(k_cavity_grad<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, (const cplx<T> *)psib, geo,
+                       rho0, R0, wd, w.pa, w.pw, n);
+    DFT_LAUNCH_CHECK();
+    *grad = sum_partials(w.pa, red_blocks(n), s);
+  }
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
+  dfree(psib);
+  w.release();
+  return rc;
 }
 
 // Standalone reverse pass of V_OT at the density of gwf (for tests): rho_bar = (dV/drho)^T v_bar.
@@ -2064,6 +2162,26 @@ int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, R
   *energy = E;
   grad[0] = g[0];
   grad[1] = g[1];
+  return rc;
+}
+
+int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
+                          REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
+  check_wf(otf, gwf, "dft_ot_adjoint_cavity");
+  if (otf->model & DFT_OT_BACKFLOW) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_cavity: the adjoint covers the PLAIN and KC models (no backflow)\n");
+    return -1;
+  }
+  if (nsteps < 1 || every < 1 || width <= 0.0 || rho0 <= 0.0 || !bubble_radius || !grad) {
+    fprintf(stderr, "libdft: dft_ot_adjoint_cavity: invalid arguments\n");
+    return -1;
+  }
+  int rc = 0;
+  double J = 0.0, g = 0.0;
+  DFT_BY_PREC(otf->device->prec, rc = adjoint_cavity_impl<T>(otf, gwf, rho0, radius, width, mu, __real__ time,
+                                                             __imag__ time, nsteps, every, &J, &g));
+  *bubble_radius = J;
+  *grad = g;
   return rc;
 }
 

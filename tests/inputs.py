@@ -68,3 +68,12 @@ def smooth_modulated(shape, step, rho0, amp=0.3, seed=SEED):
     phase = (np.sin(2 * np.pi * x / L[0] + 3 * u[3]) + 0.7 * np.cos(2 * np.pi * z / L[2] + 3 * u[4])
              + 0.4 * np.sin(2 * np.pi * (y / L[1] + z / L[2]) + 3 * u[5]))
     return np.sqrt(rho) * np.exp(1j * phase)
+
+
+def cavity(shape, step, rho0, radius_ang=17.0, width_ang=0.5):
+    """Bulk rho0 with a spherical cavity: rho = rho0 (1 + tanh((r - R)/w)) / 2,
+    zero phase (the state a 1000 K (1 - tanh((r - 17 A)/0.5 A))/2 barrier holds)."""
+    x, y, z = np.meshgrid(*coords(shape, step), indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z)
+    rho = 0.5 * rho0 * (1.0 + np.tanh((r - radius_ang / AUTOANG) / (width_ang / AUTOANG)))
+    return np.sqrt(rho).astype(np.complex128)

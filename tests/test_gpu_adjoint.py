@@ -1,8 +1,10 @@
 """Adjoint (reverse-mode) checks on the GPU: the transposed Jacobian of
 V_OT(rho) against central finite differences of the potential (dot-product
-test), and the adjoint split-operator sweep against central finite
-differences of the final energy with respect to the gaussian well's depth and
-width (config 4(ii): 128^3 droplet with the -50 K / 2 A impurity well).
+test), the adjoint split-operator sweep against central finite differences
+of the final energy with respect to the gaussian well's depth and width
+(config 4(ii): 128^3 droplet with the -50 K / 2 A impurity well), and of the
+bubble radius of a released cavity with respect to its initial radius
+(config 5: 512^3, 17 A cavity).
 """
 import numpy as np
 import pytest
@@ -110,6 +112,42 @@ def test_adjoint_config4_droplet128():
     psi0 = inputs.droplet(shape, step)
     amp, width = -50.0 / L.AUTOK, 2.0 / L.AUTOANG
     _fd_check(1, shape, step, psi0, amp, width, 0.0, 5.0 / L.AUTOFS, 500, 50, 1e-4)
+
+
+def _cavity_fd(model, shape, step, prec, rho0, R0, w, dt, nsteps, every, tol):
+    def run(radius):
+        wv = L.Wave(shape, step, prec)
+        f = L.Functional(model, wv)
+        mu = f.bulk("dEdRho", rho0)
+        J, g = f.adjoint_cavity(rho0, radius, w, mu, dt, nsteps, every)
+        f.free(), wv.free()
+        return J, g
+    J, g = run(R0)
+    dR = 1e-4 * R0
+    fd = (run(R0 + dR)[0] - run(R0 - dR)[0]) / (2 * dR)
+    _log("cavity model %d %s K=%d every=%d: R_b %.10f (R0 %.6f); dR_b/dR0 adjoint %.10e FD %.10e (rel %.2e)"
+         % (model, shape, nsteps, every, J, R0, g, fd, abs(g / fd - 1)))
+    assert abs(g / fd - 1) < tol
+    return J, g
+
+
+@pytest.mark.parametrize("model", [0, 1], ids=["plain", "kc"])
+def test_adjoint_cavity_small(model):
+    """Released cavity at 64^3 (h = 0.8 A, R0 = 10 A, 1 A wall), 12 real-time steps of 5 fs,
+    checkpoints every 5: bubble-radius gradient against central FD of R_b(R0)."""
+    rho0 = 0.0218360 * L.AUTOANG ** 3
+    _cavity_fd(model, (64, 64, 64), 0.8 / L.AUTOANG, L.GRID_FP64, rho0, 10.0 / L.AUTOANG, 1.0 / L.AUTOANG,
+               5.0 / L.AUTOFS, 12, 5, 1e-6)
+
+
+def test_adjoint_cavity_config5_512():
+    """Config 5 at its size (512^3, h = 0.4 A, 17 A cavity with a 0.5 A wall, released;
+    KC functional: the adjoint covers PLAIN and KC), 40 steps of 2 fs with checkpoints
+    every 10, FP64: dR_b/dR0 against central FD, <= 1e-4."""
+    rho0 = 0.0218360 * L.AUTOANG ** 3
+    J, g = _cavity_fd(1, (512, 512, 512), 0.4 / L.AUTOANG, L.GRID_FP64, rho0, 17.0 / L.AUTOANG, 0.5 / L.AUTOANG,
+                      2.0 / L.AUTOFS, 40, 10, 1e-4)
+    assert abs(J * L.AUTOANG - 17.0) < 1.0  # the bubble has barely moved in 80 fs
 
 
 def test_adjoint_rejects_backflow():
