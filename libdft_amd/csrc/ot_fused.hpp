@@ -112,9 +112,12 @@ __device__ __forceinline__ T grad_d(const void *f, const Q &q, int w, int i, con
   return (q.template nb<T, D, 1>(f, w, i) - q.template nb<T, D, -1>(f, w, i)) * pw.inv2h;
 }
 
-// Pair kernels use the plan with the fewest points per thread: 4 (8 for N = 512,
-// where a line must still fit one wavefront).
-template <int N> constexpr int pair_em() { return N > 256 ? 8 : 4; }
+// Pair kernels use the plan with the fewest points per thread, 4: a line is
+// one wavefront up to N = 256 and two at N = 512 (exchanges then take the
+// block barrier).
+template <int N> constexpr int pair_em() { return 4; }
+template <int N>
+using PairSync = std::conditional_t<(rf::RP<N, pair_em<N>()>::TPL <= 64), rf::WaveSync, rf::LineBlockSync>;
 
 // Common prologue of the pair kernels.
 #define FUSED_PAIR_SETUP()                                                   \
@@ -137,8 +140,8 @@ template <int N> constexpr int pair_em() { return N > 256 ? 8 : 4; }
 template <typename T, int N>
 __device__ __forceinline__ void r2c_store(rf::Regs<T, N, pair_em<N>()> &v, cplx<T> *X, const cplx<T> *tw, int t,
                                           void *out, uint32_t p, int P, bool act) {
-  rf::rfft<T, N, pair_em<N>(), -1, rf::WaveSync>(v, X, tw, t);
-  fft::split_store<T, N, pair_em<N>(), rf::WaveSync>(v, X, t, out, p, P, act);
+  rf::rfft<T, N, pair_em<N>(), -1, PairSync<N>>(v, X, tw, t);
+  fft::split_store<T, N, pair_em<N>(), PairSync<N>>(v, X, t, out, p, P, act);
 }
 
 // Load + c2r of both lines of the pair from half-spectrum field `in`.
@@ -146,7 +149,7 @@ template <typename T, int N>
 __device__ __forceinline__ void load_c2r(rf::Regs<T, N, pair_em<N>()> &v, cplx<T> *X, const cplx<T> *tw, int t,
                                          const void *in, uint32_t p, int P) {
   fft::load_merge<T, N, pair_em<N>()>(v, in, p, P, t);
-  rf::rfft<T, N, pair_em<N>(), +1, rf::WaveSync>(v, X, tw, t);
+  rf::rfft<T, N, pair_em<N>(), +1, PairSync<N>>(v, X, tw, t);
 }
 
 template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { return w ? c.y : c.x; }
@@ -745,7 +748,7 @@ __global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, con
           if (a.vsave) rf::ste<C>(a.vsave, c0, vs[i], w * N + TPL * i);
         }
       }
-      rf::rfft<T, N, pair_em<N>(), -1, rf::WaveSync>(u, X, s_tw, t);
+      rf::rfft<T, N, pair_em<N>(), -1, PairSync<N>>(u, X, s_tw, t);
       if (act)
 #pragma unroll
         for (int i = 0; i < E; ++i) rf::ste<C>(a.psi_dst, c0, u[i], w * N + TPL * i);

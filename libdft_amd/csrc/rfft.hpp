@@ -51,24 +51,32 @@ template <> struct Radix<32, 4> { static constexpr int S = 3, R0 = 4, R1 = 4, R2
 template <> struct Radix<64, 4> { static constexpr int S = 3, R0 = 4, R1 = 4, R2 = 4, R3 = 1; };
 template <> struct Radix<128, 4> { static constexpr int S = 4, R0 = 4, R1 = 4, R2 = 4, R3 = 2; };
 template <> struct Radix<256, 4> { static constexpr int S = 4, R0 = 4, R1 = 4, R2 = 4, R3 = 4; };
+// N = 512 with 4 points per thread: 128 threads (two wavefronts) per line.
+template <> struct Radix<512, 4> { static constexpr int S = 5, R0 = 4, R1 = 4, R2 = 4, R3 = 4, R4 = 2; };
 
 template <typename P, typename = void> struct R3of { static constexpr int v = 1; };
 template <typename P> struct R3of<P, decltype((void)P::R3)> { static constexpr int v = P::R3; };
+template <typename P, typename = void> struct R4of { static constexpr int v = 1; };
+template <typename P> struct R4of<P, decltype((void)P::R4)> { static constexpr int v = P::R4; };
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <int N, int EMAX = 16> struct RP {
   using P = Radix<N, EMAX>;
-  static constexpr int S = P::S, R0 = P::R0, R1 = P::R1, R2 = P::R2, R3 = R3of<P>::v;
-  static constexpr int E = cmax(cmax(R0, R1), cmax(R2, R3));
+  static constexpr int S = P::S, R0 = P::R0, R1 = P::R1, R2 = P::R2, R3 = R3of<P>::v, R4 = R4of<P>::v;
+  static constexpr int E = cmax(cmax(cmax(R0, R1), cmax(R2, R3)), R4);
   static constexpr int TPL = N / E;
-  static_assert(E % R0 == 0 && E % R1 == 0 && E % R2 == 0 && E % R3 == 0, "radices must divide E");
-  static_assert(TPL <= 64, "a line must fit one wavefront");
+  static_assert(E % R0 == 0 && E % R1 == 0 && E % R2 == 0 && E % R3 == 0 && E % R4 == 0, "radices must divide E");
 };
 
 // The registers of one thread's share of a line.
 template <typename T, int N, int EM> using Regs = cplx<T>[RP<N, EM>::E];
 
 struct BlockSync {
+  __device__ __forceinline__ static void sync() { __syncthreads(); }
+};
+// Lines whose threads are consecutive lanes (as under WaveSync) but span
+// several wavefronts: the block barrier orders the exchange.
+struct LineBlockSync {
   __device__ __forceinline__ static void sync() { __syncthreads(); }
 };
 // All threads of a line are lanes of one wavefront: LDS operations of a wave
@@ -121,7 +129,9 @@ template <int N, int EM, bool WAVE, int W> struct Lay {
     else return p + (p >> 4);
   }
 };
-template <class SYNC> constexpr bool kWave = std::is_same<SYNC, WaveSync>::value;
+// line-major lanes (the layout choice), whatever the barrier
+template <class SYNC>
+constexpr bool kWave = std::is_same<SYNC, WaveSync>::value || std::is_same<SYNC, LineBlockSync>::value;
 template <typename T, int N, int EM, class SYNC> using LayOf = Lay<N, EM, kWave<SYNC>, (int)sizeof(cplx<T>)>;
 
 // Inter-stage twiddles in LDS, stage by stage: stage s >= 1 (radix R_s, NS_s =
@@ -193,6 +203,7 @@ __device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T
   // written after set-up, so the compiler would otherwise keep the values of
   // the first transform live in registers through all later ones.
   const cplx<T> *tw = tw_ + opaque_zero();
+  static_assert(!std::is_same<SYNC, WaveSync>::value || P::TPL <= 64, "a wave-synchronised line must fit one wavefront");
   if constexpr (P::S == 1) {
     rstage<T, N, EM, P::R0, 1, true, DIR, SYNC>(v, X, tw, t);
   } else if constexpr (P::S == 2) {
@@ -202,11 +213,18 @@ __device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T
     rstage<T, N, EM, P::R0, 1, false, DIR, SYNC>(v, X, tw, t);
     rstage<T, N, EM, P::R1, P::R0, false, DIR, SYNC>(v, X, tw, t);
     rstage<T, N, EM, P::R2, P::R0 * P::R1, true, DIR, SYNC>(v, X, tw, t);
-  } else {
+  } else if constexpr (P::S == 4) {
     rstage<T, N, EM, P::R0, 1, false, DIR, SYNC>(v, X, tw, t);
     rstage<T, N, EM, P::R1, P::R0, false, DIR, SYNC>(v, X, tw, t);
     rstage<T, N, EM, P::R2, P::R0 * P::R1, false, DIR, SYNC>(v, X, tw, t);
     rstage<T, N, EM, P::R3, P::R0 * P::R1 * P::R2, true, DIR, SYNC>(v, X, tw, t);
+  } else {
+    static_assert(P::S == 5, "at most five stages");
+    rstage<T, N, EM, P::R0, 1, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R1, P::R0, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R2, P::R0 * P::R1, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R3, P::R0 * P::R1 * P::R2, false, DIR, SYNC>(v, X, tw, t);
+    rstage<T, N, EM, P::R4, P::R0 * P::R1 * P::R2 * P::R3, true, DIR, SYNC>(v, X, tw, t);
   }
 }
 
@@ -216,11 +234,12 @@ __device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T
 template <typename T, int N, int EM, int BS>
 __device__ __forceinline__ void load_tw(cplx<T> *s_tw, const cplx<T> *g_tw) {
   using P = RP<N, EM>;
-  constexpr int NS1 = P::R0, NS2 = NS1 * P::R1, NS3 = NS2 * P::R2;
+  constexpr int NS1 = P::R0, NS2 = NS1 * P::R1, NS3 = NS2 * P::R2, NS4 = NS3 * P::R3;
   for (int e = threadIdx.x; e < N - P::R0; e += BS) {
     const int q = e + P::R0;
     int ns, R;
-    if (P::S >= 4 && q >= NS3) ns = NS3, R = P::R3;
+    if (P::S >= 5 && q >= NS4) ns = NS4, R = P::R4;
+    else if (P::S >= 4 && q >= NS3) ns = NS3, R = P::R3;
     else if (P::S >= 3 && q >= NS2) ns = NS2, R = P::R2;
     else ns = NS1, R = P::R1;
     const int o = q - ns, r = o / ns + 1, k = o - (r - 1) * ns;

@@ -309,6 +309,29 @@ def test_fused_single_step_matches_oracle(oracle, prec, model):
 
 
 @pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+@pytest.mark.parametrize("shape", [(8, 8, 512), (8, 512, 8), (512, 8, 8)], ids=["z512", "y512", "x512"])
+def test_fused_single_step_512_lines(oracle, prec, shape):
+    """Full OT single step with one axis of 512 points: the z-pass line of 512
+    points spans two wavefronts (five radix stages), the column passes take the
+    512-point plans."""
+    step = 0.4 / L.AUTOANG
+    o = oracle.OracleOT(9, shape, step, 2, 8)
+    psi0 = inputs.smooth_modulated(shape, step, o.p.rho0, 0.2)
+    mu0 = o.bulk("dEdRho", o.p.rho0)
+    lib = L.lib()
+    w = L.Wave(shape, step, prec)
+    f = L.Functional(9, w, 2, 8)
+    dt = 2.0 / L.AUTOFS
+    want = psi0.copy()
+    o.single_step(want, None, mu0, dt)
+    w.set(psi0)
+    lib.dft_ot_propagate(f.h, w.h, None, mu0, L.Complex.of(dt))
+    assert relerr(w.get(), want) < (1e-10 if prec == L.GRID_FP64 else 2e-5)
+    assert relerr(f.density(), np.abs(psi0) ** 2) < TOL[prec]
+    f.free(), w.free()
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
 def test_fused_pc_step_matches_oracle(oracle, prec):
     shape, step = (32, 16, 32), 0.5 / L.AUTOANG
     o = oracle.OracleOT(9, shape, step, 2, 8)
