@@ -256,6 +256,11 @@ REAL dft_ot_bulk_chempot_pressurized(dft_ot_functional *otf, REAL pressure);
 REAL dft_ot_bulk_compressibility(dft_ot_functional *otf, REAL rho);
 REAL dft_ot_bulk_sound_speed(dft_ot_functional *otf, REAL rho);
 REAL dft_ot_bulk_dispersion(dft_ot_functional *otf, REAL *k, REAL rho0);
+/* Dispersion relation from the dynamics of a density wave (src/helium-ot-bulk.c:274):
+   energy (hartree) of the wave k (rounded to the grid, returned in *k) along
+   axis dir; predict-correct steps when pred.  0 for k = 0; -1 on invalid
+   input or when no half period is found. */
+REAL dft_ot_dispersion(wf *gwf, dft_ot_functional *otf, REAL ts, REAL *k, REAL amp, char pred, char dir);
 REAL dft_ot_bulk_istatic(dft_ot_functional *otf, REAL *k, REAL rho0);
 
 /* Kernel functions, src/common.c:31-66, 131-138, 173-179, 193-200; src/ot-private.h:11-22. */
@@ -327,6 +332,8 @@ void grid_bench_copy(rgrid *dst, rgrid *src);
 #define GRID_AUTOANG 0.529177210903
 #define GRID_AUTOK 315775.02480407
 #define GRID_AUTOFS 0.024188843265857
+#define GRID_AUTOS 2.4188843265857E-17
+#define GRID_HZTOCM1 3.3356409519815E-11
 #define GRID_AUTOAMU (1.0 / 1822.888486209)
 #define GRID_AUTOMPS 2187691.26364
 #define GRID_AUTOBAR 2.9421015697E8

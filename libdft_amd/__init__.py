@@ -123,6 +123,7 @@ _SIGS = {
     "dft_ot_potential_vjp": (None, ["otf", "rg", "rg", "wf"]),
     "dft_ot_adjoint_well": ("i", ["otf", "wf", "d", "d", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_adjoint_cavity": ("i", ["otf", "wf", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
+    "dft_ot_dispersion": ("d", ["wf", "otf", "d", "p", "d", "b", "b"]),
 }
 for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "chempot_pressurized",
            "compressibility", "sound_speed", "chempot3"):
@@ -130,7 +131,7 @@ for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "che
 for _n in ("density", "chempot", "chempot2"):
     _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf"])
 
-_T = {"i": C.c_int, "i64": C.c_int64, "d": C.c_double, "p": C.c_void_p, "s": C.c_char_p, "c": C.c_char,
+_T = {"i": C.c_int, "i64": C.c_int64, "d": C.c_double, "p": C.c_void_p, "s": C.c_char_p, "c": C.c_char, "b": C.c_byte,
       "z": Complex, "rg": _RG, "cg": C.POINTER(CGrid), "wf": C.POINTER(WF), "otf": C.POINTER(OTF)}
 
 
@@ -290,6 +291,15 @@ class Functional:
         if rc != 0:
             raise RuntimeError("dft_ot_adjoint_cavity failed")
         return J.value, g.value
+
+    def dispersion(self, dt, k, amp, pred=False, direction=2):
+        """dft_ot_dispersion on self.wave (overwritten): (energy in hartree, k rounded to the grid)."""
+        kk = C.c_double(k)
+        e = lib().dft_ot_dispersion(self.wave.h, self.h, dt, C.cast(C.byref(kk), C.c_void_p), amp, 1 if pred else 0,
+                                    direction)
+        if e < 0:
+            raise RuntimeError("dft_ot_dispersion failed")
+        return e, kk.value
 
     def free(self):
         if self.h:
