@@ -25,6 +25,7 @@
 #define LIBDFT_MI355X_H
 
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -127,6 +128,24 @@ void rgrid_copy_from_host(rgrid *grid, const REAL *src);
 void rgrid_copy_to_host(const rgrid *grid, REAL *dst);
 void cgrid_copy_from_host(cgrid *grid, const REAL_COMPLEX *src);
 void cgrid_copy_to_host(const cgrid *grid, REAL_COMPLEX *dst);
+
+/* ------------------------------------------------------------ grid files -- */
+/* Checkpoint files (replace libgrid's rgrid_/cgrid_write_grid, _read_grid,
+   _write, _read; call sites e.g. examples/gas-bubble/bubble.c:141-143,
+   examples/vortex-trace/trace.c:155,177).  The file layout is this library's
+   own: "LDFTGRD1", kind, nx, ny, nz (int64), step, x0, y0, z0 (double), then
+   FP64 values row-major (complex as re, im).  *_write_grid writes <base>.grd;
+   *_read_grid reads the named file into a grid of the same shape; *_read
+   allocates the grid when given NULL.  Errors print a libdft: message (the
+   readers then return NULL / leave the grid unchanged). */
+void rgrid_write(rgrid *grid, FILE *fp);
+rgrid *rgrid_read(rgrid *grid, FILE *fp);
+void cgrid_write(cgrid *grid, FILE *fp);
+cgrid *cgrid_read(cgrid *grid, FILE *fp);
+void rgrid_write_grid(char *base, rgrid *grid);
+void cgrid_write_grid(char *base, cgrid *grid);
+void rgrid_read_grid(rgrid *grid, char *file);
+void cgrid_read_grid(cgrid *grid, char *file);
 REAL rgrid_value_at_index(rgrid *grid, INT i, INT j, INT k);
 REAL_COMPLEX cgrid_value_at_index(cgrid *grid, INT i, INT j, INT k);
 /* Host callback maps (evaluated on the host, uploaded once). */
