@@ -311,7 +311,9 @@ template <typename X> __device__ __forceinline__ void ste(void *f, uint32_t e, X
 
 // Column-tile geometry for passes along a strided axis: COLS columns per
 // block, column-fastest lanes (plan with up to 16 points per thread).
-template <typename T, int N, int EMAX = 16, int MINCOLS = 8> struct ColCfg {
+// At N = 512 in FP32 (8 points per thread, 64 threads per column) 16 columns
+// (1024 threads) give 128-byte row segments.
+template <typename T, int N, int EMAX = 16, int MINCOLS = (N >= 512 && sizeof(T) == 4 ? 16 : 8)> struct ColCfg {
   static constexpr int EM = EMAX;
   static constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL, S = RP<N, EM>::S;
   static constexpr int LP = Lay<N, EM, false, (int)sizeof(cplx<T>)>::LP;
