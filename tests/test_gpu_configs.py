@@ -73,9 +73,9 @@ def _energy(lib, f, w, vext=None, rho=None):
 
 
 def test_droplet128_fp32_tracks_fp64():
-    """Config 2 (shortened to 100 predict-correct steps): full OT droplet with the gaussian well,
+    """Config 2 at its length (1000 predict-correct steps): full OT droplet with the gaussian well,
     real time dt = 5 fs.  FP32 fields + FP64 reductions stay within 1e-6 of the FP64 path in
-    energy and norm; both keep energy and norm to 1e-3 (backflow is not norm-conserving)."""
+    energy and norm; both keep energy and norm to 5e-3 (backflow is not norm-conserving)."""
     n, step = 128, 0.4 / L.AUTOANG
     shape = (n, n, n)
     # both precisions start from the same (FP32-representable) state and potential
@@ -94,7 +94,7 @@ def test_droplet128_fp32_tracks_fp64():
         rho = L.RField(shape, step, prec)
         mu0 = f.bulk("chempot_pressurized", 0.0)
         E0, N0 = _energy(lib, f, w, ve, rho), lib.grid_wf_norm(w.h)
-        for _ in range(100):
+        for _ in range(1000):
             lib.dft_ot_propagate_pc(f.h, w.h, wp.h, ve.h, mu0, dt)
         res[prec] = (_energy(lib, f, w, ve, rho), lib.grid_wf_norm(w.h), E0, N0)
         _log("prec %d: E0 %.10f K E %.10f K N0 %.10f N %.10f" % (prec, E0 * L.AUTOK, res[prec][0] * L.AUTOK, N0,
@@ -106,9 +106,10 @@ def test_droplet128_fp32_tracks_fp64():
     assert abs(E32 / E64 - 1) < 1e-6
     assert abs(N32 / N64 - 1) < 1e-6
     # the backflow potential has an imaginary part (src/ot.c:840-991), so even
-    # real-time dynamics change the norm slightly (7e-5 over these 100 steps in FP64)
-    assert abs(N64 / N064 - 1) < 1e-3
-    assert abs(E64 / E064 - 1) < 1e-3
+    # real-time dynamics change the norm and the energy (1.2e-3 and 1.5e-3 over
+    # these 1000 steps in FP64)
+    assert abs(N64 / N064 - 1) < 5e-3
+    assert abs(E64 / E064 - 1) < 5e-3
 
 
 def test_vortex256_single_step_matches_oracle(oracle):
@@ -138,7 +139,7 @@ def test_vortex256_single_step_matches_oracle(oracle):
 
 
 def test_vortex256_circulation_and_conservation():
-    """Config 3 (shortened to 100 steps): circulation around the core on a 10 A loop stays
+    """Config 3 at its length (500 steps): circulation around the core on a 10 A loop stays
     one quantum 2 pi hbar / M; FP32 energy within 1e-6 of FP64 from the same (FP32-representable)
     initial state.  Also logged: how far the FP64 energy moves when only the initial state is
     rounded to FP32 (the sensitivity of these dynamics to one rounding of psi)."""
@@ -156,7 +157,7 @@ def test_vortex256_circulation_and_conservation():
         w.set(psi0)
         mu = f.bulk("dEdRho", f.p.rho0)
         E0, N0 = _energy(lib, f, w), lib.grid_wf_norm(w.h)
-        for _ in range(100):
+        for _ in range(500):
             lib.dft_ot_propagate(f.h, w.h, None, mu, dt)
         psi = w.get()
         # winding of the phase on a circle of radius 10 A in the mid plane
@@ -178,10 +179,11 @@ def test_vortex256_circulation_and_conservation():
     E64, N64, E064, N064, _ = res[(L.GRID_FP64, True)]
     E64x = res[(L.GRID_FP64, False)][0]
     E32, N32, _, _, _ = res[(L.GRID_FP32, True)]
-    _log("FP32 vs FP64 energy after 100 steps: %.3e; FP64 sensitivity to rounding psi0 once: %.3e" % (
+    _log("FP32 vs FP64 energy after 500 steps: %.3e; FP64 sensitivity to rounding psi0 once: %.3e" % (
         abs(E32 / E64 - 1), abs(E64x / E64 - 1)))
     assert abs(E32 / E64 - 1) < 1e-6
-    assert abs(N64 / N064 - 1) < 1e-3
+    # the backflow potential's imaginary part changes the norm (1.2e-3 over 500 steps)
+    assert abs(N64 / N064 - 1) < 5e-3
 
 
 def test_dftrun_driver_runs():
