@@ -241,6 +241,57 @@ def size_sweep(steps=10, warmup=3):
     return out
 
 
+def config5_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
+    """The 512^3 released-cavity workload (h = 0.4 A, 17 A cavity with a 0.5 A wall, 2 fs
+    steps; KC functional, the adjoint's scope): bubble radius (missing volume within a
+    25.5 A window, as dft_ot_adjoint_cavity defines it) after `steps` steps and its
+    derivative with respect to the initial cavity radius from the checkpointed adjoint sweep
+    (ceil(steps/every) + every stored states), against central differences of two forward runs."""
+    n, step = 512, 0.4 / L.AUTOANG
+    rho0 = 0.0218360 * L.AUTOANG ** 3
+    R0, wd, dt = 17.0 / L.AUTOANG, 0.5 / L.AUTOANG, 2.0 / L.AUTOFS
+    lib = L.lib()
+    w = L.Wave((n, n, n), step, prec)
+    f = L.Functional(1, w)
+    mu = f.bulk("dEdRho", rho0)
+    log("config 5 adjoint: %d steps, checkpoints every %d" % (steps, every))
+    lib.dft_set_verbose(1)  # a progress line per checkpoint
+    lib.grid_synchronize()
+    t0 = time.perf_counter()
+    J, g = f.adjoint_cavity(rho0, R0, wd, mu, dt, steps, every)
+    t_adj = time.perf_counter() - t0
+    lib.dft_set_verbose(0)
+
+    def radius(R):
+        w.set(inputs.cavity((n, n, n), step, rho0, R * L.AUTOANG, wd * L.AUTOANG))
+        for it in range(steps):
+            lib.dft_ot_propagate(f.h, w.h, None, mu, L.Complex.of(dt))
+            if it % 500 == 0:
+                lib.grid_synchronize()
+                log("config 5 forward R0 = %.4f A: step %d" % (R * L.AUTOANG, it))
+        # the bubble measure of dft_ot_adjoint_cavity: the missing volume inside the window
+        x = (np.arange(n) - n // 2) * step
+        r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+        win = 0.5 * (1.0 - np.tanh((r - 1.5 * R) / (4.0 * wd)))  # the window moves with the cavity radius
+        del r
+        rho = np.abs(w.get()) ** 2
+        vb = step ** 3 * np.sum(win * (1.0 - rho / rho0))
+        return (3.0 * vb / (4.0 * np.pi)) ** (1.0 / 3.0)
+
+    log("config 5: forward runs for the finite difference")
+    t0 = time.perf_counter()
+    d = rel_fd * R0
+    fd = (radius(R0 + d) - radius(R0 - d)) / (2 * d)
+    t_fwd = (time.perf_counter() - t0) / 2
+    f.free(), w.free()
+    states = -(-steps // every) + every
+    return {"workload": "released 17 A cavity, 512^3, h = 0.4 A, KC functional", "dtype": "f32" if prec == L.GRID_FP32
+            else "f64", "steps": steps, "every": every, "stored_states": states,
+            "stored_GB": states * n ** 3 * (8 if prec == L.GRID_FP32 else 16) / 1e9,
+            "R_b_A": J * L.AUTOANG, "dRb_dR0_adjoint": g, "dRb_dR0_fd": fd, "rel_err": abs(g / fd - 1),
+            "fd_step_A": d * L.AUTOANG, "adjoint_s": t_adj, "forward_s": t_fwd}
+
+
 def accuracy(n, step, psi0, dt, k):
     """FP32 fields + FP64 reductions against the FP64 path: k steps from the same
     (FP32-representable) state; and one step against the FP64 oracle (kernels sampled
@@ -272,6 +323,8 @@ def main():
     ap.add_argument("--acc-steps", type=int, default=50,
                     help="steps of the FP32 vs FP64 energy comparison (0: skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the per-size steps/s table")
+    ap.add_argument("--config5-adjoint", action="store_true",
+                    help="run the 512^3 released-cavity adjoint (2000 steps) against finite differences and exit")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the HBM traffic table from two rocprofv3 --pmc counter_collection.csv files and exit")
     ap.add_argument("--kernel-stats", metavar="STATS_CSV",
@@ -282,6 +335,9 @@ def main():
         return
     if args.kernel_stats:
         print(summarize_kernel_stats(args.kernel_stats))
+        return
+    if args.config5_adjoint:
+        print(json.dumps(config5_adjoint()))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,

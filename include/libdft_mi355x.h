@@ -328,13 +328,17 @@ int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, R
 /* Released cavity: writes psi_0 = sqrt(rho0 (1 + tanh((r - radius)/width)) / 2)
    (r = |r_i|, r_i = (i - n/2) step) into gwf, propagates it nsteps steps of
    dft_ot_propagate(otf, ., no V_ext, mu, time) and returns the bubble radius
-   R_b = (3 V_b / 4 pi)^(1/3), V_b = h^3 sum (1 - rho / rho0), of the final state
-   and its derivative with respect to the initial cavity radius, from the same
+   R_b = (3 V_b / 4 pi)^(1/3), V_b = h^3 sum W(r) (1 - rho / rho0) with the window
+   W = (1 - tanh((r - 1.5 radius) / (4 width))) / 2 (the missing volume of the
+   whole box is the conserved N / rho0), of the final state and its derivative
+   with respect to the initial cavity radius, from the same
    checkpointed adjoint sweep as dft_ot_adjoint_well.  PLAIN and KC models;
    returns 0, or -1 (message on stderr) for backflow models, invalid arguments
    or lack of device memory. */
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad);
+/* Progress lines of the adjoint sweeps on stderr (level > 0), one per checkpoint. */
+void dft_set_verbose(int level);
 
 /* ---- measurement (extensions; no counterpart in the reference) ---- */
 /* Per-launch HIP-event timing of the library's kernels on its stream. */

@@ -124,6 +124,7 @@ _SIGS = {
     "dft_ot_adjoint_well": ("i", ["otf", "wf", "d", "d", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_adjoint_cavity": ("i", ["otf", "wf", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_dispersion": ("d", ["wf", "otf", "d", "p", "d", "b", "b"]),
+    "dft_set_verbose": (None, ["i"]),
     "rgrid_write_grid": (None, ["s", "rg"]),
     "rgrid_read_grid": (None, ["rg", "s"]),
     "cgrid_write_grid": (None, ["s", "cg"]),

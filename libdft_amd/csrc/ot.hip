@@ -1671,6 +1671,9 @@ void adj_step(dft_ot_functional *otf, const cplx<T> *psi, cplx<T> *psib, double 
 
 // Gradient of E[psi_K] with respect to (amp, width) of the well, for K single
 // steps of dft_ot_propagate from the state in gwf (left unchanged), by an
+// Progress lines of the long-running sweeps on stderr (dft_set_verbose).
+int g_verbose = 0;
+
 // Checkpointed adjoint of K single steps of dft_ot_propagate from the state in
 // gwf (left unchanged), with the well wl as V_ext.  The forward pass keeps every
 // `every`-th state; the reverse pass recomputes each segment from its
@@ -1711,7 +1714,13 @@ int adjoint_sweep(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, do
   // forward sweep
   DFT_HIP_CHECK(hipMemcpyAsync(cur->grid->dev, gwf->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
   for (int64_t it = 0; it < K; ++it) {
-    if (it % every == 0) DFT_HIP_CHECK(hipMemcpyAsync(cp[it / every], cur->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
+    if (it % every == 0) {
+      DFT_HIP_CHECK(hipMemcpyAsync(cp[it / every], cur->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
+      if (g_verbose) {
+        DFT_HIP_CHECK(hipStreamSynchronize(s));
+        fprintf(stderr, "libdft: adjoint forward step %lld / %lld\n", (long long)it, (long long)K);
+      }
+    }
     dft_ot_propagate(otf, cur, vext, mu, dt);
   }
   seed(cur, psib, th);
@@ -1725,6 +1734,7 @@ int adjoint_sweep(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, do
     }
     for (int64_t it = n1 - 1; it >= n0; --it)
       adj_step<T>(otf, seg[it - n0], psib, mass, wl, mu, dtr, dti, w, th);
+    if (g_verbose) fprintf(stderr, "libdft: adjoint reverse segment %lld / %lld done\n", (long long)(ncp - j), (long long)ncp);
   }
   cleanup();
   return 0;
@@ -1782,19 +1792,40 @@ template <typename T> __global__ void k_cavity(cplx<T> *psi, Well g, double rho0
   }
 }
 
-// Per block: sum (1 - |psi|^2 / rho0) (the missing volume over h^3).
-template <typename T> __global__ void k_cavity_volume(const cplx<T> *psi, double rho0, double *pa, double *pb, int64_t n) {
-  double sa = 0.0;
-  GRID_STRIDE(c, n) {
-    const double pr = psi[c].x, pi = psi[c].y;
-    sa += 1.0 - (pr * pr + pi * pi) / rho0;
-  }
-  block_sum2(sa, 0.0, pa, pb);
+// Window of the bubble measure: W(r) = (1 - tanh((r - Rc)/wc)) / 2.  The
+// missing volume of the whole box is the conserved norm (N / rho0) and says
+// nothing about where the deficit is; inside the window it is the bubble's.
+// Also dW/dRc = (1 - tanh^2) / (2 wc).
+__device__ inline double cavity_window(const Well &g, int64_t c, double Rc, double wc, double *dW = nullptr) {
+  const int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((int64_t)g.nz * g.ny));
+  const double x = (i - g.nx / 2) * g.h, y = (j - g.ny / 2) * g.h, z = (k - g.nz / 2) * g.h;
+  const double th = tanh((sqrt(x * x + y * y + z * z) - Rc) / wc);
+  if (dW) *dW = (1.0 - th * th) / (2.0 * wc);
+  return 0.5 * (1.0 - th);
 }
 
-// psib = f psi (f real).
-template <typename T> __global__ void k_scale_into(cplx<T> *psib, const cplx<T> *psi, double f, int64_t n) {
-  GRID_STRIDE(c, n) psib[c] = cmk<cplx<T>>((T)(f * (double)psi[c].x), (T)(f * (double)psi[c].y));
+// Per block: sum W (1 - |psi|^2 / rho0) (the missing volume in the window over
+// h^3) and sum dW/dRc (1 - |psi|^2 / rho0).
+template <typename T>
+__global__ void k_cavity_volume(const cplx<T> *psi, Well g, double Rc, double wc, double rho0, double *pa, double *pb,
+                                int64_t n) {
+  double sa = 0.0, sb = 0.0;
+  GRID_STRIDE(c, n) {
+    const double pr = psi[c].x, pi = psi[c].y, m = 1.0 - (pr * pr + pi * pi) / rho0;
+    double dW;
+    sa += cavity_window(g, c, Rc, wc, &dW) * m;
+    sb += dW * m;
+  }
+  block_sum2(sa, sb, pa, pb);
+}
+
+// psib = f W psi (f real).
+template <typename T>
+__global__ void k_cavity_seed(cplx<T> *psib, const cplx<T> *psi, Well g, double Rc, double wc, double f, int64_t n) {
+  GRID_STRIDE(c, n) {
+    const double fw = f * cavity_window(g, c, Rc, wc);
+    psib[c] = cmk<cplx<T>>((T)(fw * (double)psi[c].x), (T)(fw * (double)psi[c].y));
+  }
 }
 
 // Per block: sum Re(conj(psib) dpsi_0/dR0) with dpsi_0/dR0 = -psi_0 (1 - tanh s) / (2w), s = (r - R0)/w.
@@ -1813,7 +1844,9 @@ __global__ void k_cavity_grad(const cplx<T> *psib, Well g, double rho0, double R
 }
 
 // Released cavity: psi_0(R0) written to gwf, K steps without V_ext, J = the
-// bubble radius (3 V_b / 4 pi)^(1/3), V_b = h^3 sum (1 - rho / rho0), and dJ/dR0.
+// bubble radius (3 V_b / 4 pi)^(1/3), V_b = h^3 sum W (1 - rho / rho0) with the
+// window W of radius Rc = 1.5 R0 and wall wc = 4 w, and dJ/dR0: the path
+// through psi_0 (adjoint sweep) plus the window's own dependence on R0.
 template <typename T>
 int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0, double wd, double mu, double dtr,
                         double dti, int64_t K, int64_t every, double *J, double *grad) {
@@ -1822,22 +1855,25 @@ int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0,
   const int64_t n = d->n;
   const double h3 = d->step * d->step * d->step;
   const Well geo{0.0, 1.0, 0.0, 0.0, 0.0, d->step, d->nx, d->ny, d->nz};
+  const double Rc = 1.5 * R0, wc = 4.0 * wd;
   hipLaunchKernelGGL(k_cavity<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, (cplx<T> *)gwf->grid->dev, geo, rho0, R0, wd,
                      n);
   DFT_LAUNCH_CHECK();
   AdjWork<T> w;
   cplx<T> *psib = nullptr;
   bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP) && (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
-  double th[2] = {0.0, 0.0};
+  double th[2] = {0.0, 0.0}, explicit_term = 0.0;
   auto seed = [&](wf *cur, cplx<T> *pb, double *) {
     const cplx<T> *psiK = (const cplx<T> *)cur->grid->dev;
-    hipLaunchKernelGGL(k_cavity_volume<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, psiK, rho0, w.pa, w.pw,
-                       n);
+    hipLaunchKernelGGL(k_cavity_volume<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, psiK, geo, Rc, wc, rho0,
+                       w.pa, w.pw, n);
     DFT_LAUNCH_CHECK();
     const double V = h3 * sum_partials(w.pa, red_blocks(n), s);
     *J = std::cbrt(3.0 * V / (4.0 * M_PI));
-    // dJ/dpsi = (J / 3V) dV/dpsi, dV/dpsi = -2 h^3 psi / rho0
-    hipLaunchKernelGGL(k_scale_into<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, pb, psiK,
+    // dJ/dR0 at fixed psi_K: (J / 3V) h^3 sum dW/dRc (1 - rho/rho0) dRc/dR0
+    explicit_term = (*J / (3.0 * V)) * h3 * sum_partials(w.pw, red_blocks(n), s) * 1.5;
+    // dJ/dpsi = (J / 3V) dV/dpsi, dV/dpsi = -2 h^3 W psi / rho0
+    hipLaunchKernelGGL(k_cavity_seed<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, pb, psiK, geo, Rc, wc,
                        (*J / (3.0 * V)) * (-2.0 * h3 / rho0), n);
     DFT_LAUNCH_CHECK();
   };
@@ -1848,7 +1884,7 @@ int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0,
     hipLaunchKernelGGL(k_cavity_grad<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, (const cplx<T> *)psib, geo,
                        rho0, R0, wd, w.pa, w.pw, n);
     DFT_LAUNCH_CHECK();
-    *grad = sum_partials(w.pa, red_blocks(n), s);
+    *grad = sum_partials(w.pa, red_blocks(n), s) + explicit_term;
   }
   DFT_HIP_CHECK(hipStreamSynchronize(s));
   dfree(psib);
@@ -2164,6 +2200,8 @@ int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, R
   grad[1] = g[1];
   return rc;
 }
+
+void dft_set_verbose(int level) { g_verbose = level; }
 
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
