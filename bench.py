@@ -241,7 +241,7 @@ def size_sweep(steps=10, warmup=3):
     return out
 
 
-def config5_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
+def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
     """The 512^3 released-cavity workload (h = 0.4 A, 17 A cavity with a 0.5 A wall, 2 fs
     steps; KC functional, the adjoint's scope): bubble radius (missing volume within a
     25.5 A window, as dft_ot_adjoint_cavity defines it) after `steps` steps and its
@@ -254,7 +254,7 @@ def config5_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
     w = L.Wave((n, n, n), step, prec)
     f = L.Functional(1, w)
     mu = f.bulk("dEdRho", rho0)
-    log("config 5 adjoint: %d steps, checkpoints every %d" % (steps, every))
+    log("512^3 cavity adjoint: %d steps, checkpoints every %d" % (steps, every))
     lib.dft_set_verbose(1)  # a progress line per checkpoint
     lib.grid_synchronize()
     t0 = time.perf_counter()
@@ -268,7 +268,7 @@ def config5_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
             lib.dft_ot_propagate(f.h, w.h, None, mu, L.Complex.of(dt))
             if it % 500 == 0:
                 lib.grid_synchronize()
-                log("config 5 forward R0 = %.4f A: step %d" % (R * L.AUTOANG, it))
+                log("512^3 cavity forward R0 = %.4f A: step %d" % (R * L.AUTOANG, it))
         # the bubble measure of dft_ot_adjoint_cavity: the missing volume inside the window
         x = (np.arange(n) - n // 2) * step
         r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
@@ -278,7 +278,7 @@ def config5_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
         vb = step ** 3 * np.sum(win * (1.0 - rho / rho0))
         return (3.0 * vb / (4.0 * np.pi)) ** (1.0 / 3.0)
 
-    log("config 5: forward runs for the finite difference")
+    log("512^3 cavity: forward runs for the finite difference")
     t0 = time.perf_counter()
     d = rel_fd * R0
     fd = (radius(R0 + d) - radius(R0 - d)) / (2 * d)
@@ -323,7 +323,7 @@ def main():
     ap.add_argument("--acc-steps", type=int, default=50,
                     help="steps of the FP32 vs FP64 energy comparison (0: skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the per-size steps/s table")
-    ap.add_argument("--config5-adjoint", action="store_true",
+    ap.add_argument("--cavity512-adjoint", action="store_true",
                     help="run the 512^3 released-cavity adjoint (2000 steps) against finite differences and exit")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the HBM traffic table from two rocprofv3 --pmc counter_collection.csv files and exit")
@@ -336,8 +336,8 @@ def main():
     if args.kernel_stats:
         print(summarize_kernel_stats(args.kernel_stats))
         return
-    if args.config5_adjoint:
-        print(json.dumps(config5_adjoint()))
+    if args.cavity512_adjoint:
+        print(json.dumps(cavity512_adjoint()))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,

@@ -28,7 +28,7 @@ def coords(shape, step):
 
 
 def noisy_bulk(shape, rho, amp, seed=SEED):
-    """psi = sqrt(rho) (1 + amp U(-1,1)), zero phase (config 1 style)."""
+    """psi = sqrt(rho) (1 + amp U(-1,1)), zero phase (the imaginary-time bulk start)."""
     u = splitmix64_uniform(int(np.prod(shape)), seed).reshape(shape)
     return (np.sqrt(rho) * (1.0 + amp * u)).astype(np.complex128)
 

@@ -2,9 +2,9 @@
 V_OT(rho) against central finite differences of the potential (dot-product
 test), the adjoint split-operator sweep against central finite differences
 of the final energy with respect to the gaussian well's depth and width
-(config 4(ii): 128^3 droplet with the -50 K / 2 A impurity well), and of the
+(128^3 droplet with the -50 K / 2 A impurity well), and of the
 bubble radius of a released cavity with respect to its initial radius
-(config 5: 512^3, 17 A cavity).
+(512^3, 17 A cavity).
 """
 import numpy as np
 import pytest
@@ -104,8 +104,8 @@ def test_adjoint_well_imaginary_time():
     _fd_check(1, shape, step, psi0, amp, width, 0.0, -1j * 2.0 / L.AUTOFS, 6, 4, 1e-6)
 
 
-def test_adjoint_config4_droplet128():
-    """Config 4(ii) at its size: 128^3 droplet (N = 1000) with the -50 K, 2 A gaussian well,
+def test_adjoint_droplet128_well():
+    """At full size: 128^3 droplet (N = 1000) with the -50 K, 2 A gaussian well,
     500 real-time steps of 5 fs, checkpoints every 50; KC functional (the adjoint covers the
     PLAIN and KC models).  Adjoint gradient vs central FD of E_final: <= 1e-4."""
     shape, step = (128, 128, 128), 0.4 / L.AUTOANG
@@ -140,8 +140,8 @@ def test_adjoint_cavity_small(model):
                5.0 / L.AUTOFS, 12, 5, 1e-6)
 
 
-def test_adjoint_cavity_config5_512():
-    """Config 5 at its size (512^3, h = 0.4 A, 17 A cavity with a 0.5 A wall, released;
+def test_adjoint_cavity_512():
+    """At full size (512^3, h = 0.4 A, 17 A cavity with a 0.5 A wall, released;
     KC functional: the adjoint covers PLAIN and KC), 40 steps of 2 fs with checkpoints
     every 10, FP64: dR_b/dR0 against central FD, <= 1e-4."""
     rho0 = 0.0218360 * L.AUTOANG ** 3

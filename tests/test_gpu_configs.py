@@ -1,4 +1,4 @@
-"""Workload-scale GPU checks (the BASELINE configurations), against the FP64 oracle
+"""Workload-scale GPU checks (64^3 bulk, 128^3 droplet, 256^3 vortex line, 512^3 cavity), against the FP64 oracle
 where the oracle finishes in seconds and through size-independent properties
 (norm and energy conservation, FP32 vs FP64 agreement, quantised circulation)
 where it does not.
@@ -23,7 +23,7 @@ def _log(msg):
 
 
 def test_bulk64_imaginary_time_matches_oracle(oracle):
-    """Config 1: plain OT 64^3, 200 predict-correct imaginary-time steps (dt = 10 fs) with
+    """Plain OT 64^3, 200 predict-correct imaginary-time steps (dt = 10 fs) with
     renormalisation; final E/N and mu against the CPU oracle run of the same steps."""
     n, step = 64, 0.5 / L.AUTOANG
     shape = (n, n, n)
@@ -73,7 +73,7 @@ def _energy(lib, f, w, vext=None, rho=None):
 
 
 def test_droplet128_fp32_tracks_fp64():
-    """Config 2 at its length (1000 predict-correct steps): full OT droplet with the gaussian well,
+    """Full length (1000 predict-correct steps): full OT 128^3 droplet with the gaussian well,
     real time dt = 5 fs.  FP32 fields + FP64 reductions stay within 1e-6 of the FP64 path in
     energy and norm; both keep energy and norm to 5e-3 (backflow is not norm-conserving)."""
     n, step = 128, 0.4 / L.AUTOANG
@@ -113,7 +113,7 @@ def test_droplet128_fp32_tracks_fp64():
 
 
 def test_vortex256_single_step_matches_oracle(oracle):
-    """Config 3 grid (256^3, h = 0.3 A, full OT, vortex line): one fused single step on the
+    """Vortex line at 256^3 (h = 0.3 A, full OT): one fused single step on the
     FP64 device path against the oracle (kernels sampled once per cell on both sides)."""
     n, step = 256, 0.3 / L.AUTOANG
     shape = (n, n, n)
@@ -139,7 +139,7 @@ def test_vortex256_single_step_matches_oracle(oracle):
 
 
 def test_vortex256_circulation_and_conservation():
-    """Config 3 at its length (500 steps): circulation around the core on a 10 A loop stays
+    """Vortex line at 256^3 over 500 steps: circulation around the core on a 10 A loop stays
     one quantum 2 pi hbar / M; FP32 energy within 1e-6 of FP64 from the same (FP32-representable)
     initial state.  Also logged: how far the FP64 energy moves when only the initial state is
     rounded to FP32 (the sensitivity of these dynamics to one rounding of psi)."""
