@@ -93,7 +93,7 @@ void launch_zc2r(const Plan3D<T> &p, const fft::Batch &b, int nb, T scale, hipSt
 template <typename T, int N, int DIR>
 void launch_strided(const Plan3D<T> &p, const fft::Batch &b, int nb, int axis, int cols, int pitch, T scale,
                     hipStream_t s) {
-  using G = rf::ColCfg<T, N>;
+  using G = fft::StrCfg<T, N>;
   const int ntile = (cols + G::COLS - 1) / G::COLS;
   int64_t es, os;
   int outer;
@@ -152,7 +152,7 @@ void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cpl
 template <typename T, int N>
 void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, double cx, double cy, double cz, double dtr, double dti,
                  hipStream_t s) {
-  using G = rf::ColCfg<T, N>;
+  using G = fft::KinCfg<T, N>;
   const int ntile = (p.nz + G::COLS - 1) / G::COLS;
   dim3 grid((unsigned)(p.ny * ntile));
   DFT_PROF_BEGIN(s);

@@ -140,12 +140,23 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------- strided c2c pass (y or x axis) --
+// Geometry: DFT_STR_EM points per thread, at least DFT_STR_COLS columns per
+// block (N = 512 keeps the column default).
+#ifndef DFT_STR_EM
+#define DFT_STR_EM 16
+#endif
+#ifndef DFT_STR_COLS
+#define DFT_STR_COLS 8
+#endif
+template <typename T, int N>
+using StrCfg = typename std::conditional<(N >= 512), ColCfg<T, N>, ColCfg<T, N, DFT_STR_EM, DFT_STR_COLS>>::type;
+
 // Line (o, c): element e at o*os + e*es + c, c in [0, P).  blockIdx.x = o*ntile + tile.
 template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((ColCfg<T, N>::BS))
+__global__ void __launch_bounds__((StrCfg<T, N>::BS))
     k_strided(Batch b, int P, int64_t es, int64_t os, int ntile, const cplx<T> *__restrict__ g_tw, T scale) {
   using C = cplx<T>;
-  using G = ColCfg<T, N>;
+  using G = StrCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];
@@ -283,12 +294,20 @@ __global__ void __launch_bounds__(256)
 // rounded value plus its residual, so that its modulus is exact: a phase of
 // the same mode is applied every step, and a rounded one would change that
 // mode's amplitude by the same factor every time.
+// Geometry: at least DFT_XKIN_COLS columns per block (N = 512 keeps the
+// column default).
+#ifndef DFT_XKIN_COLS
+#define DFT_XKIN_COLS 8
+#endif
 template <typename T, int N>
-__global__ void __launch_bounds__((ColCfg<T, N>::BS))
+using KinCfg = typename std::conditional<(N >= 512), ColCfg<T, N>, ColCfg<T, N, 16, DFT_XKIN_COLS>>::type;
+
+template <typename T, int N>
+__global__ void __launch_bounds__((KinCfg<T, N>::BS))
     k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, double cx, double cy, double cz,
            double dtr, double dti) {
   using C = cplx<T>;
-  using G = ColCfg<T, N>;
+  using G = KinCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];

@@ -890,9 +890,11 @@ template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho,
 }
 
 // x-middle implementation: 0 = all outputs of a column tile in one block
-// (k_xmid), 1 = one output per block (k_xconv)
+// (k_xmid), 1 = one output per block (k_xconv) in FP32 for the programs
+// without derivative terms, 2 = k_xconv for every FP32 program, 3 = as 1 in
+// FP64 too (N <= 256)
 #ifndef DFT_XMID_KIND
-#define DFT_XMID_KIND 1
+#define DFT_XMID_KIND 3
 #endif
 
 template <typename T, int N, int NK, bool DV>
@@ -909,13 +911,13 @@ const char *launch_xconv(const fused::MidProg *pg, const fused::MidProg &hp, boo
   // the accumulating one
   int n1 = 0;
   while (n1 < hp.nout && hp.o[n1].nterms == 1 && hp.o[n1].t[0].kern >= 0 && hp.o[n1].t[0].deriv < 0) ++n1;
-  using Cfg = fused::XConvCfg<T, N>;
-  const int nb = fused::mid_blocks<Cfg::COLS>(ny, P);
+  using Cfg1 = fused::XConvCfg<T, N, true>;
+  using Cfg = fused::XConvCfg<T, N, false>;
   if (n1 > 0)
-    hipLaunchKernelGGL((fused::k_xconv<T, N, false, true>), dim3((unsigned)(nb * n1)), dim3(Cfg::BS), 0, s, pg, 0, n1,
-                       ny, P, PP, tw);
+    hipLaunchKernelGGL((fused::k_xconv<T, N, false, true>), dim3((unsigned)(fused::mid_blocks<Cfg1::COLS>(ny, P) * n1)),
+                       dim3(Cfg1::BS), 0, s, pg, 0, n1, ny, P, PP, tw);
   if (n1 < hp.nout) {
-    const dim3 grid((unsigned)(nb * (hp.nout - n1)));
+    const dim3 grid((unsigned)(fused::mid_blocks<Cfg::COLS>(ny, P) * (hp.nout - n1)));
     if (deriv)
       hipLaunchKernelGGL((fused::k_xconv<T, N, true, false>), grid, dim3(Cfg::BS), 0, s, pg, n1, hp.nout - n1, ny, P,
                          PP, tw);
@@ -926,12 +928,12 @@ const char *launch_xconv(const fused::MidProg *pg, const fused::MidProg &hp, boo
   return n1 == hp.nout ? "ot_xconv_single" : (n1 == 0 ? "ot_xconv_multi" : "ot_xconv_mixed");
 }
 
-// x-middle pass of program pg (device copy; host copy hp).  In FP32, programs
-// without derivative terms run one output per block (k_xconv); the others,
-// and FP64, all outputs of a column tile per block (k_xmid, instantiated per
-// number of distinct kernels): measured at 256^3 FP32, k_xconv takes 337 and
-// 139 us for the first and last programs against 427 and 158, k_xmid 266 us
-// for the derivative program against 380.
+// x-middle pass of program pg (device copy; host copy hp).  Programs without
+// derivative terms run one output per block (k_xconv; FP64 only up to
+// N = 256); the others, and FP64 at N = 512, all outputs of a column tile per
+// block (k_xmid, instantiated per number of distinct kernels): measured at
+// 256^3 FP32, k_xconv takes 337 and 139 us for the first and last programs
+// against 427 and 158, k_xmid 266 us for the derivative program against 380.
 template <typename T, int N>
 void launch_xmid(const fused::MidProg *pg, const fused::MidProg &hp, int ny, int P, int PP, const cplx<T> *tw,
                  hipStream_t s, double bytes) {
@@ -941,10 +943,11 @@ void launch_xmid(const fused::MidProg *pg, const fused::MidProg &hp, int ny, int
   DFT_PROF_BEGIN(s);
   bool one = true;
   for (int m = 0; m < hp.nout; ++m) one = one && hp.o[m].nterms == 1;
-  if constexpr (DFT_XMID_KIND == 1 && sizeof(T) == 4) {
+  if constexpr ((DFT_XMID_KIND == 1 || DFT_XMID_KIND == 2) && sizeof(T) == 4 ||
+                DFT_XMID_KIND == 3 && (sizeof(T) == 4 || N <= 256)) {
     // per-output blocks unless the program shares inputs between
     // multi-term outputs with derivative symbols (J_d and T1 = F sum_d D_d G_d)
-    if (one || !deriv) {
+    if (one || !deriv || DFT_XMID_KIND == 2) {
       const char *label = launch_xconv<T, N>(pg, hp, deriv, ny, P, PP, tw, s);
       DFT_LAUNCH_CHECK();
       DFT_PROF_END(s, label, bytes);

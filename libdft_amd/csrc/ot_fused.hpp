@@ -372,16 +372,41 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
 // over the terms of output m.  An input shared by several outputs is read once
 // per output; the outputs of one column tile are consecutive logical blocks
 // on one XCD, so those re-reads mostly hit that XCD's L2.  Holding one line
-// (16 points per thread, 16 columns = 128-byte row segments per block) plus
-// an accumulator keeps the register count low enough for four waves per SIMD.
-template <typename T, int N> using XConvCfg = ColCfg<T, N, 16, 16>;
+// (16 points per thread) plus an accumulator keeps the register count low
+// enough for four waves per SIMD.  Single-term outputs take 32 columns per
+// block (256-byte row segments in FP32), the accumulating form 16.
+// DFT_XCONV1_EM / DFT_XCONV1_COLS set that geometry for the single-term form,
+// DFT_XCONV_EM / DFT_XCONV_COLS for the accumulating one; DFT_XCONV_MINB asks
+// the register allocator for that many waves per SIMD (0: no request).
+#ifndef DFT_XCONV1_EM
+#define DFT_XCONV1_EM 16
+#endif
+#ifndef DFT_XCONV1_COLS
+#define DFT_XCONV1_COLS 32
+#endif
+#ifndef DFT_XCONV_EM
+#define DFT_XCONV_EM 16
+#endif
+#ifndef DFT_XCONV_COLS
+#define DFT_XCONV_COLS 16
+#endif
+#ifndef DFT_XCONV_MINB
+#define DFT_XCONV_MINB 0
+#endif
+// (N = 512 keeps 16 points per thread and 16 columns; FP64 takes at most 32
+// columns, which fill the LDS.)
+template <typename T, int N, bool ONE>
+using XConvCfg = ColCfg<T, N, N >= 512 ? 16 : (ONE ? DFT_XCONV1_EM : DFT_XCONV_EM),
+                        N >= 512 ? 16
+                                 : (ONE ? (sizeof(T) == 8 && DFT_XCONV1_COLS > 32 ? 32 : DFT_XCONV1_COLS)
+                                        : (sizeof(T) == 8 && DFT_XCONV_COLS > 32 ? 32 : DFT_XCONV_COLS))>;
 
 template <typename T, int N, bool DERIV, bool ONE>
-__global__ void __launch_bounds__((XConvCfg<T, N>::BS)) k_xconv(const MidProg *__restrict__ pgp, int m0, int nl, int ny,
+__global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), DFT_XCONV_MINB) k_xconv(const MidProg *__restrict__ pgp, int m0, int nl, int ny,
                                                               int P, int PP, const cplx<T> *__restrict__ g_tw) {
   const MidProg &pg = *pgp;
   using C = cplx<T>;
-  using G = XConvCfg<T, N>;
+  using G = XConvCfg<T, N, ONE>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
   __shared__ C s_tw[2 * N];

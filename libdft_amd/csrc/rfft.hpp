@@ -319,6 +319,7 @@ template <typename T, int N, int EMAX = 16, int MINCOLS = (N >= 512 && sizeof(T)
   static constexpr int LP = Lay<N, EM, false, (int)sizeof(cplx<T>)>::LP;
   static constexpr int COLS = 256 / TPL < MINCOLS ? MINCOLS : 256 / TPL;
   static constexpr int BS = COLS * TPL;
+  static_assert(BS <= 1024, "column block larger than 1024 threads");
 };
 
 // Line geometry for passes along the contiguous axis: one line (or line pair)
