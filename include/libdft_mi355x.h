@@ -202,7 +202,8 @@ REAL grid_timer_wall_clock_time(grid_timer *timer);
 
 /* ============================================================ OT functional */
 
-/* Model flags, src/ot.h:37-60 (PLAIN, KC and BACKFLOW are implemented). */
+/* Model flags, src/ot.h:37-60 (PLAIN, KC and BACKFLOW are implemented, and the
+   HD/HD2 high-density penalty without BACKFLOW). */
 #define DFT_OT_PLAIN    0
 #define DFT_OT_KC       1
 #define DFT_OT_HD       2

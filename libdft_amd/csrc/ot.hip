@@ -175,6 +175,7 @@ __global__ void k_stageB(StageBIn in, StageBOut out, PW<T> pw) {
     if constexpr (LOC) {
       const T rb = ((const T *)in.rhobar)[c];
       vre = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
+      if (pw.hdC != (T)0) vre += hd_pot(r, pw);
       ((T *)out.Q)[c] = (rb * pw.c2 + rb * rb * pw.c3) * r;
     }
     T g[3] = {0, 0, 0};
@@ -286,6 +287,7 @@ __global__ void k_energyB(EnergyIn in, T *eps, T *Gx, T *Gy, T *Gz, PW<T> pw, in
       e += (T)0.5 * r * ((const T *)in.ljr)[c];
       e += (pw.c2 * (T)0.5) * (rb * rb * r);
       e += (pw.c3 / (T)3) * (rb * rb * rb * r);
+      if (pw.hdC != (T)0) e += hd_eps(r, pw);
     }
     if constexpr (KC) {
       const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
@@ -387,7 +389,7 @@ template <typename T, bool KC> __global__ void k_vjp_a(VjpA a, PW<T> pw) {
     const T qb = ((const T *)a.qb)[c];
     const T dl = pw.c2 * rb + pw.c3 * rb * rb;  // d(local V)/d rb, and Q/rho
     ((T *)a.rbb)[c] = vb * dl + qb * r * (pw.c2 + (T)2 * pw.c3 * rb);
-    ((T *)a.rp)[c] = ((const T *)a.lv)[c] + qb * dl;
+    ((T *)a.rp)[c] = ((const T *)a.lv)[c] + qb * dl + (pw.hdC != (T)0 ? vb * hd_dpot(r, pw) : (T)0);
     if constexpr (KC) {
       const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
       const T *rho = (const T *)a.rho;
@@ -692,6 +694,9 @@ template <typename T> PW<T> make_pw(dft_ot_functional *otf, double wf_mass, doub
   pw.bfmax = (T)otf->max_bfpot;
   pw.clamp_v = clamp_v && otf->max_veloc < (999.0 / GRID_AUTOMPS);
   pw.clamp_bf = otf->max_bfpot < (10000.0 / GRID_AUTOK);
+  pw.hdC = (T)otf->C;
+  pw.hdB = (T)otf->beta;
+  pw.hdRm = (T)otf->rhom;
   return pw;
 }
 
@@ -1412,7 +1417,8 @@ template <typename T> bool build_kernels(dft_ot_functional *otf, int min_sub, in
   kp.lj_sigma = otf->lj_params.sigma;
   kp.lj_eps = otf->lj_params.epsilon;
   kp.lj_cval = otf->lj_params.cval;
-  kp.sa_radius = otf->lj_params.h;  // original spherical-average radius (no HD2)
+  // spherical-average radius: h, or 1.065 h for HD2 (src/ot.c:164-170)
+  kp.sa_radius = (otf->model & DFT_OT_HD2) ? otf->lj_params.h * 1.065 : otf->lj_params.h;
   kp.g_inv_width = 1.0 / otf->l_g;
   kp.g11 = otf->bf_params.g11;
   kp.g12 = otf->bf_params.g12;
@@ -1482,8 +1488,12 @@ rgrid *spectral_handle(dft_ot_functional *otf, int w, const char *id) {
   return g;
 }
 
+// PLAIN, KC and BACKFLOW, and the high-density penalty (HD, HD2) without
+// backflow: with HD the reference also switches the backflow to another
+// density dependence (src/ot.c:843-982), which is not built here.
 bool supported_model(INT model) {
-  return (model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW)) == 0;
+  if (model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2)) return false;
+  return !((model & (DFT_OT_HD | DFT_OT_HD2)) && (model & DFT_OT_BACKFLOW));
 }
 
 void check_wf(dft_ot_functional *otf, wf *gwf, const char *fn) {
@@ -1932,7 +1942,11 @@ void dft_ot_init_params(dft_ot_functional *otf, INT model) {
   otf->mu0 = 0.0;
   otf->xi = 1E4 / a3;
   otf->rhobf = 0.033 * a3;
-  (void)model;
+  if (model & (DFT_OT_HD | DFT_OT_HD2)) {  // high-density penalty, src/ot.c:1004-1007
+    otf->beta = 40.0 / a3;
+    otf->rhom = 0.37 * a3;
+    otf->C = 0.1;
+  }
   otf->b /= GRID_AUTOK * a3;
   otf->c2 /= GRID_AUTOK * std::pow(GRID_AUTOANG, 3.0 * otf->c2_exp);
   otf->c3 /= GRID_AUTOK * std::pow(GRID_AUTOANG, 3.0 * otf->c3_exp);
@@ -1961,7 +1975,8 @@ void dft_ot_init_params(dft_ot_functional *otf, INT model) {
 // src/ot.c:66-258
 dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_substeps) {
   if (!supported_model(model)) {
-    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC and BACKFLOW only)\n", (long)model);
+    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC, BACKFLOW, and HD or HD2 without BACKFLOW)\n",
+            (long)model);
     return nullptr;
   }
   cgrid *g = gwf->grid;

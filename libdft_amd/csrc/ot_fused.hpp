@@ -590,11 +590,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : 0) k_zAB(
     T Q[2];
 #pragma unroll
     for (int w = 0; w < 2; ++w) {
-      const T rb = part(v[i], w);
-      const T loc = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
+      const T rb = part(v[i], w), r = ql.template at<T>(rho, w, i);
+      T loc = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
+      if (pw.hdC != (T)0) loc += hd_pot(r, pw);
       if constexpr (BF) vre[w][i] += loc;
       else vre[w][i] = loc;
-      Q[w] = (rb * pw.c2 + rb * rb * pw.c3) * ql.template at<T>(rho, w, i);
+      Q[w] = (rb * pw.c2 + rb * rb * pw.c3) * r;
     }
     v[i] = cmk<C>(Q[0], Q[1]);
   }

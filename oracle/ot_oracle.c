@@ -231,12 +231,15 @@ static void init_params(orc_ot *o, int64_t model) {
   double a3 = ORC_AUTOANG * ORC_AUTOANG * ORC_AUTOANG;
   o->b = -718.99; o->c2 = -2.411857e4; o->c2_exp = 2; o->c3 = 1.858496e6; o->c3_exp = 3;
   o->c4 = 0.0; o->temp = 0.0; o->rho0 = 0.0218360; o->lj_h = 2.1903;
-  (void) model;
+  if (model & (ORC_OT_HD | ORC_OT_HD2)) {       /* src/ot.c:1004-1007 */
+    o->hd_beta = 40.0 / a3; o->hd_rhom = 0.37 * a3; o->hd_C = 0.1;
+  }
   o->b /= ORC_AUTOK * a3;
   o->c2 /= ORC_AUTOK * pow(ORC_AUTOANG, 3.0 * o->c2_exp);
   o->c3 /= ORC_AUTOK * pow(ORC_AUTOANG, 3.0 * o->c3_exp);
   o->rho0 *= a3;
   o->lj_h /= ORC_AUTOANG;
+  o->sa_h = (model & ORC_OT_HD2) ? o->lj_h * 1.065 : o->lj_h;
   o->lj_sigma = 2.556 / ORC_AUTOANG;
   o->lj_epsilon = 10.22 / ORC_AUTOK;
   o->lj_cval = 0.0;
@@ -271,7 +274,7 @@ static inline double kfun(const orc_ot *o, int which, double x, double y, double
     return 4.0 * o->lj_epsilon * s * (s - 1.0);
   }
   case 1: {
-    double h = o->lj_h;
+    double h = o->sa_h;
     return (r2 <= h * h) ? 3.0 / (4.0 * M_PI * h * h * h) : 0.0;
   }
   case 2: {
@@ -320,7 +323,7 @@ void orc_kernel_map(orc_ot *o, int which, double *out, int min_sub, int max_sub)
 static double complex *cbuf(long n) { return calloc((size_t) n, sizeof(double complex)); }
 static double *rbuf(long n) { return calloc((size_t) n, sizeof(double)); }
 
-/* src/ot.c:66-258 (3-D, T = 0, no DR/HD/GP). */
+/* src/ot.c:66-258 (3-D, T = 0, no DR/GP; HD2's spherical-average radius). */
 orc_ot *orc_ot_alloc(int64_t model, int nx, int ny, int nz, double step, int min_sub, int max_sub) {
   orc_ot *o = calloc(1, sizeof(orc_ot));
   if (!o) return NULL;
@@ -371,6 +374,19 @@ void orc_ot_free(orc_ot *o) {
 void orc_ot_set_kc_mode(orc_ot *o, int exact) { o->kc_coef23 = exact ? 1.0 / o->rho_0s : 1.0; }
 
 /* -------------------------------------------------------- potential (a3) -- */
+
+/* High-density penalty (src/ot.c:374-378, 814-819; src/ot-energy.c:96-99),
+   whose formulas live in libgrid (grid_func4/5) and are restated from
+   Ancilotto, Barranco, Caupin, Mayol and Pi, PRB 72, 214522 (2005):
+   eps = C rho (1 + tanh(beta (rho - rho_m))), V = d eps / d rho.
+   1 + tanh(x) = 2 / (1 + e^{-2x}) and 1 - tanh(x) = 2 / (1 + e^{2x}). */
+static inline double hd_s(const orc_ot *o, double r) { return 2.0 / (1.0 + exp(-2.0 * o->hd_beta * (r - o->hd_rhom))); }
+static inline double hd_t(const orc_ot *o, double r) { return 2.0 / (1.0 + exp(2.0 * o->hd_beta * (r - o->hd_rhom))); }
+static inline double hd_pot(const orc_ot *o, double r) {
+  return o->hd_C * hd_s(o, r) * (1.0 + o->hd_beta * r * hd_t(o, r));
+}
+static inline double hd_eps(const orc_ot *o, double r) { return o->hd_C * r * hd_s(o, r); }
+static inline int has_hd(const orc_ot *o) { return (o->model & (ORC_OT_HD | ORC_OT_HD2)) != 0; }
 
 /* src/ot.c:524-532 and 547-586: LJ + local correlation.  rho_tf is FFT(rho). */
 static void add_lj_local(orc_ot *o, const geom *g, double complex *pot, const double *rho,
@@ -508,6 +524,8 @@ static void potential_body(orc_ot *o, double complex *pot, const double *rho, do
   r_fft(&g, rho, rho_tf);
   add_lj_local(o, &g, pot, rho, rho_tf, cw1, w[0], w[1]);
   if (o->model & ORC_OT_KC) add_kc(o, &g, pot, rho, rho_tf, cw1, cw2, w[0], w[1], w[2], w[3], w[4]);
+  if (has_hd(o))                               /* src/ot.c:374-378 */
+    for (long i = 0; i < n; i++) pot[i] += hd_pot(o, rho[i]);
   if (o->model & ORC_OT_BACKFLOW) {
     if (o->max_veloc < 999.0 / ORC_AUTOMPS) {
       r_clamp(vx, n, o->max_veloc, -o->max_veloc, o->max_veloc, -o->max_veloc);
@@ -539,7 +557,7 @@ void orc_ot_potential2(orc_ot *o, double complex *pot, const double *rho, double
 
 /* ------------------------------------------------- energy density (a10) -- */
 
-/* src/ot-energy.c:38-110, 123-191, 202-288 (no HD, no thermal). */
+/* src/ot-energy.c:38-110, 123-191, 202-288 (HD penalty, no thermal). */
 void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, double wf_mass) {
   geom g = G(o);
   long n = g.n;
@@ -558,6 +576,8 @@ void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, do
   for (long i = 0; i < n; i++) eps[i] += (o->c2 / 2.0) * (w1[i] * w1[i] * rho[i]);
   #pragma omp parallel for
   for (long i = 0; i < n; i++) eps[i] += (o->c3 / 3.0) * (w1[i] * w1[i] * w1[i] * rho[i]);
+  if (has_hd(o))                               /* src/ot-energy.c:96-99 */
+    for (long i = 0; i < n; i++) eps[i] += hd_eps(o, rho[i]);
 
   if (o->model & ORC_OT_KC) {
     double *gr[3], *w[3];
@@ -639,6 +659,8 @@ void orc_ot_energy_gradient(orc_ot *o, double *grad, const double *rho) {
   r_fft(&g, rho, rho_tf);
   add_lj_local(o, &g, pot, rho, rho_tf, cw, w1, w2);
   for (long i = 0; i < n; i++) grad[i] = creal(pot[i]);
+  if (has_hd(o))
+    for (long i = 0; i < n; i++) grad[i] += hd_pot(o, rho[i]);
   if (o->model & ORC_OT_KC) {
     double c = o->alpha_s / (2.0 * o->mass);
     double *rho_st = rbuf(n), *gd = rbuf(n), *J = rbuf(n), *H = rbuf(n), *M = rbuf(n);

@@ -13,8 +13,12 @@
  *   src/ot-energy.c:38-288 dft_ot_energy_density (+ _kc, _bf)
  *   src/common.c:31-66,131-138,193-200, src/ot-private.h:11-22  kernel functions
  *   src/helium-ot-bulk.c:21-258, 379-463  bulk closed forms
+ *   src/ot.c:164-170, 374-378, 1004-1007, src/ot-energy.c:96-99  HD/HD2 penalty
  * and the libgrid primitives the path uses (libgrid is not vendored; its
  * semantics are fixed as stated next to each function in ot_oracle.c).
+ * The HD penalty's formula is libgrid's (grid_func4/5) and is restated from
+ * PRB 72, 214522 (2005): parity with libgrid itself is unpinned for it; the
+ * tests pin it to its own energy by finite differences.
  *
  * Layout: row-major [i][j][k], k fastest; x_i = (i - nx/2) * step.
  * All quantities in atomic units (hbar = 1).
@@ -25,9 +29,12 @@
 #include <stdint.h>
 #include <complex.h>
 
-/* Model flags: same values as src/ot.h:37-60 (only PLAIN/KC/BACKFLOW are on the path). */
+/* Model flags: same values as src/ot.h:37-60 (PLAIN/KC/BACKFLOW, and the
+   high-density penalty HD/HD2 without backflow). */
 #define ORC_OT_PLAIN    0
 #define ORC_OT_KC       1
+#define ORC_OT_HD       2
+#define ORC_OT_HD2      4
 #define ORC_OT_BACKFLOW 8
 
 /* Unit conversions: CODATA 2018 (libgrid's au.h, which the reference uses, is not available). */
@@ -54,6 +61,9 @@ typedef struct {
   double kc_coef23;
   /* full complex spectra (N entries), centred sampling: r = 0 at index n/2 */
   double complex *lj_tf, *sa_tf, *g_tf, *gx_tf, *gy_tf, *gz_tf, *bf_tf;
+  /* high-density penalty (src/ot.c:1004-1011; zero without HD/HD2) and the
+     spherical-average radius (h, or 1.065 h for HD2, src/ot.c:164-170) */
+  double hd_beta, hd_rhom, hd_C, sa_h;
 } orc_ot;
 
 /* Parameters (a1) and kernel spectra (a2). min_sub/max_sub as in dft_ot_alloc. */

@@ -18,7 +18,7 @@ AUTOAMU = 1.0 / 1822.888486209
 AUTOMPS = 2187691.26364
 AUTOBAR = 2.9421015697e8
 HELIUM_MASS = 4.002602 / AUTOAMU
-PLAIN, KC, BACKFLOW = 0, 1, 8
+PLAIN, KC, HD, HD2, BACKFLOW = 0, 1, 2, 4, 8
 
 
 class _OrcOT(C.Structure):
@@ -27,7 +27,8 @@ class _OrcOT(C.Structure):
                     "b c2 c2_exp c3 c3_exp c4 rho_0s alpha_s l_g mass rho0 temp "
                     "lj_h lj_sigma lj_epsilon lj_cval g11 g12 g21 g22 a1 a2 "
                     "div_epsilon max_veloc max_bfpot c_bfpot kc_coef23").split()] + \
-              [(n, C.c_void_p) for n in "lj_tf sa_tf g_tf gx_tf gy_tf gz_tf bf_tf".split()]
+              [(n, C.c_void_p) for n in "lj_tf sa_tf g_tf gx_tf gy_tf gz_tf bf_tf".split()] + \
+              [(n, C.c_double) for n in "hd_beta hd_rhom hd_C sa_h".split()]
 
 
 def lib():

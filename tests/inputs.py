@@ -43,6 +43,16 @@ def droplet(shape, step, natoms=1000.0, rho0_ang=0.0218, width_ang=1.0):
     return np.sqrt(rho).astype(np.complex128)
 
 
+def dense_bump(shape, step, rho0, peak_ang=0.4, width_ang=1.0):
+    """Bulk rho0 with a gaussian density bump rising to peak_ang (A^-3) at the
+    origin, zero phase: reaches the densities where the high-density penalty acts."""
+    x, y, z = np.meshgrid(*coords(shape, step), indexing="ij")
+    w = width_ang / AUTOANG
+    peak = peak_ang * AUTOANG ** 3
+    rho = rho0 + (peak - rho0) * np.exp(-(x * x + y * y + z * z) / (2.0 * w * w))
+    return np.sqrt(rho).astype(np.complex128)
+
+
 def vortex_line(shape, step, rho0, noise=1e-3, seed=SEED):
     """Bulk rho0 with phase atan2(y, x) along z and seeded density noise."""
     x, y, _ = np.meshgrid(*coords(shape, step), indexing="ij")

@@ -144,6 +144,56 @@ def test_kc_reference_potential_is_not_exact_derivative(oracle):
     assert abs(fd - an) > 1e-4 * abs(fd)
 
 
+@pytest.mark.parametrize("model", [2, 4], ids=["hd", "hd2"])
+def test_hd_potential_is_energy_derivative(oracle, model):
+    """With the high-density penalty on a density that reaches it, the PLAIN
+    potential is still the exact functional derivative of the energy."""
+    po = oracle
+    shape, step = (16, 16, 16), _ang(0.6, po)
+    o = po.OracleOT(model, shape, step, 2, 8)
+    psi = inputs.dense_bump(shape, step, o.p.rho0)
+    fd, an = _fd_check(o, psi, step)
+    assert abs(fd - an) < 1e-8 * abs(fd)
+    # the penalty is on: it dominates the potential at the bump
+    o0 = po.OracleOT(0, shape, step, 2, 8)
+    assert np.abs(o.potential(psi) - o0.potential(psi)).max() > 10 * np.abs(o0.potential(psi)).max()
+
+
+def test_hd_uniform_high_density(oracle):
+    """Uniform density above the penalty's onset: V = dE_bulk/drho + C(1 + tanh x)
+    + C beta rho (1 - tanh^2 x) and eps = E_bulk + C rho (1 + tanh x), x = beta (rho - rho_m),
+    with beta = 40 A^3, rho_m = 0.37 A^-3, C = 0.1 hartree (src/ot.c:1004-1007)."""
+    po = oracle
+    shape = (16, 16, 16)
+    o = po.OracleOT(2, shape, _ang(0.5, po), 2, 8)
+    a3 = po.AUTOANG ** 3
+    beta, rhom, C = 40.0 / a3, 0.37 * a3, 0.1
+    for rho_ang in (0.3, 0.37, 0.45):
+        ru = rho_ang * a3
+        x = beta * (ru - rhom)
+        psi = np.full(shape, np.sqrt(ru), np.complex128)
+        V = o.potential(psi)
+        want = o.bulk("dEdRho", ru) + C * (1 + np.tanh(x)) + C * beta * ru * (1 - np.tanh(x) ** 2)
+        assert np.abs(V - want).max() < 1e-12 * abs(want)
+        eps = o.energy_density(psi)
+        ew = o.bulk("energy", ru) + C * ru * (1 + np.tanh(x))
+        assert np.abs(eps - ew).max() < 1e-12 * abs(ew)
+
+
+def test_hd2_spherical_average_radius(oracle):
+    """HD2 samples the spherical average with radius 1.065 h (src/ot.c:164-170)."""
+    po = oracle
+    n, step = 32, _ang(0.4, po)
+    o = po.OracleOT(4, (n, n, n), step, 4, 32)
+    L = n * step
+    for m in (1, 2, 3):
+        k = 2 * np.pi * m / L
+        cell = np.sin(k * step / 2) / (k * step / 2)
+        pi_num = o.spectrum("sa_tf")[m, 0, 0].real * (-1.0) ** m * step ** 3
+        kh = k * o.p.lj_h * 1.065
+        assert abs(pi_num - 3 * (np.sin(kh) - kh * np.cos(kh)) / kh ** 3 * cell) < 1e-3
+
+
 # ------------------------------------------------- kernels vs closed forms --
 def test_kernel_spectra_closed_forms(oracle):
     """Spectra at small k against the continuum transforms
