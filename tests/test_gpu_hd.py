@@ -1,9 +1,15 @@
 """GPU parity of the high-density penalty (models HD = 2 and HD2 = 4, alone or
 with KC) against the FP64 oracle, on a density bump that reaches the
 penalty's onset (0.4 A^-3 at the centre): potential, density and energy
-density, the fused split-operator step, the energy gradient, and the
-transposed Jacobian of the potential (dot-product test against central
-differences).  HD together with BACKFLOW is rejected at allocation.
+density, the fused split-operator step, the energy gradient, the transposed
+Jacobian of the potential (dot-product test against central differences) and
+the adjoint sweep (against central differences of the final energy).  HD
+together with BACKFLOW is rejected at allocation.
+
+The FP32 fused real-time step lands near 1.4e-5 of the oracle here (bar
+2e-5): at 0.4 A^-3 the penalty is ~0.7 hartree and beta*rho ~ 16 amplifies
+the FP32 rounding of rho into ~1e-6 of V, which a 0.5 fs step turns into
+~1e-5 rad of phase.
 """
 import numpy as np
 import pytest
@@ -133,3 +139,15 @@ def test_hd_with_backflow_is_rejected():
     assert not L.lib().dft_ot_alloc(2 | 8, w.h, 2, 8)
     assert not L.lib().dft_ot_alloc(4 | 9, w.h, 2, 8)
     w.free()
+
+
+@pytest.mark.parametrize("model", [2, 3], ids=["hd", "kc-hd"])
+def test_hd_adjoint_well_matches_fd(model):
+    """The checkpointed adjoint sweep through the penalty (6 real-time steps of
+    0.5 fs on the density bump, checkpoints every 4): dE/d(well depth, width)
+    against central differences."""
+    from tests.test_gpu_adjoint import _fd_check
+    rho0 = 0.0218 * L.AUTOANG ** 3
+    psi0 = inputs.dense_bump(SHAPE, STEP, rho0)
+    amp, width = -50.0 / L.AUTOK, 2.0 / L.AUTOANG
+    _fd_check(model, SHAPE, STEP, psi0, amp, width, 0.0, 0.5 / L.AUTOFS, 6, 4, 1e-6)
