@@ -312,7 +312,8 @@ void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density)
 REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density);
 
 /* ---- adjoint (extensions; no counterpart in the reference) ---- */
-/* Reverse pass of dft_ot_potential (PLAIN and KC models) at the state gwf:
+/* Reverse pass of dft_ot_potential (PLAIN and KC models, with or without the
+   HD/HD2 penalty) at the state gwf:
    rho_bar = (dV_OT/drho)^T v_bar, the transpose of the potential's Jacobian
    applied to v_bar. */
 void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf);
@@ -322,7 +323,7 @@ void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, 
    r_i = (i - n/2) step - r0, with respect to (amp, width): grad[0..1].  An
    adjoint sweep that keeps every `every`-th state and recomputes each segment
    in the reverse pass (memory: nsteps/every + every states).  PLAIN and KC
-   models; returns 0, or -1 (message on stderr) for backflow models, invalid
+   models (with or without HD/HD2); returns 0, or -1 (message on stderr) for backflow models, invalid
    arguments or lack of device memory. */
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad);
@@ -333,7 +334,8 @@ int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, R
    W = (1 - tanh((r - 1.5 radius) / (4 width))) / 2 (the missing volume of the
    whole box is the conserved N / rho0), of the final state and its derivative
    with respect to the initial cavity radius, from the same
-   checkpointed adjoint sweep as dft_ot_adjoint_well.  PLAIN and KC models;
+   checkpointed adjoint sweep as dft_ot_adjoint_well.  PLAIN and KC models (with
+   or without HD/HD2);
    returns 0, or -1 (message on stderr) for backflow models, invalid arguments
    or lack of device memory. */
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
