@@ -502,6 +502,16 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), DFT_XCONV_MINB) k_x
 #ifndef DFT_ZBC_MINB
 #define DFT_ZBC_MINB 2
 #endif
+// the same requests for the FP64 instantiations (0: none).  FP64 zBC at 2
+// waves per SIMD spills 240 bytes per lane and still beats 1 wave (256 VGPRs +
+// 56 AGPRs): 192.5 -> 198 steps/s at 256^3; zAB at 2 waves (412 bytes of
+// spills) is slower.
+#ifndef DFT_ZAB_MINB64
+#define DFT_ZAB_MINB64 0
+#endif
+#ifndef DFT_ZBC_MINB64
+#define DFT_ZBC_MINB64 2
+#endif
 struct ZABArgs {
   Ptrs8 in;       // rhobar, [rhot], [A, Bx, By, Bz, C]
   Ptrs8 out;      // Q, [G_x, G_y, G_z]
@@ -510,7 +520,7 @@ struct ZABArgs {
 };
 
 template <typename T, int N, bool KC, bool BF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : 0) k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : DFT_ZAB_MINB64) k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   constexpr int IA = KC ? 2 : 1;  // first backflow input
   const void *rho = a.rho;
@@ -646,7 +656,7 @@ struct ZBCArgs {
 };
 
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : 0) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : DFT_ZBC_MINB64) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   const void *rho = a.rho, *rhot = a.rhot;
   C v[E];
