@@ -299,11 +299,17 @@ __global__ void __launch_bounds__(256)
 #ifndef DFT_XKIN_COLS
 #define DFT_XKIN_COLS 8
 #endif
+// waves per SIMD asked of the register allocator (0: no request): at 3 the
+// FP32 pass fits 168 VGPRs with 28 bytes per lane of spills and runs 70 -> 58 us
+// at 256^3 (178 VGPRs and 2 waves without); more waves do not fit its LDS.
+#ifndef DFT_XKIN_MINB
+#define DFT_XKIN_MINB 3
+#endif
 template <typename T, int N>
 using KinCfg = typename std::conditional<(N >= 512), ColCfg<T, N>, ColCfg<T, N, 16, DFT_XKIN_COLS>>::type;
 
 template <typename T, int N>
-__global__ void __launch_bounds__((KinCfg<T, N>::BS))
+__global__ void __launch_bounds__((KinCfg<T, N>::BS), DFT_XKIN_MINB)
     k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, double cx, double cy, double cz,
            double dtr, double dti) {
   using C = cplx<T>;

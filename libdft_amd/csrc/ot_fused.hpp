@@ -155,8 +155,16 @@ __device__ __forceinline__ void load_c2r(rf::Regs<T, N, pair_em<N>()> &v, cplx<T
 template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { return w ? c.y : c.x; }
 
 // -------------------------------------------------------------------- zA --
+// waves per SIMD asked of the register allocator for the FP32 zA and zC
+// kernels (0: no request)
+#ifndef DFT_ZA_MINB
+#define DFT_ZA_MINB 0
+#endif
+#ifndef DFT_ZC_MINB
+#define DFT_ZC_MINB 0
+#endif
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZA_MINB : 0)
     k_zA(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
          const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
@@ -730,7 +738,7 @@ struct ZCArgs {
 };
 
 template <typename T, int N, bool STEP>
-__global__ void __launch_bounds__(256) k_zC(ZCArgs a, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZCArgs a, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
   using C = cplx<T>;
   using G = LineCfg<T, N, pair_em<N>()>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
