@@ -403,12 +403,16 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
 #ifndef DFT_XCONV_MINB
 #define DFT_XCONV_MINB 4
 #endif
-// (N = 512 keeps 16 points per thread and 16 columns; FP64 takes at most 32
-// columns, which fill the LDS.)
+// (N = 512 keeps 16 points per thread and 16 columns.  FP64 single-term
+// outputs take DFT_XCONV1_COLS64 columns: 16 measured faster than 32, which
+// fill the LDS with one block per CU; the accumulating form takes at most 32.)
+#ifndef DFT_XCONV1_COLS64
+#define DFT_XCONV1_COLS64 16
+#endif
 template <typename T, int N, bool ONE>
 using XConvCfg = ColCfg<T, N, N >= 512 ? 16 : (ONE ? DFT_XCONV1_EM : DFT_XCONV_EM),
                         N >= 512 ? 16
-                                 : (ONE ? (sizeof(T) == 8 && DFT_XCONV1_COLS > 32 ? 32 : DFT_XCONV1_COLS)
+                                 : (ONE ? (sizeof(T) == 8 ? DFT_XCONV1_COLS64 : DFT_XCONV1_COLS)
                                         : (sizeof(T) == 8 && DFT_XCONV_COLS > 32 ? 32 : DFT_XCONV_COLS))>;
 
 template <typename T, int N, bool DERIV, bool ONE>
