@@ -1,2 +1,3 @@
-/* Drop-in header name of the reference build (<dft/ot.h>): everything is declared in libdft_mi355x.h. */
-#include "../libdft_mi355x.h"
+/* Drop-in header name of the reference build (<dft/ot.h>): everything is declared in libdft_mi355x.h;
+   <grid/grid.h> adds the libgrid names drivers use next to it. */
+#include "../grid/grid.h"

@@ -1,2 +1,3 @@
-/* Drop-in header name of the reference build (<grid/au.h>): everything is declared in libdft_mi355x.h. */
-#include "../libdft_mi355x.h"
+/* Drop-in header name of the reference build (<grid/au.h>): the unit conversions are declared in
+   libdft_mi355x.h; <grid/grid.h> adds libgrid's math names and HBAR. */
+#include "grid.h"

@@ -31,8 +31,8 @@
 extern "C" {
 #endif
 
-typedef double REAL;
-typedef int64_t INT;
+#define REAL double
+#define INT int64_t
 typedef double _Complex REAL_COMPLEX;
 #define FMT_R "%le"
 #define FMT_I "%ld"
@@ -55,6 +55,10 @@ struct rgrid_struct {
   int flag;            /* claim flag (rgrid_claim / rgrid_release) */
   int spectral;        /* 1: holds a real half spectrum [nx][ny][nz/2+1] */
   char id[32];
+  /* boundary function given to rgrid_alloc (value at an index outside the
+     grid; only periodic boundaries exist here) and its parameter pointer */
+  REAL (*value_outside)(struct rgrid_struct *, INT, INT, INT);
+  void *outside_params_ptr;
 };
 
 /* Complex grid handle (libgrid cgrid). */
@@ -66,8 +70,10 @@ struct cgrid_struct {
   INT dev_elems;
   int prec;
   int flag;
-  REAL kx0, ky0, kz0;  /* momentum frame (cgrid_set_momentum); only 0 is supported */
+  REAL kx0, ky0, kz0;  /* momentum frame (cgrid_set_momentum) */
   char id[32];
+  REAL_COMPLEX (*value_outside)(struct cgrid_struct *, INT, INT, INT);
+  void *outside_params_ptr;
 };
 
 #define WF_PERIODIC_BOUNDARY 1
@@ -177,14 +183,40 @@ void rgrid_gradient_x(rgrid *src, rgrid *dst);
 void rgrid_gradient_y(rgrid *src, rgrid *dst);
 void rgrid_gradient_z(rgrid *src, rgrid *dst);
 
+/* Further libgrid calls of the reference drivers (examples/vortex-ring/vring.c:63-71,149,
+   examples/gas-bubble/bubble.c:104).  cgrid_set_momentum sets the frame k0 of the
+   propagator and the kinetic energy, whose symbol becomes (k - k0)^2 / 2M. */
+void cgrid_set_momentum(cgrid *grid, REAL kx0, REAL ky0, REAL kz0);
+/* dst = Laplacian of src (2nd-order central difference, periodic) */
+void cgrid_fd_laplace(cgrid *src, cgrid *dst);
+/* dst = conj(a) * b */
+void cgrid_conjugate_product(cgrid *dst, cgrid *a, cgrid *b);
+/* dst = |src|^p */
+void cgrid_abs_power(cgrid *dst, cgrid *src, REAL p);
+void rgrid_abs_power(rgrid *dst, rgrid *src, REAL p);
+/* rot = |curl (fx, fy, fz)| (central differences, periodic) */
+void rgrid_abs_rot(rgrid *rot, rgrid *fx, rgrid *fy, rgrid *fz);
+/* j_d = (hbar / M) Im(psi* d_d psi), central differences; any output may be NULL */
+void grid_wf_probability_flux(wf *gwf, rgrid *jx, rgrid *jy, rgrid *jz);
+/* h^3 sum f(x, y, z) grid (f evaluated on the host) */
+REAL rgrid_weighted_integral(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg);
+
 /* Reductions: FP64 accumulation with a compensated final sum. */
 REAL rgrid_integral(rgrid *grid);
 REAL rgrid_integral_of_product(rgrid *a, rgrid *b);
+/* h^3 sum over the points with x in [xl, xu], y in [yl, yu], z in [zl, zu]
+   (x_i = (i - nx/2) step - x0; bounds inclusive) */
+REAL rgrid_integral_region(rgrid *grid, REAL xl, REAL xu, REAL yl, REAL yu, REAL zl, REAL zu);
+REAL_COMPLEX cgrid_integral_region(cgrid *grid, REAL xl, REAL xu, REAL yl, REAL yu, REAL zl, REAL zu);
+REAL_COMPLEX cgrid_integral(cgrid *grid);
+/* h^3 sum conj(a) b */
+REAL_COMPLEX cgrid_integral_of_conjugate_product(cgrid *a, cgrid *b);
 /* h^3 sum |psi|^2 Re V */
 REAL grid_wf_potential_energy(wf *gwf, cgrid *potential);
 REAL grid_wf_norm(wf *gwf);
 void grid_wf_normalize(wf *gwf);
-/* Kinetic energy h^3/N sum_k (k^2/2M)|psi_k|^2 (the potential argument must be NULL). */
+/* Kinetic energy h^3/N sum_k ((k - k0)^2/2M)|psi_k|^2, plus h^3 sum |psi|^2 Re V
+   when potential is not NULL. */
 REAL grid_wf_energy(wf *gwf, cgrid *potential);
 
 /* ------------------------------------------------ split-operator stepping -- */
@@ -315,8 +347,9 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density);
 /* Reverse pass of dft_ot_potential (PLAIN and KC models, with or without the
    HD/HD2 penalty) at the state gwf:
    rho_bar = (dV_OT/drho)^T v_bar, the transpose of the potential's Jacobian
-   applied to v_bar. */
-void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf);
+   applied to v_bar.  Returns 0, or -1 (message on stderr) for a backflow
+   model, mismatched grids or lack of device memory. */
+int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf);
 /* Gradient of the total energy E[psi_K] = E_kin + E_OT + h^3 sum rho V_ext
    after nsteps steps of dft_ot_propagate(otf, ., V_ext, mu, time) from gwf
    (unchanged), with V_ext(r) = amp exp(-|r - r0|^2 / (2 width^2)) on the grid
@@ -363,6 +396,12 @@ void grid_bench_copy(rgrid *dst, rgrid *src);
 #define GRID_AUTOAMU (1.0 / 1822.888486209)
 #define GRID_AUTOMPS 2187691.26364
 #define GRID_AUTOBAR 2.9421015697E8
+#define GRID_AUTOPA 2.9421015697E13
+#define GRID_AUTOCM1 219474.6313632
+#define GRID_AUTOEV 27.211386245988
+#define GRID_AUTOM 5.29177210903E-11
+#define GRID_AUTOJ 4.3597447222071E-18
+#define GRID_AUTOKG 9.1093837015E-31
 
 #ifdef __cplusplus
 }

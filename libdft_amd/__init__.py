@@ -50,14 +50,15 @@ class RGrid(C.Structure):
     _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64), ("step", C.c_double),
                 ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double), ("dev", C.c_void_p),
                 ("dev_elems", C.c_int64), ("prec", C.c_int), ("flag", C.c_int), ("spectral", C.c_int),
-                ("id", C.c_char * 32)]
+                ("id", C.c_char * 32), ("value_outside", C.c_void_p), ("outside_params_ptr", C.c_void_p)]
 
 
 class CGrid(C.Structure):
     _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64), ("step", C.c_double),
                 ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double), ("dev", C.c_void_p),
                 ("dev_elems", C.c_int64), ("prec", C.c_int), ("flag", C.c_int), ("kx0", C.c_double),
-                ("ky0", C.c_double), ("kz0", C.c_double), ("id", C.c_char * 32)]
+                ("ky0", C.c_double), ("kz0", C.c_double), ("id", C.c_char * 32), ("value_outside", C.c_void_p),
+                ("outside_params_ptr", C.c_void_p)]
 
 
 class WF(C.Structure):
@@ -120,7 +121,20 @@ _SIGS = {
     "grid_bench_copy": (None, ["rg", "rg"]),
     "dft_ot_propagate": (None, ["otf", "wf", "rg", "d", "z"]),
     "dft_ot_propagate_pc": (None, ["otf", "wf", "wf", "rg", "d", "z"]),
-    "dft_ot_potential_vjp": (None, ["otf", "rg", "rg", "wf"]),
+    "dft_ot_potential_vjp": ("i", ["otf", "rg", "rg", "wf"]),
+    "rgrid_value_at_index": ("d", ["rg", "i64", "i64", "i64"]),
+    "cgrid_value_at_index": ("z", ["cg", "i64", "i64", "i64"]),
+    "cgrid_set_momentum": (None, ["cg", "d", "d", "d"]),
+    "cgrid_fd_laplace": (None, ["cg", "cg"]),
+    "cgrid_conjugate_product": (None, ["cg", "cg", "cg"]),
+    "cgrid_abs_power": (None, ["cg", "cg", "d"]),
+    "rgrid_integral_region": ("d", ["rg", "d", "d", "d", "d", "d", "d"]),
+    "cgrid_integral_region": ("z", ["cg", "d", "d", "d", "d", "d", "d"]),
+    "cgrid_integral": ("z", ["cg"]),
+    "cgrid_integral_of_conjugate_product": ("z", ["cg", "cg"]),
+    "rgrid_abs_power": (None, ["rg", "rg", "d"]),
+    "rgrid_abs_rot": (None, ["rg", "rg", "rg", "rg"]),
+    "grid_wf_probability_flux": (None, ["wf", "rg", "rg", "rg"]),
     "dft_ot_adjoint_well": ("i", ["otf", "wf", "d", "d", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_adjoint_cavity": ("i", ["otf", "wf", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_dispersion": ("d", ["wf", "otf", "d", "p", "d", "b", "b"]),

@@ -151,13 +151,16 @@ void launch_zc2c(const Plan3D<T> &p, const cplx<T> *src, cplx<T> *dst, const cpl
 
 template <typename T, int N>
 void launch_xkin(const Plan3D<T> &p, cplx<T> *psi, double cx, double cy, double cz, double dtr, double dti,
-                 hipStream_t s) {
+                 const double *k0, hipStream_t s) {
+  // momentum frame: (k - k0)^2, k0 in units of each axis' 2 pi / L
+  const double ox = k0 ? k0[0] * p.nx * p.step / (2.0 * M_PI) : 0.0, oy = k0 ? k0[1] * p.ny * p.step / (2.0 * M_PI) : 0.0,
+               oz = k0 ? k0[2] * p.nz * p.step / (2.0 * M_PI) : 0.0;
   using G = fft::KinCfg<T, N>;
   const int ntile = (p.nz + G::COLS - 1) / G::COLS;
   dim3 grid((unsigned)(p.ny * ntile));
   DFT_PROF_BEGIN(s);
   hipLaunchKernelGGL((fft::k_xkin<T, N>), grid, dim3(G::BS), 0, s, psi, p.ny, p.nz, ntile, p.twx, cx, cy, cz,
-                     dtr, dti);
+                     dtr, dti, ox, oy, oz);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "fft_x_kinetic_c2c", 2.0 * (double)p.n() * sizeof(cplx<T>));
 }
@@ -235,7 +238,7 @@ void c2r3d(const Plan3D<T> &p, int nin, const cplx<T> *const *in, int nout, cons
 
 template <typename T>
 void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, double dti, double mass,
-            double *partial, hipStream_t s) {
+            double *partial, hipStream_t s, const double *k0) {
   fft::Batch by{};
   by.in[0] = psi;
   by.out[0] = psi;
@@ -244,7 +247,7 @@ void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, do
   const double cx = L2x / (2.0 * mass), cy = L2y / (2.0 * mass), cz = L2z / (2.0 * mass);
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, -1>(p, psi, psi, pot, dtr, dti, (T)1, nullptr, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
-  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, s)));
+  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, k0, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
   DFT_DISPATCH_N(p.nz, (launch_zc2c<T, N_, +1>(p, psi, psi, pot, dtr, dti, inv, partial, s)));
@@ -266,7 +269,7 @@ void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, 
 
 template <typename T>
 void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double dtr, double dti, double mass,
-                 hipStream_t s) {
+                 hipStream_t s, const double *k0) {
   fft::Batch by{};
   by.in[0] = psi;
   by.out[0] = psi;
@@ -274,7 +277,7 @@ void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double 
                L2z = std::pow(2.0 * M_PI / (p.nz * p.step), 2);
   const double cx = L2x / (2.0 * mass), cy = L2y / (2.0 * mass), cz = L2z / (2.0 * mass);
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, -1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
-  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, s)));
+  DFT_DISPATCH_N(p.nx, (launch_xkin<T, N_>(p, psi, cx, cy, cz, dtr, dti, k0, s)));
   DFT_DISPATCH_N(p.ny, (launch_strided<T, N_, +1>(p, by, 1, 1, p.nz, p.nz, (T)1, s)));
   const T inv = (T)(1.0 / (double)p.n());
   DFT_DISPATCH_N(p.nz, (launch_zc2c_phase_inv<T, N_>(p, psi, phase, dtr, dti, inv, s)));
@@ -308,12 +311,12 @@ template <typename T> void c2c_inv(const Plan3D<T> &p, const cplx<T> *in, cplx<T
   template void c2r3d<T>(const Plan3D<T> &, int, const cplx<T> *const *, int, const fft::Combine *,          \
                          const fft::KernTab &, cplx<T> *const *, T *const *, hipStream_t);                   \
   template void strang<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double, double *,   \
-                          hipStream_t);                                                                      \
+                          hipStream_t, const double *);                                                      \
   template void c2c_fwd<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
   template void c2c_inv<T>(const Plan3D<T> &, const cplx<T> *, cplx<T> *, hipStream_t);                    \
   template void ypass<T>(const Plan3D<T> &, int, cplx<T> *const *, int, int, int, hipStream_t);                  \
   template void strang_tail<T>(const Plan3D<T> &, cplx<T> *, const cplx<T> *, double, double, double,        \
-                               hipStream_t);
+                               hipStream_t, const double *);
 DFT_INST(float)
 DFT_INST(double)
 

@@ -311,7 +311,7 @@ using KinCfg = typename std::conditional<(N >= 512), ColCfg<T, N>, ColCfg<T, N, 
 template <typename T, int N>
 __global__ void __launch_bounds__((KinCfg<T, N>::BS), DFT_XKIN_MINB)
     k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, double cx, double cy, double cz,
-           double dtr, double dti) {
+           double dtr, double dti, double ox, double oy, double oz) {
   using C = cplx<T>;
   using G = KinCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
@@ -331,8 +331,8 @@ __global__ void __launch_bounds__((KinCfg<T, N>::BS), DFT_XKIN_MINB)
   for (int i = 0; i < E; ++i) v[i] = gld<C>((const char *)psi + (size_t)i * st, ob);
   // exp(-i K dt) = exp(K dti) (cos(K dtr) - i sin(K dtr)) for the x part
   for (int e = threadIdx.x; e < N; e += G::BS) {
-    const int mx = e < N / 2 ? e : e - N;
-    const double K = cx * (double)(mx * mx);
+    const double mx = (double)(e < N / 2 ? e : e - N) - ox;
+    const double K = cx * (mx * mx);
     double s, c;
     sincos(K * dtr, &s, &c);
     const double m = exp(K * dti);
@@ -340,9 +340,9 @@ __global__ void __launch_bounds__((KinCfg<T, N>::BS), DFT_XKIN_MINB)
   }
   __syncthreads();
   rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, s_x + col * LP, s_tw, t);
-  const int my = o < ny / 2 ? o : o - ny;
-  const int mz = cc < nz / 2 ? cc : cc - nz;
-  const double Kyz = cy * (double)(my * my) + cz * (double)(mz * mz);
+  const double my = (double)(o < ny / 2 ? o : o - ny) - oy;
+  const double mz = (double)(cc < nz / 2 ? cc : cc - nz) - oz;
+  const double Kyz = cy * (my * my) + cz * (mz * mz);
   double sy, cyz;
   sincos(Kyz * dtr, &sy, &cyz);
   const double myz = exp(Kyz * dti);

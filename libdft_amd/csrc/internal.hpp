@@ -35,9 +35,10 @@ template <typename T>
 void c2r3d(const Plan3D<T> &p, int nin, const cplx<T> *const *in, int nout, const fft::Combine *cmb,
            const fft::KernTab &kt, cplx<T> *const *tmp, T *const *out, hipStream_t s);
 // One Strang step on psi; partial (may be null) receives per-block sums of |psi|^2.
+// k0 (may be null): momentum frame (kx0, ky0, kz0), kinetic symbol (k - k0)^2 / 2M.
 template <typename T>
 void strang(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *pot, double dtr, double dti, double mass,
-            double *partial, hipStream_t s);
+            double *partial, hipStream_t s, const double *k0 = nullptr);
 int64_t strang_partials(int nx, int ny, int nz, int prec);
 // y pass (dir -1 forward, +1 inverse) over nb fields of `cols` columns with
 // row pitch `pitch`, in place.
@@ -48,7 +49,7 @@ void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, 
 // potential the first half step used).
 template <typename T>
 void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double dtr, double dti, double mass,
-                 hipStream_t s);
+                 hipStream_t s, const double *k0 = nullptr);
 // Full complex forward transform (out of place).
 template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s);
 // Its inverse, normalised by 1/N (out of place).
@@ -62,9 +63,10 @@ template <typename T> void reduce_sum(const T *a, const T *b, int64_t n, double 
 template <typename T> void reduce_abs2(const cplx<T> *a, int64_t n, double *partial, hipStream_t s);
 template <typename T>
 void reduce_density_re(const cplx<T> *a, const cplx<T> *v, int64_t n, double *partial, hipStream_t s);
+// K = cx (mx - ox)^2 + cy (my - oy)^2 + cz (mz - oz)^2 (o: momentum frame in units of 2 pi / L)
 template <typename T>
 void reduce_kinetic(const cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz, double *partial,
-                    hipStream_t s);
+                    hipStream_t s, const double *o = nullptr);
 
 // ------------------------------------------------------------- elementwise --
 template <typename T> void ew_density(const cplx<T> *psi, T *rho, int64_t n, hipStream_t s);

@@ -5,7 +5,8 @@
 //   int64    nx, ny, nz
 //   double   step, x0, y0, z0
 //   data     FP64 row-major [i][j][k], k fastest; complex as (re, im); a half
-//            spectrum as [nx][ny][nz/2+1] complex
+//            spectrum (a functional's kernel grid: the kernels are even, so
+//            their spectra are real) as [nx][ny][nz/2+1] real values
 // Values are written in FP64 whatever the device precision of the grid and
 // rounded to it on reading.
 #include <cstdint>
@@ -52,7 +53,7 @@ bool same_shape(const Header &h, INT nx, INT ny, INT nz, const char *who) {
 extern "C" {
 
 void rgrid_write(rgrid *grid, FILE *fp) {
-  const int64_t n = grid->spectral ? 2 * half_points(grid->nx, grid->ny, grid->nz) : points(grid->nx, grid->ny, grid->nz);
+  const int64_t n = grid->spectral ? half_points(grid->nx, grid->ny, grid->nz) : points(grid->nx, grid->ny, grid->nz);
   std::vector<REAL> v(n);
   rgrid_copy_to_host(grid, v.data());
   Header h;
@@ -73,20 +74,25 @@ rgrid *rgrid_read(rgrid *grid, FILE *fp) {
   }
   const bool own = grid == nullptr;
   if (own) {
+    // a new grid has room for a real field or a half spectrum
     grid = rgrid_alloc(h.nx, h.ny, h.nz, h.step, nullptr, nullptr, (char *)"rgrid_read");
     if (!grid) return nullptr;
     rgrid_set_origin(grid, h.x0, h.y0, h.z0);
+    grid->spectral = h.kind == kHalf;
   } else if (!same_shape(h, grid->nx, grid->ny, grid->nz, "rgrid_read")) {
     return nullptr;
+  } else if ((h.kind == kHalf) != (grid->spectral != 0)) {
+    fprintf(stderr, "libdft: rgrid_read: the file holds a %s, grid %s a %s\n",
+            h.kind == kHalf ? "half spectrum" : "real field", grid->id, grid->spectral ? "half spectrum" : "real field");
+    return nullptr;
   }
-  const int64_t n = h.kind == kHalf ? 2 * half_points(h.nx, h.ny, h.nz) : points(h.nx, h.ny, h.nz);
+  const int64_t n = h.kind == kHalf ? half_points(h.nx, h.ny, h.nz) : points(h.nx, h.ny, h.nz);
   std::vector<REAL> v(n);
   if (fread(v.data(), sizeof(REAL), n, fp) != (size_t)n) {
     fprintf(stderr, "libdft: rgrid_read: file too short\n");
     if (own) rgrid_free(grid);
     return nullptr;
   }
-  grid->spectral = h.kind == kHalf;
   rgrid_copy_from_host(grid, v.data());
   return grid;
 }

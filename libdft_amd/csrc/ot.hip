@@ -637,7 +637,7 @@ struct dft_ot_device {
   void *plan;                // Plan3D<float> or Plan3D<double>
   void *slot[NSLOT];
   void *kern[4];             // real half spectra (LJ, SA, F, U)
-  void *phase;               // complex: V_eff of the current step, whose e^{-iV dt/2} both z passes apply (allocated on first step)
+  void *phase;               // complex: V_eff of the current step, whose e^{-iV dt/2} both z passes apply
   void *vsave;               // complex: predictor potential V_n of a predict-correct step
   // x-middle programs of the fused schedule in device memory, and the host
   // copies they were uploaded from (re-uploaded only when one changes)
@@ -1019,13 +1019,6 @@ void launch_zC(bool step, const fused::ZCArgs &a, int64_t npairs, int P, const c
 
 // Device copy of x-middle program k (uploaded when it differs from the last one).
 const fused::MidProg *mid_prog(dft_ot_device *d, int k, const fused::MidProg &pg, hipStream_t s) {
-  if (!d->mid_dev) {
-    d->mid_dev = (fused::MidProg *)dmalloc(3 * sizeof(fused::MidProg));
-    if (!d->mid_dev) {
-      fprintf(stderr, "libdft: out of device memory for the x-middle programs\n");
-      abort();
-    }
-  }
   if (!d->mid_valid[k] || memcmp(&d->mid_host[k], &pg, sizeof pg) != 0) {
     DFT_HIP_CHECK(hipStreamSynchronize(s));
     memcpy(&d->mid_host[k], &pg, sizeof pg);
@@ -1233,16 +1226,6 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   }
 }
 
-void ensure_step_buffers(dft_ot_device *d) {
-  const size_t bytes = (d->prec == GRID_FP32 ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)d->n;
-  if (!d->phase) d->phase = dmalloc(bytes);
-  if (!d->vsave) d->vsave = dmalloc(bytes);
-  if (!d->phase || !d->vsave) {
-    fprintf(stderr, "libdft: out of device memory for the propagation buffers\n");
-    abort();
-  }
-}
-
 // Energy density: eps = (acc ? eps : 0) + selected parts.  rho holds the
 // density on entry when psi == nullptr (the _kc / _bf entry points).
 template <typename T>
@@ -1258,7 +1241,7 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
   T *vx = S<T>(d, VX), *vy = S<T>(d, VY), *vz = S<T>(d, VZ);
   if (BF && !psi) {
     fprintf(stderr, "libdft: backflow energy needs the wave function\n");
-    abort();
+    return;
   }
   if (!rho_given) {
     // density (and velocities) from psi
@@ -1496,29 +1479,45 @@ bool supported_model(INT model) {
   return !((model & (DFT_OT_HD | DFT_OT_HD2)) && (model & DFT_OT_BACKFLOW));
 }
 
-void check_wf(dft_ot_functional *otf, wf *gwf, const char *fn) {
+// Argument checks of the entry points: false (with a libdft: message) when
+// a handle is missing or its shape or precision differs from the functional's.
+bool check_otf(dft_ot_functional *otf, const char *fn) {
+  if (!otf || !otf->device) {
+    fprintf(stderr, "libdft: %s: no functional\n", fn);
+    return false;
+  }
+  return true;
+}
+
+bool check_wf(dft_ot_functional *otf, wf *gwf, const char *fn) {
+  if (!check_otf(otf, fn)) return false;
   dft_ot_device *d = otf->device;
-  cgrid *g = gwf->grid;
-  if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
+  cgrid *g = gwf ? gwf->grid : nullptr;
+  if (!g || g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
     fprintf(stderr, "libdft: %s: wave function does not match the functional's grid\n", fn);
-    abort();
+    return false;
   }
+  return true;
 }
 
-void check_c(dft_ot_functional *otf, cgrid *g, const char *fn) {
+bool check_c(dft_ot_functional *otf, cgrid *g, const char *fn) {
+  if (!check_otf(otf, fn)) return false;
   dft_ot_device *d = otf->device;
-  if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
+  if (!g || g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
     fprintf(stderr, "libdft: %s: grid does not match the functional's grid\n", fn);
-    abort();
+    return false;
   }
+  return true;
 }
 
-void check_r(dft_ot_functional *otf, rgrid *g, const char *fn) {
+bool check_r(dft_ot_functional *otf, rgrid *g, const char *fn) {
+  if (!check_otf(otf, fn)) return false;
   dft_ot_device *d = otf->device;
-  if (g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec) {
+  if (!g || g->nx != d->nx || g->ny != d->ny || g->nz != d->nz || g->prec != d->prec || g->spectral) {
     fprintf(stderr, "libdft: %s: grid does not match the functional's grid\n", fn);
-    abort();
+    return false;
   }
+  return true;
 }
 
 
@@ -1906,19 +1905,20 @@ int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0,
 }
 
 // Standalone reverse pass of V_OT at the density of gwf (for tests): rho_bar = (dV/drho)^T v_bar.
-template <typename T> void potential_vjp_impl(dft_ot_functional *otf, T *rho_bar, const T *v_bar, wf *gwf) {
+template <typename T> bool potential_vjp_impl(dft_ot_functional *otf, T *rho_bar, const T *v_bar, wf *gwf) {
   dft_ot_device *d = otf->device;
   AdjWork<T> w;
   if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->PP)) {
     fprintf(stderr, "libdft: dft_ot_potential_vjp: out of device memory\n");
     w.release();
-    return;
+    return false;
   }
   Well none{0.0, 1.0, 0.0, 0.0, 0.0, d->step, d->nx, d->ny, d->nz};
   adj_potential<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, none, 0.0, w);
   vjp_ot<T>(otf, v_bar, w.r[ARHO], nullptr, nullptr, rho_bar, w);
   DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
   w.release();
+  return true;
 }
 
 }  // namespace
@@ -1979,6 +1979,10 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
             (long)model);
     return nullptr;
   }
+  if (!gwf || !gwf->grid) {
+    fprintf(stderr, "libdft: dft_ot_alloc: no wave function\n");
+    return nullptr;
+  }
   cgrid *g = gwf->grid;
   if (g->nx < 4 || g->ny < 4 || g->nz < 4) {
     fprintf(stderr, "libdft: dft_ot_alloc: 1-D and 2-D grids are not supported\n");
@@ -2016,6 +2020,11 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
   bool ok = true;
   for (int i = 0; i < NSLOT && ok; ++i) ok = (d->slot[i] = dmalloc(es * d->slot_elems)) != nullptr;
   for (int i = 0; i < 4 && ok; ++i) ok = (d->kern[i] = dmalloc(es * nh)) != nullptr;
+  // the step's potential and the predictor's potential (complex fields), and
+  // the x-middle programs of the fused schedule
+  if (ok) ok = (d->phase = dmalloc(2 * es * d->n)) != nullptr;
+  if (ok) ok = (d->vsave = dmalloc(2 * es * d->n)) != nullptr;
+  if (ok) ok = (d->mid_dev = (fused::MidProg *)dmalloc(3 * sizeof(fused::MidProg))) != nullptr;
   if (ok) {
     if (d->prec == GRID_FP32) {
       auto *p = new Plan3D<float>();
@@ -2046,6 +2055,7 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
   grid_set_precision(d->prec);
   char *nm = (char *)"OT Density";
   otf->density = rgrid_alloc(g->nx, g->ny, g->nz, g->step, RGRID_PERIODIC_BOUNDARY, nullptr, nm);
+  ok = otf->density != nullptr;
   rgrid **ws[9] = {&otf->workspace1, &otf->workspace2, &otf->workspace3, &otf->workspace4, &otf->workspace5,
                    &otf->workspace6, &otf->workspace7, &otf->workspace8, &otf->workspace9};
   int nws = 3;
@@ -2055,8 +2065,14 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
     char id[32];
     snprintf(id, sizeof(id), "OT Workspace %d", i + 1);
     *ws[i] = rgrid_alloc(g->nx, g->ny, g->nz, g->step, RGRID_PERIODIC_BOUNDARY, nullptr, id);
+    ok = ok && *ws[i] != nullptr;
   }
   grid_set_precision(save);
+  if (!ok) {
+    fprintf(stderr, "libdft: Error in dft_ot_alloc(): device allocation failed.\n");
+    dft_ot_free(otf);
+    return nullptr;
+  }
   return otf;
 }
 
@@ -2076,10 +2092,7 @@ void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact) { otf->kc_coef23 = ex
 
 // src/ot.c:314-408
 void dft_ot_potential(dft_ot_functional *otf, cgrid *potential, wf *gwf) {
-  check_wf(otf, gwf, "dft_ot_potential");
-  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
-  (void)parts;
-  check_c(otf, potential, "dft_ot_potential");
+  if (!check_wf(otf, gwf, "dft_ot_potential") || !check_c(otf, potential, "dft_ot_potential")) return;
   DFT_BY_PREC(otf->device->prec, fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass,
                                                (T *)otf->density->dev, (cplx<T> *)potential->dev, nullptr));
 }
@@ -2089,16 +2102,15 @@ void dft_ot_potential(dft_ot_functional *otf, cgrid *potential, wf *gwf) {
 // dft_ot_potential(otf, pot, gwf); cgrid_add(pot, -mu); grid_wf_propagate(gwf, pot, time)
 // (single-step driver pattern, doc/libdft-manual.tex:496-504).
 void dft_ot_propagate(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REAL_COMPLEX time) {
-  check_wf(otf, gwf, "dft_ot_propagate");
-  if (vext) check_r(otf, vext, "dft_ot_propagate");
+  if (!check_wf(otf, gwf, "dft_ot_propagate") || (vext && !check_r(otf, vext, "dft_ot_propagate"))) return;
   dft_ot_device *d = otf->device;
-  ensure_step_buffers(d);
   StepIn st{vext ? vext->dev : nullptr, mu, nullptr, nullptr, d->phase, gwf->grid->dev, gwf->grid->dev,
             __real__ time, __imag__ time};
+  const double k0[3] = {gwf->grid->kx0, gwf->grid->ky0, gwf->grid->kz0};
   DFT_BY_PREC(d->prec, {
     fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &st);
     strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwf->grid->dev, (const cplx<T> *)d->phase, __real__ time,
-                   __imag__ time, gwf->mass, lib_stream());
+                   __imag__ time, gwf->mass, lib_stream(), k0);
   });
 }
 
@@ -2107,30 +2119,31 @@ void dft_ot_propagate(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REA
 //   V_p = V_OT[psi_p] + V_ext - mu;  psi_n+1 = Step(psi_n; (V_n + V_p)/2)
 // gwfp receives psi_p.
 void dft_ot_propagate_pc(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu, REAL_COMPLEX time) {
-  check_wf(otf, gwf, "dft_ot_propagate_pc");
-  check_wf(otf, gwfp, "dft_ot_propagate_pc");
-  if (vext) check_r(otf, vext, "dft_ot_propagate_pc");
+  if (!check_wf(otf, gwf, "dft_ot_propagate_pc") || !check_wf(otf, gwfp, "dft_ot_propagate_pc") ||
+      (vext && !check_r(otf, vext, "dft_ot_propagate_pc")))
+    return;
   dft_ot_device *d = otf->device;
-  ensure_step_buffers(d);
   const void *ve = vext ? vext->dev : nullptr;
   StepIn pred{ve, mu, nullptr, d->vsave, d->phase, gwf->grid->dev, gwfp->grid->dev, __real__ time, __imag__ time};
   StepIn corr{ve, mu, d->vsave, nullptr, d->phase, gwf->grid->dev, gwf->grid->dev, __real__ time, __imag__ time};
+  const double k0[3] = {gwf->grid->kx0, gwf->grid->ky0, gwf->grid->kz0};
   DFT_BY_PREC(d->prec, {
     fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &pred);
     strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwfp->grid->dev, (const cplx<T> *)d->phase, __real__ time,
-                   __imag__ time, gwf->mass, lib_stream());
+                   __imag__ time, gwf->mass, lib_stream(), k0);
     fused_impl<T>(otf, (const cplx<T> *)gwfp->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &corr);
     strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwf->grid->dev, (const cplx<T> *)d->phase, __real__ time,
-                   __imag__ time, gwf->mass, lib_stream());
+                   __imag__ time, gwf->mass, lib_stream(), k0);
   });
 }
 
 // src/ot.c:431-510
 void dft_ot_potential2(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *vx, rgrid *vy, rgrid *vz) {
-  check_r(otf, density, "dft_ot_potential2");
+  if (!check_c(otf, potential, "dft_ot_potential2") || !check_r(otf, density, "dft_ot_potential2")) return;
   PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
   if (parts.bf)
-    for (rgrid *v : {vx, vy, vz}) check_r(otf, v, "dft_ot_potential2");
+    for (rgrid *v : {vx, vy, vz})
+      if (!check_r(otf, v, "dft_ot_potential2")) return;
   DFT_BY_PREC(otf->device->prec,
               potential_impl<T>(otf, potential, nullptr, 0.0, (T *)density->dev, parts.bf ? (T *)vx->dev : nullptr,
                                 parts.bf ? (T *)vy->dev : nullptr, parts.bf ? (T *)vz->dev : nullptr, parts, true));
@@ -2139,11 +2152,14 @@ void dft_ot_potential2(dft_ot_functional *otf, cgrid *potential, rgrid *density,
 // src/ot.c:840-991 (the workspace arguments are accepted and unused)
 void dft_ot_backflow_potential(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *vx, rgrid *vy,
                                rgrid *vz, rgrid *, rgrid *, rgrid *, rgrid *, rgrid *, rgrid *) {
+  if (!check_otf(otf, "dft_ot_backflow_potential")) return;
   if (!(otf->model & DFT_OT_BACKFLOW)) {
-    fprintf(stderr, "libdft: dft_ot_backflow_potential: functional allocated without DFT_OT_BACKFLOW\n");
-    abort();
+    fprintf(stderr, "libdft: dft_ot_backflow_potential: functional allocated without DFT_OT_BACKFLOW (no kernel U)\n");
+    return;
   }
-  for (rgrid *v : {density, vx, vy, vz}) check_r(otf, v, "dft_ot_backflow_potential");
+  if (!check_c(otf, potential, "dft_ot_backflow_potential")) return;
+  for (rgrid *v : {density, vx, vy, vz})
+    if (!check_r(otf, v, "dft_ot_backflow_potential")) return;
   PotParts parts{false, false, true};
   DFT_BY_PREC(otf->device->prec, potential_impl<T>(otf, potential, nullptr, 0.0, (T *)density->dev, (T *)vx->dev,
                                                    (T *)vy->dev, (T *)vz->dev, parts, true));
@@ -2151,8 +2167,7 @@ void dft_ot_backflow_potential(dft_ot_functional *otf, cgrid *potential, rgrid *
 
 // src/ot-energy.c:38-110
 void dft_ot_energy_density(dft_ot_functional *otf, rgrid *energy_density, wf *gwf) {
-  check_wf(otf, gwf, "dft_ot_energy_density");
-  check_r(otf, energy_density, "dft_ot_energy_density");
+  if (!check_wf(otf, gwf, "dft_ot_energy_density") || !check_r(otf, energy_density, "dft_ot_energy_density")) return;
   PotParts parts{true, (otf->model & DFT_OT_KC) != 0, (otf->model & DFT_OT_BACKFLOW) != 0};
   DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)energy_density->dev, (const cplx<T> *)gwf->grid->dev,
                                                 gwf->mass, (T *)otf->density->dev, false, parts, false));
@@ -2160,8 +2175,12 @@ void dft_ot_energy_density(dft_ot_functional *otf, rgrid *energy_density, wf *gw
 
 // src/ot-energy.c:123-191 (adds the KC part)
 void dft_ot_energy_density_kc(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density) {
-  check_r(otf, energy_density, "dft_ot_energy_density_kc");
-  check_r(otf, density, "dft_ot_energy_density_kc");
+  if (!check_r(otf, energy_density, "dft_ot_energy_density_kc") || !check_r(otf, density, "dft_ot_energy_density_kc"))
+    return;
+  if (!(otf->model & DFT_OT_KC)) {
+    fprintf(stderr, "libdft: dft_ot_energy_density_kc: functional allocated without DFT_OT_KC (no kernel F)\n");
+    return;
+  }
   PotParts parts{false, true, false};
   DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)energy_density->dev, nullptr, gwf ? gwf->mass : 0.0,
                                                 (T *)density->dev, true, parts, true));
@@ -2169,27 +2188,31 @@ void dft_ot_energy_density_kc(dft_ot_functional *otf, rgrid *energy_density, wf 
 
 // src/ot-energy.c:202-288 (adds the BF part)
 void dft_ot_energy_density_bf(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density) {
-  check_wf(otf, gwf, "dft_ot_energy_density_bf");
-  check_r(otf, energy_density, "dft_ot_energy_density_bf");
-  check_r(otf, density, "dft_ot_energy_density_bf");
+  if (!check_wf(otf, gwf, "dft_ot_energy_density_bf") || !check_r(otf, energy_density, "dft_ot_energy_density_bf") ||
+      !check_r(otf, density, "dft_ot_energy_density_bf"))
+    return;
+  if (!(otf->model & DFT_OT_BACKFLOW)) {
+    fprintf(stderr, "libdft: dft_ot_energy_density_bf: functional allocated without DFT_OT_BACKFLOW (no kernel U)\n");
+    return;
+  }
   PotParts parts{false, false, true};
   DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)energy_density->dev, (const cplx<T> *)gwf->grid->dev,
                                                 gwf->mass, (T *)density->dev, true, parts, true));
 }
 
 REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf) {
+  if (!check_wf(otf, gwf, "dft_ot_energy")) return NAN;
   dft_ot_energy_density(otf, otf->workspace1, gwf);
   return rgrid_integral(otf->workspace1);
 }
 
 void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density) {
-  check_r(otf, grad, "dft_ot_energy_gradient");
-  check_r(otf, density, "dft_ot_energy_gradient");
+  if (!check_r(otf, grad, "dft_ot_energy_gradient") || !check_r(otf, density, "dft_ot_energy_gradient")) return;
   DFT_BY_PREC(otf->device->prec, gradient_impl<T>(otf, (T *)grad->dev, (const T *)density->dev));
 }
 
 REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
-  check_r(otf, density, "dft_ot_energy_rho");
+  if (!check_r(otf, density, "dft_ot_energy_rho")) return NAN;
   PotParts parts{true, (otf->model & DFT_OT_KC) != 0, false};
   DFT_BY_PREC(otf->device->prec, energy_impl<T>(otf, (T *)otf->workspace1->dev, nullptr, 1.0, (T *)density->dev,
                                                 true, parts, false));
@@ -2199,7 +2222,7 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
 
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
-  check_wf(otf, gwf, "dft_ot_adjoint_well");
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_well")) return -1;
   if (otf->model & DFT_OT_BACKFLOW) {
     fprintf(stderr, "libdft: dft_ot_adjoint_well: the adjoint covers the PLAIN and KC models (no backflow)\n");
     return -1;
@@ -2223,7 +2246,7 @@ void dft_set_verbose(int level) { g_verbose = level; }
 
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
-  check_wf(otf, gwf, "dft_ot_adjoint_cavity");
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity")) return -1;
   if (otf->model & DFT_OT_BACKFLOW) {
     fprintf(stderr, "libdft: dft_ot_adjoint_cavity: the adjoint covers the PLAIN and KC models (no backflow)\n");
     return -1;
@@ -2241,15 +2264,17 @@ int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radiu
   return rc;
 }
 
-void dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf) {
-  check_wf(otf, gwf, "dft_ot_potential_vjp");
-  check_r(otf, rho_bar, "dft_ot_potential_vjp");
-  check_r(otf, v_bar, "dft_ot_potential_vjp");
+int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf) {
+  if (!check_wf(otf, gwf, "dft_ot_potential_vjp") || !check_r(otf, rho_bar, "dft_ot_potential_vjp") ||
+      !check_r(otf, v_bar, "dft_ot_potential_vjp"))
+    return -1;
   if (otf->model & DFT_OT_BACKFLOW) {
-    fprintf(stderr, "libdft: dft_ot_potential_vjp: the adjoint covers the PLAIN and KC models (no backflow)\n");
-    abort();
+    fprintf(stderr, "libdft: dft_ot_potential_vjp: V_OT of a backflow model depends on the phase of psi; use dft_ot_potential_vjp_psi\n");
+    return -1;
   }
-  DFT_BY_PREC(otf->device->prec, potential_vjp_impl<T>(otf, (T *)rho_bar->dev, (const T *)v_bar->dev, gwf));
+  bool ok = false;
+  DFT_BY_PREC(otf->device->prec, ok = potential_vjp_impl<T>(otf, (T *)rho_bar->dev, (const T *)v_bar->dev, gwf));
+  return ok ? 0 : -1;
 }
 
 }  // extern "C"

@@ -43,7 +43,7 @@ def test_ctypes_binding_loads():
         assert hasattr(lib, name), name
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
     """The ctypes mirrors have the sizes the C compiler gives the structs."""
     import ctypes
 
@@ -53,18 +53,19 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "libdft_mi355x.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(rgrid), sizeof(cgrid), sizeof(wf), sizeof(dft_ot_functional),
-         offsetof(dft_ot_functional, density), offsetof(dft_ot_functional, device));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rgrid), sizeof(cgrid), sizeof(wf), sizeof(dft_ot_functional),
+         offsetof(dft_ot_functional, density), offsetof(dft_ot_functional, device),
+         offsetof(rgrid, value_outside), offsetof(cgrid, outside_params_ptr));
   return 0;
 }
 '''
-    tmp = os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp"
-    src, exe = os.path.join(tmp, "layout.c"), os.path.join(tmp, "layout")
+    src, exe = str(tmp_path / "layout.c"), str(tmp_path / "layout")
     open(src, "w").write(code)
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
     got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(L.RGrid), ctypes.sizeof(L.CGrid), ctypes.sizeof(L.WF), ctypes.sizeof(L.OTF),
-            L.OTF.density.offset, L.OTF.device.offset]
+            L.OTF.density.offset, L.OTF.device.offset, L.RGrid.value_outside.offset,
+            L.CGrid.outside_params_ptr.offset]
     assert got == want
 
 

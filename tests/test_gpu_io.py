@@ -55,3 +55,39 @@ def test_checkpoint_restart_continues_exactly(tmp_path):
         lib.dft_ot_propagate(f.h, w2.h, None, mu, dt)
     assert np.array_equal(w2.get(), w.get())
     f.free(), w.free(), w2.free()
+
+
+@pytest.mark.parametrize("prec", [L.GRID_FP64, L.GRID_FP32], ids=["fp64", "fp32"])
+def test_kernel_spectrum_files(tmp_path, prec):
+    """A functional's kernel grids (real half spectra, rows at a padded pitch)
+    round-trip through .grd files in place, at nz = 64 where the pitch (48)
+    exceeds nz/2 + 1 (33); the potential is unchanged afterwards, so no
+    neighbouring device memory was touched.  Files hold nx*ny*(nz/2+1) values,
+    and a file of a half spectrum is refused by a real-field grid."""
+    shape, step = (16, 16, 64), 0.6 / L.AUTOANG
+    lib = L.lib()
+    w = L.Wave(shape, step, prec)
+    w.set(inputs.smooth_modulated(shape, step, 0.0218 * L.AUTOANG ** 3, 0.2))
+    f = L.Functional(9, w, 2, 8)
+    pot = L.CField(shape, step, prec)
+    f.potential(pot)
+    v0 = pot.get()
+    half = (shape[0], shape[1], shape[2] // 2 + 1)
+    lj0, sa0 = L.rgrid_get(f.p.lennard_jones, half), L.rgrid_get(f.p.spherical_avg, half)
+    lib.rgrid_write_grid(str(tmp_path / "lj").encode(), f.p.lennard_jones)
+    lib.rgrid_write_grid(str(tmp_path / "sa").encode(), f.p.spherical_avg)
+    assert (tmp_path / "lj.grd").stat().st_size == 72 + 8 * int(np.prod(half))
+    lib.rgrid_read_grid(f.p.spherical_avg, str(tmp_path / "lj.grd").encode())
+    assert np.array_equal(L.rgrid_get(f.p.spherical_avg, half), lj0)
+    lib.rgrid_read_grid(f.p.spherical_avg, str(tmp_path / "sa.grd").encode())
+    assert np.array_equal(L.rgrid_get(f.p.spherical_avg, half), sa0)
+    assert np.array_equal(L.rgrid_get(f.p.lennard_jones, half), lj0)
+    r = L.RField(shape, step, prec)
+    r.set(np.full(shape, 3.0))
+    lib.rgrid_read_grid(r.h, str(tmp_path / "lj.grd").encode())
+    assert np.all(r.get() == 3.0)
+    pot.set(np.zeros(shape))
+    f.potential(pot)
+    assert np.array_equal(pot.get(), v0)
+    for x in (r, pot, f, w):
+        x.free()
