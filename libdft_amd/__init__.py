@@ -109,6 +109,7 @@ _SIGS = {
     "grid_wf_propagate_predict": (None, ["wf", "wf", "cg", "z"]),
     "grid_wf_propagate_correct": (None, ["wf", "cg", "z"]),
     "dft_ot_alloc": ("otf", ["i64", "wf", "i64", "i64"]), "dft_ot_free": (None, ["otf"]),
+    "dft_ot_init_params": (None, ["otf", "i64"]),
     "dft_ot_potential": (None, ["otf", "cg", "wf"]),
     "dft_ot_potential2": (None, ["otf", "cg", "rg", "rg", "rg", "rg"]),
     "dft_ot_backflow_potential": (None, ["otf", "cg", "rg", "rg", "rg", "rg"] + ["rg"] * 6),
@@ -149,6 +150,8 @@ for _n in ("energy", "dEdRho", "pressure", "dPdRho", "density_pressurized", "che
     _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf", "d"])
 for _n in ("density", "chempot", "chempot2"):
     _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf"])
+for _n in ("dispersion", "istatic"):
+    _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf", "p", "d"])
 
 _T = {"i": C.c_int, "i64": C.c_int64, "d": C.c_double, "p": C.c_void_p, "s": C.c_char_p, "c": C.c_char, "b": C.c_byte,
       "z": Complex, "rg": _RG, "cg": C.POINTER(CGrid), "wf": C.POINTER(WF), "otf": C.POINTER(OTF)}
@@ -254,6 +257,15 @@ class RField:
         if self.h:
             lib().rgrid_free(self.h)
             self.h = None
+
+
+def host_params(model):
+    """A dft_ot_functional on the host holding only dft_ot_init_params(model)
+    (enough for the bulk closed forms; no device state)."""
+    otf = OTF()
+    otf.model = model
+    lib().dft_ot_init_params(C.byref(otf), model)
+    return otf
 
 
 def rgrid_get(h, shape):

@@ -33,6 +33,17 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+/* Planted mistakes (tests only, default 0): each number breaks one term, so
+   that the tests can show the checks that pin the oracle would catch it.
+   1 Re V_BF and eps_BF: -2 v.B -> +2 v.B      2 Re V_BF and eps_BF: C dropped
+   3 eps_BF: m/4 -> m/2 (potential unchanged)  4 Re V_BF and eps_BF both x2
+   5 Im V_BF: (d rho/(rho+eps)) W term negated 6 Im V_BF: 1/2 d W -> d W
+   7 W = v A - B -> v A + B                     8 KC term 1: rho_st -> 1
+   9 KC term 1 negated                         10 KC term 2 negated
+  11 KC term 3 dropped */
+static int g_plant = 0;
+void orc_set_plant(int which) { g_plant = which; }
+
 double orc_wall_time(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -439,7 +450,7 @@ static void add_kc(orc_ot *o, const geom *g, double complex *pot, const double *
     r_convolute(g, cw1, gd_tf[d], cw2);
     r_inv_norm2(g, cw1, w3);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w3[i] = w3[i] * rho_st[i] * c;
+    for (long i = 0; i < n; i++) w3[i] = w3[i] * (g_plant == 8 ? 1.0 : rho_st[i]) * (g_plant == 9 ? -c : c);
     add_re(pot, w3, n);
     /* term 2: c F * (d rho J), J = F * G */
     r_convolute(g, cw1, o->g_tf, cw2);
@@ -450,13 +461,13 @@ static void add_kc(orc_ot *o, const geom *g, double complex *pot, const double *
     r_convolute(g, cw1, o->g_tf, cw1);
     r_inv_norm2(g, cw1, w4);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w4[i] = w4[i] * (c * k23);
+    for (long i = 0; i < n; i++) w4[i] = w4[i] * (g_plant == 10 ? -c * k23 : c * k23);
     add_re(pot, w4, n);
     /* term 3: -c J (dF * rho) */
     r_convolute(g, cw1, gd_tf[d], rho_tf);
     r_inv_norm2(g, cw1, w2);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w2[i] = w2[i] * w3[i] * (-c * k23);
+    for (long i = 0; i < n; i++) w2[i] = w2[i] * w3[i] * (g_plant == 11 ? 0.0 : -c * k23);
     add_re(pot, w2, n);
   }
 }
@@ -483,12 +494,13 @@ static void add_bf(orc_ot *o, const geom *g, double complex *pot, const double *
   #pragma omp parallel for
   for (long i = 0; i < n; i++) {
     double t = vz[i] * vz[i] + vy[i] * vy[i] + vx[i] * vx[i];
+    const double mb = g_plant == 1 ? 2.0 : -2.0;
     t = t * A[i];
-    t += -2.0 * vx[i] * Bx[i];
-    t += -2.0 * vy[i] * By[i];
-    t += -2.0 * vz[i] * Bz[i];
-    t = t + C[i];
-    w6[i] = t * (-0.5 * o->mass);
+    t += mb * vx[i] * Bx[i];
+    t += mb * vy[i] * By[i];
+    t += mb * vz[i] * Bz[i];
+    if (g_plant != 2) t = t + C[i];
+    w6[i] = t * (g_plant == 4 ? -o->mass : -0.5 * o->mass);
   }
   if (o->max_bfpot < 10000.0 / ORC_AUTOK) r_clamp(w6, n, o->max_bfpot, -o->max_bfpot, o->max_bfpot, -o->max_bfpot);
   if (o->c_bfpot != 1.0) for (long i = 0; i < n; i++) w6[i] *= o->c_bfpot;
@@ -497,17 +509,17 @@ static void add_bf(orc_ot *o, const geom *g, double complex *pot, const double *
   memset(w6, 0, sizeof(double) * (size_t) n);
   for (int d = 0; d < 3; d++) {
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) v[d][i] = v[d][i] * A[i] - B[d][i];
+    for (long i = 0; i < n; i++) v[d][i] = v[d][i] * A[i] + (g_plant == 7 ? B[d][i] : -B[d][i]);
   }
   for (int d = 0; d < 3; d++) {
     grad_dir(rho, C, g->nx, g->ny, g->nz, g->step, d);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w6[i] += 0.5 * (C[i] / (rho[i] + o->div_epsilon)) * v[d][i];
+    for (long i = 0; i < n; i++) w6[i] += (g_plant == 5 ? -0.5 : 0.5) * (C[i] / (rho[i] + o->div_epsilon)) * v[d][i];
   }
   for (int d = 0; d < 3; d++) {
     grad_dir(v[d], C, g->nx, g->ny, g->nz, g->step, d);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w6[i] += 0.5 * C[i];
+    for (long i = 0; i < n; i++) w6[i] += (g_plant == 6 ? 1.0 : 0.5) * C[i];
   }
   if (o->max_bfpot < 10000.0 / ORC_AUTOK) r_clamp(w6, n, o->max_bfpot, -o->max_bfpot, o->max_bfpot, -o->max_bfpot);
   if (o->c_bfpot != 1.0) for (long i = 0; i < n; i++) w6[i] *= o->c_bfpot;
@@ -557,13 +569,13 @@ void orc_ot_potential2(orc_ot *o, double complex *pot, const double *rho, double
 
 /* ------------------------------------------------- energy density (a10) -- */
 
-/* src/ot-energy.c:38-110, 123-191, 202-288 (HD penalty, no thermal). */
-void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, double wf_mass) {
+/* src/ot-energy.c:38-110, 123-191, 202-288 (HD penalty, no thermal), with the
+   density rho and, for backflow, the velocity vel given. */
+static void energy_body(orc_ot *o, double *eps, const double *rho, double *const *vel) {
   geom g = G(o);
   long n = g.n;
-  double *rho = rbuf(n), *w1 = rbuf(n), *w2 = rbuf(n);
+  double *w1 = rbuf(n), *w2 = rbuf(n);
   double complex *cw1 = cbuf(n), *cw2 = cbuf(n);
-  orc_wf_density(psi, rho, n);
   memset(eps, 0, sizeof(double) * (size_t) n);
   r_fft(&g, rho, cw1);
   r_convolute(&g, cw2, cw1, o->lj_tf);
@@ -606,12 +618,12 @@ void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, do
   }
 
   if (o->model & ORC_OT_BACKFLOW) {
-    double *v[3], *v2 = rbuf(n);
-    for (int d = 0; d < 3; d++) v[d] = rbuf(n);
-    orc_wf_velocity(psi, v[0], v[1], v[2], g.nx, g.ny, g.nz, g.step, wf_mass, ORC_DFT_EPS);
+    double *const *v = vel, *v2 = rbuf(n);
     #pragma omp parallel for
     for (long i = 0; i < n; i++) v2[i] = (v[0][i] * v[0][i] + v[1][i] * v[1][i]) + v[2][i] * v[2][i];
-    double mq = o->c_bfpot * o->mass / 4.0, mh = o->c_bfpot * o->mass / 2.0;
+    double mq = o->c_bfpot * o->mass / (g_plant == 3 || g_plant == 4 ? 2.0 : 4.0), mh = o->c_bfpot * o->mass / 2.0;
+    if (g_plant == 1) mh = -mh;
+    if (g_plant == 4) mh *= 2.0;
     r_fft(&g, rho, cw1);
     r_convolute(&g, cw1, o->bf_tf, cw1);
     r_inv_norm2(&g, cw1, w1);
@@ -632,11 +644,29 @@ void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, do
     r_convolute(&g, cw1, o->bf_tf, cw1);
     r_inv_norm2(&g, cw1, w1);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) eps[i] += -mq * (w1[i] * rho[i]);
-    for (int d = 0; d < 3; d++) free(v[d]);
+    for (long i = 0; i < n; i++) eps[i] += g_plant == 2 ? 0.0 : -mq * (w1[i] * rho[i]);
     free(v2);
   }
-  free(rho); free(w1); free(w2); free(cw1); free(cw2);
+  free(w1); free(w2); free(cw1); free(cw2);
+}
+
+void orc_ot_energy_density(orc_ot *o, double *eps, const double complex *psi, double wf_mass) {
+  geom g = G(o);
+  double *rho = rbuf(g.n), *v[3] = {NULL, NULL, NULL};
+  orc_wf_density(psi, rho, g.n);
+  if (o->model & ORC_OT_BACKFLOW) {
+    for (int d = 0; d < 3; d++) v[d] = rbuf(g.n);
+    orc_wf_velocity(psi, v[0], v[1], v[2], g.nx, g.ny, g.nz, g.step, wf_mass, ORC_DFT_EPS);
+  }
+  energy_body(o, eps, rho, v);
+  free(rho);
+  for (int d = 0; d < 3; d++) free(v[d]);
+}
+
+void orc_ot_energy_density_rv(orc_ot *o, double *eps, const double *rho, const double *vx, const double *vy,
+                              const double *vz) {
+  double *v[3] = {(double *) vx, (double *) vy, (double *) vz};
+  energy_body(o, eps, rho, v);
 }
 
 /* ------------------------------------------------- backward of E[rho] -- */

@@ -77,6 +77,13 @@ void orc_ot_potential(orc_ot *otf, double complex *pot, const double complex *ps
 void orc_ot_potential2(orc_ot *otf, double complex *pot, const double *rho, double *vx, double *vy, double *vz);
 /* Overwrites eps with the OT energy density (kinetic part excluded). */
 void orc_ot_energy_density(orc_ot *otf, double *eps, const double complex *psi, double wf_mass);
+/* Same with the density and the velocity given (the backflow part takes v as
+   is; the other parts depend on rho only). */
+void orc_ot_energy_density_rv(orc_ot *otf, double *eps, const double *rho, const double *vx, const double *vy,
+                              const double *vz);
+/* Tests only: plant a numbered mistake in one term (0 = none), so that the
+   tests can show that the checks pinning the oracle catch it. */
+void orc_set_plant(int which);
 
 /* Backward of E[rho]: exact discrete dE/drho of the LJ, c2/c3 and KC energy. */
 void orc_ot_energy_gradient(orc_ot *otf, double *grad, const double *rho);

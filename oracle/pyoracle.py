@@ -46,6 +46,8 @@ def lib():
             "orc_ot_potential": (None, [P, P, P, D, P]),
             "orc_ot_potential2": (None, [P, P, P, P, P, P]),
             "orc_ot_energy_density": (None, [P, P, P, D]),
+            "orc_ot_energy_density_rv": (None, [P, P, P, P, P, P]),
+            "orc_set_plant": (None, [I]),
             "orc_ot_energy_gradient": (None, [P, P, P]),
             "orc_ot_energy_rho": (D, [P, P]),
             "orc_fft3d": (None, [P, I, I, I, I]),
@@ -145,6 +147,13 @@ class OracleOT:
         lib().orc_ot_energy_density(self._h, _p(eps), _p(psi), wf_mass)
         return eps
 
+    def energy_density_rv(self, rho, vx, vy, vz):
+        """Energy density with the density and the (backflow) velocity given."""
+        a = [np.ascontiguousarray(x, np.float64) for x in (rho, vx, vy, vz)]
+        eps = np.empty(self.shape)
+        lib().orc_ot_energy_density_rv(self._h, _p(eps), *[_p(x) for x in a])
+        return eps
+
     def energy_gradient(self, rho):
         """Backward of E[rho]: exact discrete dE/drho (LJ, c2/c3, KC)."""
         rho = np.ascontiguousarray(rho, np.float64)
@@ -214,6 +223,11 @@ def norm(psi, step):
 def kinetic_energy(psi, step, mass=HELIUM_MASS):
     psi = np.ascontiguousarray(psi, np.complex128)
     return lib().orc_wf_kinetic_energy(_p(psi), *psi.shape, step, mass)
+
+
+def set_plant(which):
+    """Tests only: plant mistake number `which` in the oracle (0 = none)."""
+    lib().orc_set_plant(int(which))
 
 
 def wall_time():

@@ -1,0 +1,104 @@
+"""The product library's host closed forms (dft_ot_init_params, dft_ot_bulk_*,
+libdft_amd/csrc/bulk.cpp) on a host functional struct: no device is touched.
+
+They are run through the reference's equation-of-state fixture
+(examples/bulk/equation_of_state.dat, printed by examples/bulk/bulk_properties.c:54-61)
+and compared with the oracle's restatement of src/helium-ot-bulk.c for every
+model, and the thermodynamic relations between them are checked.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import libdft_amd as L
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _eos_rows():
+    rows = []
+    with open(os.path.join(GOLDEN, "equation_of_state.dat")) as f:
+        for line in f:
+            t = line.replace("\t", " ").split()
+            if t and t[0] == "Pressure:":
+                rows.append((float(t[1]), float(t[4]), float(t[7])))
+    return rows
+
+
+def _bulk(otf, name, *args):
+    return getattr(L.lib(), "dft_ot_bulk_" + name)(C.byref(otf), *[float(a) for a in args])
+
+
+def _kbulk(otf, name, k, rho):
+    kk = C.c_double(k)
+    return getattr(L.lib(), "dft_ot_bulk_" + name)(C.byref(otf), C.cast(C.byref(kk), C.c_void_p), float(rho))
+
+
+def test_init_params_table():
+    """src/ot.c:1026-1036 (T = 0 row) and 1234-1270 (common block), in atomic units."""
+    p = L.host_params(L.DFT_OT_KC | L.DFT_OT_BACKFLOW)
+    a3 = L.AUTOANG ** 3
+    assert abs(p.b * L.AUTOK * a3 - (-718.99)) < 1e-9
+    assert abs(p.c2 * L.AUTOK * a3 ** 2 - (-2.411857e4)) < 1e-6
+    assert abs(p.c3 * L.AUTOK * a3 ** 3 - 1.858496e6) < 1e-3
+    assert abs(p.rho0 / a3 - 0.0218360) < 1e-12
+    assert abs(p.lj_h * L.AUTOANG - 2.1903) < 1e-12 and abs(p.lj_sigma * L.AUTOANG - 2.556) < 1e-12
+    assert abs(p.lj_epsilon * L.AUTOK - 10.22) < 1e-12
+    assert abs(p.rho_0s / a3 - 0.04) < 1e-12 and abs(p.alpha_s * a3 - 54.31) < 1e-9
+    assert abs(p.l_g * L.AUTOANG - 1.0) < 1e-12 and abs(p.mass * L.AUTOAMU - 4.0026) < 1e-12
+    assert p.c_bfpot == 1.0 and abs(p.div_epsilon - 1e-4) < 1e-18
+
+
+def test_equation_of_state_fixture_product():
+    otf = L.host_params(L.DFT_OT_PLAIN)
+    rows = _eos_rows()
+    assert len(rows) == 241
+    for P, rho_ref, mu_ref in rows:
+        p_au = P / L.AUTOBAR
+        rho = _bulk(otf, "density_pressurized", p_au) / L.AUTOANG ** 3
+        mu = _bulk(otf, "chempot_pressurized", p_au) * L.AUTOK
+        assert abs(rho - rho_ref) < 1e-6, (P, rho, rho_ref)
+        # the Newton stop |P(rho) - P| < 1e-12 au (src/helium-ot-bulk.c:204) and libgrid's bar
+        # constant (~6e-6 K per bar from CODATA) bound the agreement in mu
+        assert abs(mu - mu_ref) < 1.2e-4 + 6e-6 * P, (P, mu, mu_ref)
+
+
+@pytest.mark.parametrize("model", [0, 1, 8, 9])
+def test_product_bulk_matches_oracle(oracle, model):
+    po = oracle
+    otf = L.host_params(model)
+    o = po.OracleOT(model, (8, 8, 8), 1.0, 1, 1)
+    for rho in (0.85 * otf.rho0, otf.rho0, 1.3 * otf.rho0):
+        for name in ("energy", "dEdRho", "pressure", "dPdRho", "compressibility", "sound_speed"):
+            a, b = _bulk(otf, name, rho), o.bulk(name, rho)
+            if np.isnan(b):  # below the spinodal dP/drho < 0: no sound speed
+                assert np.isnan(a), (name, rho, a)
+                continue
+            assert abs(a - b) <= 1e-12 * abs(b), (name, rho, a, b)
+        for k_ang in (0.2, 0.9, 1.9):
+            k = k_ang * L.AUTOANG
+            a, b = _kbulk(otf, "dispersion", k, rho), o.bulk("dispersion", k, rho)
+            assert abs(a - b) <= 1e-12 * abs(b), ("dispersion", k_ang, a, b)
+            a, b = _kbulk(otf, "istatic", k, rho), o.bulk("istatic", k, rho)
+            assert abs(a - b) <= 1e-12 * abs(b), ("istatic", k_ang, a, b)
+    for P in (0.0, 5.0 / L.AUTOBAR, 20.0 / L.AUTOBAR):
+        a, b = _bulk(otf, "density_pressurized", P), o.bulk("density_pressurized", P)
+        assert abs(a - b) <= 1e-12 * b
+
+
+def test_bulk_relations_product():
+    """P = rho dE/drho - E at the zero-pressure density, mu there = dE/drho, and
+    c^2 = (rho/m) d^2E/drho^2 . rho (sound speed from the compressibility)."""
+    otf = L.host_params(L.DFT_OT_PLAIN)
+    r0 = _bulk(otf, "density")
+    assert abs(_bulk(otf, "pressure", r0)) < 1e-12
+    assert abs(_bulk(otf, "chempot") - _bulk(otf, "dEdRho", r0)) < 1e-15
+    assert abs(_bulk(otf, "chempot2") - _bulk(otf, "dEdRho", otf.rho0)) < 1e-15
+    e = 1e-6 * r0
+    d2 = (_bulk(otf, "dEdRho", r0 + e) - _bulk(otf, "dEdRho", r0 - e)) / (2 * e)
+    c = _bulk(otf, "sound_speed", r0)
+    assert abs(c * c - r0 * d2 / otf.mass) < 1e-6 * c * c
+    # the sound speed of the OT functional at zero pressure, ~237 m/s (the fit's target)
+    assert 200.0 < c * L.AUTOMPS < 260.0

@@ -1,0 +1,305 @@
+"""Pins the oracle's backflow (BF) and kinetic-correlation (KC) terms with
+checks that the mathematics fixes (no GPU needed).
+
+Backflow, src/ot.c:840-991 and src/ot-energy.c:202-288:
+  eps_BF = -(m/4) c_bf [rho v^2 A - 2 rho v.B + rho C],   A = U*rho,
+           B_d = U*(rho v_d), C = U*(rho v^2),
+  Re V_BF = -(m/2) c_bf [v^2 A - 2 v.B + C],
+  Im V_BF = (c_bf/2) sum_d [(D_d rho / (rho + eps)) W_d + D_d W_d],  W_d = v_d A - B_d.
+Checks:
+  pair form    h^3 sum eps_BF = -(m/4) c_bf h^6 sum_ij U_ij rho_i rho_j |v_i - v_j|^2,
+               the published form of the OT backflow energy (Dalfovo et al.,
+               PRB 52, 1193 (1995)), by brute force over all pairs with the
+               lattice kernel values;
+  d rho        Re V_BF = (1/h^3) dE/drho at fixed v (E is quadratic in rho at
+               fixed v, so central differences are exact);
+  d v          (1/h^3) dE/dv_d = -m c_bf rho W_d with W_d the v_d A - B_d that
+               dft_ot_potential2 leaves in its velocity inputs (src/ot.c:923-931);
+  norm         sum rho Im V_BF = 0 for eps = 0 (summation by parts with the
+               periodic central difference): the imaginary potential moves no
+               norm;
+  response     Im V_BF of a small phase wave on uniform rho0, to first order:
+               -(rho0 c / 2M) (U^(0) - U^(k)) sin^2(kh)/h^2 phi_k (c = rho0/(rho0+eps)),
+               from the lattice spectrum of U;
+  Galilean     a plane wave on uniform density has no backflow energy or potential.
+Kinetic correlation, src/ot.c:604-797:
+  response     term 1 of a small density wave, to first order:
+               -c rho_st0^2 F^(k) sin^2(kh)/h^2 delta_k, rho_st0 = 1 - rho0 F^(0)/rho_0s,
+               c = alpha_s/2M; its small-k limit is the KC term of
+               dft_ot_bulk_istatic (src/helium-ot-bulk.c:445-463);
+  k23          terms 2 and 3 carry the coefficient k23 linearly, so the
+               reference's potential (k23 = 1) is the k23 = 1/rho_0s one with
+               those two terms scaled;
+  convergence  with k23 = 1/rho_0s the potential is the product-rule expansion
+               of the exact discrete derivative c sum_d D_d(rho_st J_d) +
+               (c/rho_0s) F*(sum_d D_d rho J_d): it differs from central
+               differences of the KC energy by the lattice product-rule error,
+               O(h^2), which falls 4x when h halves at a fixed box.
+test_checks_catch_planted_mistakes plants one mistake at a time in the oracle
+(flipped -2 v.B, dropped C, wrong m/4, both scaled, each Im term changed, W's
+sign, KC term 1 changed) and shows that at least one check fails for each.
+"""
+import numpy as np
+import pytest
+
+from tests import inputs
+
+SHAPE, STEP_ANG = (16, 16, 16), 0.6
+
+
+def _setup(po, model, shape=SHAPE, step_ang=STEP_ANG):
+    return po.OracleOT(model, shape, step_ang / po.AUTOANG, 2, 8)
+
+
+def _smooth_random(shape, seed, kmax=3):
+    """A seeded smooth random field: white noise keeping the modes |m_d| <= kmax, unit maximum."""
+    u = inputs.splitmix64_uniform(int(np.prod(shape)), seed).reshape(shape)
+    f = np.fft.fftn(u)
+    m = np.meshgrid(*[np.fft.fftfreq(n) * n for n in shape], indexing="ij")
+    f[(np.abs(m[0]) > kmax) | (np.abs(m[1]) > kmax) | (np.abs(m[2]) > kmax)] = 0.0
+    g = np.fft.ifftn(f).real
+    return g / np.abs(g).max()
+
+
+def _rho_v(po, shape, step, seed=5):
+    """A strongly modulated density and a smooth random velocity field (a.u.)."""
+    rho = 0.0218 * po.AUTOANG ** 3 * (1.0 + 0.4 * _smooth_random(shape, seed))
+    v = [1e-4 * _smooth_random(shape, seed + 1 + d) for d in range(3)]
+    return rho, v
+
+
+def _ebf(po, o8, o0, rho, v):
+    """h^3 sum eps_BF with rho and v given (model 8 minus model 0)."""
+    h3 = o8.step ** 3
+    return (o8.energy_density_rv(rho, *v) - o0.energy_density_rv(rho, *v)).sum() * h3
+
+
+# ------------------------------------------------------------------ checks --
+# Each returns err / tol (<= 1 passes).
+
+def check_pair_form(po):
+    shape, step = (8, 8, 8), 0.9 / po.AUTOANG
+    o8, o0 = po.OracleOT(8, shape, step, 2, 8), po.OracleOT(0, shape, step, 2, 8)
+    rho, v = _rho_v(po, shape, step, seed=11)
+    got = _ebf(po, o8, o0, rho, v)
+    # lattice kernel: value at displacement D (mod n) is the centred map at D + n/2
+    U = np.roll(o8.kernel_map(3, 2, 8), tuple(-n // 2 for n in shape), axis=(0, 1, 2))
+    idx = np.indices(shape).reshape(3, -1)
+    diff = [(idx[a][:, None] - idx[a][None, :]) % shape[a] for a in range(3)]
+    Uij = U[diff[0], diff[1], diff[2]]
+    r, vv = rho.ravel(), [x.ravel() for x in v]
+    dv2 = sum((vv[d][:, None] - vv[d][None, :]) ** 2 for d in range(3))
+    want = -(o8.p.mass / 4.0) * o8.p.c_bfpot * step ** 6 * np.sum(Uij * r[:, None] * r[None, :] * dv2)
+    return abs(got - want) / (1e-10 * abs(want))
+
+
+def check_density_derivative(po):
+    o8, o0 = _setup(po, 8), _setup(po, 0)
+    step = o8.step
+    rho, v = _rho_v(po, SHAPE, step)
+    eta = 0.1 * o8.p.rho0 * inputs.splitmix64_uniform(rho.size, 7).reshape(SHAPE)
+    t = 1e-2
+    fd = (_ebf(po, o8, o0, rho + t * eta, v) - _ebf(po, o8, o0, rho - t * eta, v)) / (2 * t)
+    vre = (o8.potential2(rho, *v) - o0.potential2(rho, *v)).real
+    an = (vre * eta).sum() * step ** 3
+    return abs(fd - an) / (1e-9 * abs(an))
+
+
+def check_velocity_derivative(po):
+    o8, o0 = _setup(po, 8), _setup(po, 0)
+    step = o8.step
+    rho, v = _rho_v(po, SHAPE, step)
+    zeta = [1e-4 * inputs.splitmix64_uniform(rho.size, 20 + d).reshape(SHAPE) for d in range(3)]
+    t = 1e-1
+    fd = (_ebf(po, o8, o0, rho, [v[d] + t * zeta[d] for d in range(3)])
+          - _ebf(po, o8, o0, rho, [v[d] - t * zeta[d] for d in range(3)])) / (2 * t)
+    # dft_ot_potential2 leaves W_d = v_d A - B_d in its velocity inputs
+    W = [a.copy() for a in v]
+    po.lib().orc_ot_potential2(o8._h, po._p(np.zeros(SHAPE, np.complex128)), po._p(rho), *[po._p(a) for a in W])
+    an = -o8.p.mass * o8.p.c_bfpot * step ** 3 * sum((rho * W[d] * zeta[d]).sum() for d in range(3))
+    return abs(fd - an) / (1e-9 * abs(an))
+
+
+def check_norm_identity(po):
+    o8 = _setup(po, 8)
+    o8.p.div_epsilon = 0.0
+    rho, v = _rho_v(po, SHAPE, o8.step)
+    vim = o8.potential2(rho, *v).imag
+    return abs((rho * vim).sum()) / (1e-12 * np.abs(rho * vim).sum())
+
+
+def _lattice_spec(o, name, m, axis=0):
+    idx = [0, 0, 0]
+    idx[axis] = m
+    return o.spectrum(name)[tuple(idx)].real * (-1.0) ** m * o.step ** 3
+
+
+def check_imag_response(po):
+    n, step = 32, 0.4 / po.AUTOANG
+    o8 = po.OracleOT(8, (n, n, n), step, 2, 8)
+    rho0 = o8.p.rho0
+    x = np.arange(n) * step
+    err = 0.0
+    for m in (1, 3, 6):
+        k = 2 * np.pi * m / (n * step)
+        a = 1e-4
+        phi = a * np.cos(k * x)[:, None, None] * np.ones((n, n, n))
+        psi = np.sqrt(rho0) * np.exp(1j * phi)
+        vim = o8.potential(psi).imag
+        amp = 2 * np.fft.fft(vim[:, 0, 0])[m].real / n / a
+        c = rho0 / (rho0 + o8.p.div_epsilon)
+        pred = -(rho0 * c / (2 * po.HELIUM_MASS)) * (_lattice_spec(o8, "bf_tf", 0) - _lattice_spec(o8, "bf_tf", m)) \
+            * np.sin(k * step) ** 2 / step ** 2
+        err = max(err, abs(amp - pred) / (1e-6 * abs(pred)))
+    return err
+
+
+def check_galilean(po):
+    o8, o0 = _setup(po, 8), _setup(po, 0)
+    step = o8.step
+    L = SHAPE[0] * step
+    x, y, z = np.meshgrid(*[np.arange(n) * step for n in SHAPE], indexing="ij")
+    kv = [2 * np.pi * 1 / L, 2 * np.pi * -2 / L, 2 * np.pi * 3 / L]
+    psi = np.sqrt(o8.p.rho0) * np.exp(1j * (kv[0] * x + kv[1] * y + kv[2] * z))
+    dV = o8.potential(psi) - o0.potential(psi)
+    de = o8.energy_density(psi) - o0.energy_density(psi)
+    # scale: one of the three terms, (m/2) v^2 A
+    v2 = sum((np.sin(k * step) / step / po.HELIUM_MASS) ** 2 for k in kv)
+    scale = 0.5 * o8.p.mass * v2 * o8.p.rho0 * abs(_lattice_spec(o8, "bf_tf", 0))
+    return max(np.abs(dV).max() / (1e-9 * scale), np.abs(de).max() / (1e-9 * scale * o8.p.rho0))
+
+
+def _kc_fd_mismatch(po, n, box_ang=9.6):
+    step = box_ang / n / po.AUTOANG
+    o = po.OracleOT(1, (n, n, n), step, 2, 8)
+    o.set_kc_exact(True)
+    L = n * step
+    x, y, z = np.meshgrid(*[np.arange(n) * step for _ in range(3)], indexing="ij")
+    f = (np.sin(2 * np.pi * x / L) * np.cos(2 * np.pi * y / L) + 0.5 * np.sin(4 * np.pi * z / L + 1)
+         + 0.4 * np.cos(2 * np.pi * (x + z) / L + 0.3))
+    rho = o.p.rho0 * (1 + 0.3 * f)
+    eta = o.p.rho0 * (f + 0.5 * np.sin(2 * np.pi * y / L + 2))
+    V = o.potential(np.sqrt(rho).astype(complex)).real
+    E = lambda r: o.energy_density(np.sqrt(r).astype(complex)).sum() * step ** 3
+    t = 1e-3
+    fd = (E(rho + t * eta) - E(rho - t * eta)) / (2 * t)
+    # the KC share of the derivative (the PLAIN part is exact, pinned elsewhere)
+    o0 = po.OracleOT(0, (n, n, n), step, 2, 8)
+    V0 = o0.potential(np.sqrt(rho).astype(complex)).real
+    E0 = lambda r: o0.energy_density(np.sqrt(r).astype(complex)).sum() * step ** 3
+    fd0 = (E0(rho + t * eta) - E0(rho - t * eta)) / (2 * t)
+    an = ((V - V0) * eta).sum() * step ** 3
+    return abs((fd - fd0) - an) / abs(fd - fd0)
+
+
+def check_kc_convergence(po):
+    e16, e32 = _kc_fd_mismatch(po, 16), _kc_fd_mismatch(po, 32)
+    # second order: the mismatch falls ~4x (3x allowed) and is small on the finer grid
+    return max(e32 / (e16 / 3.0), e32 / 1e-3)
+
+
+def check_kc_response(po):
+    n, step = 32, 0.4 / po.AUTOANG
+    o1, o0 = po.OracleOT(1, (n, n, n), step, 2, 8), po.OracleOT(0, (n, n, n), step, 2, 8)
+    rho0 = o1.p.rho0
+    c = o1.p.alpha_s / (2 * o1.p.mass)
+    rst0 = 1 - rho0 * _lattice_spec(o1, "g_tf", 0) / o1.p.rho_0s
+    x = np.arange(n) * step
+    err = 0.0
+    for m in (1, 3, 6):
+        k = 2 * np.pi * m / (n * step)
+        d = 1e-6 * rho0
+        rho = rho0 + d * np.cos(k * x)[:, None, None] * np.ones((n, n, n))
+        psi = np.sqrt(rho).astype(complex)
+        dv = (o1.potential(psi) - o0.potential(psi)).real
+        amp = 2 * np.fft.fft(dv[:, 0, 0])[m].real / n / d
+        pred = -c * rst0 ** 2 * _lattice_spec(o1, "g_tf", m) * np.sin(k * step) ** 2 / step ** 2
+        err = max(err, abs(amp - pred) / (1e-5 * abs(pred)))
+    return err
+
+
+CHECKS = {"pair_form": check_pair_form, "density_derivative": check_density_derivative,
+          "velocity_derivative": check_velocity_derivative, "norm_identity": check_norm_identity,
+          "imag_response": check_imag_response, "galilean": check_galilean, "kc_response": check_kc_response,
+          "kc_convergence": check_kc_convergence}
+
+
+@pytest.mark.parametrize("name", list(CHECKS))
+def test_pin(oracle, name):
+    oracle.set_plant(0)
+    assert CHECKS[name](oracle) <= 1.0
+
+
+PLANTS = {1: "-2v.B sign", 2: "C dropped", 3: "m/4 in eps only", 4: "Re V and eps both x2",
+          5: "Im (d rho/rho) W sign", 6: "Im 1/2 dW -> dW", 7: "W = vA + B", 8: "KC term 1 without rho_st",
+          9: "KC term 1 sign", 10: "KC term 2 sign", 11: "KC term 3 dropped"}
+
+
+@pytest.mark.parametrize("plant", list(PLANTS), ids=list(PLANTS.values()))
+def test_checks_catch_planted_mistakes(oracle, plant):
+    oracle.set_plant(plant)
+    try:
+        worst = max(f(oracle) for f in CHECKS.values())
+    finally:
+        oracle.set_plant(0)
+    assert worst > 10.0, (PLANTS[plant], worst)
+
+
+# ---------------------------------------------- continuum links of the kernels --
+def test_backflow_kernel_closed_form(oracle):
+    """Lattice spectrum of U_J against its continuum transform (gaussians times
+    quadratics, the form dft_ot_bulk_dispersion uses, src/helium-ot-bulk.c:397-409),
+    with the cell average's transform (the adaptive map's 32-point midpoint
+    rule: sin(kh/2) / (32 sin(kh/64))).  U's integral is a near-cancellation
+    (0.006 from a core of -20), so the tolerance is relative to the integral of
+    |U|'s gaussians: the adaptive cell average at its 32 sub-samples leaves ~1e-7 of it."""
+    po = oracle
+    n, step = 64, 0.4 / po.AUTOANG
+    o = po.OracleOT(8, (n, n, n), step, 4, 32)
+    p = o.p
+    scale = sum(abs(g1) * (np.pi / a) ** 1.5 + abs(g2) * 1.5 * np.pi ** 1.5 / a ** 2.5
+                for g1, g2, a in ((p.g11, p.g12, p.a1), (p.g21, p.g22, p.a2)))
+    for m in (0, 1, 2, 4, 8, 16):
+        k = 2 * np.pi * m / (n * step)
+        # cell average by the map's 32-point midpoint rule
+        cell = np.sin(k * step / 2) / (32 * np.sin(k * step / 64)) if m else 1.0
+        ft = 0.0
+        for g1, g2, a in ((p.g11, p.g12, p.a1), (p.g21, p.g22, p.a2)):
+            ft += (g1 * (np.pi / a) ** 1.5 + g2 * np.pi ** 1.5 * (6 * a - k * k) / (4 * a ** 3.5)) * np.exp(-k * k / (4 * a))
+        got = _lattice_spec(o, "bf_tf", m)
+        assert abs(got - ft * cell) < 2e-7 * scale, (m, got, ft * cell)
+
+
+def test_kc_response_small_k_limit(oracle):
+    """The lattice KC response is the KC term of dft_ot_bulk_istatic divided by
+    rho0, -(alpha_s/2m) (1 - rho0/rho_0s)^2 exp(-k^2 l^2/4) k^2, with the
+    lattice's own factors: sin^2(kh)/(kh)^2 of the central differences and
+    sin(kh/2) / (32 sin(kh/64)) of the cell-averaged kernel."""
+    po = oracle
+    n, step = 32, 0.4 / po.AUTOANG
+    o1 = po.OracleOT(1, (n, n, n), step, 4, 32)
+    p = o1.p
+    c = p.alpha_s / (2 * p.mass)
+    rst0 = 1 - p.rho0 * _lattice_spec(o1, "g_tf", 0) / p.rho_0s
+    for m in (1, 2, 4):
+        k = 2 * np.pi * m / (n * step)
+        lat = -c * rst0 ** 2 * _lattice_spec(o1, "g_tf", m) * np.sin(k * step) ** 2 / step ** 2
+        cont = -c * (1 - p.rho0 / p.rho_0s) ** 2 * np.exp(-k * k * p.l_g ** 2 / 4) * k * k
+        cont *= (np.sin(k * step) / (k * step)) ** 2 * np.sin(k * step / 2) / (32 * np.sin(k * step / 64))
+        assert abs(lat - cont) < 1e-5 * abs(cont), (m, lat, cont)
+
+
+def test_kc_terms_2_3_scale_with_k23(oracle):
+    """The reference KC potential (k23 = 1) differs from the exact derivative
+    (k23 = 1/rho_0s) only by the coefficient of terms 2 and 3, which enter linearly."""
+    po = oracle
+    o = _setup(po, 1)
+    psi = inputs.smooth_modulated(SHAPE, o.step, o.p.rho0, 0.3)
+    V = {}
+    for k23 in (0.0, 1.0, 1.0 / o.p.rho_0s):
+        o.p.kc_coef23 = k23
+        V[k23] = o.potential(psi).real
+    d1, dx = V[1.0] - V[0.0], V[1.0 / o.p.rho_0s] - V[0.0]
+    assert np.abs(dx - d1 / o.p.rho_0s).max() < 1e-9 * np.abs(dx).max()
+
+
