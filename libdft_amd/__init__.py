@@ -123,6 +123,7 @@ _SIGS = {
     "dft_ot_propagate": (None, ["otf", "wf", "rg", "d", "z"]),
     "dft_ot_propagate_pc": (None, ["otf", "wf", "wf", "rg", "d", "z"]),
     "dft_ot_potential_vjp": ("i", ["otf", "rg", "rg", "wf"]),
+    "dft_ot_potential_vjp_psi": ("i", ["otf", "cg", "cg", "wf"]),
     "rgrid_value_at_index": ("d", ["rg", "i64", "i64", "i64"]),
     "cgrid_value_at_index": ("z", ["cg", "i64", "i64", "i64"]),
     "cgrid_set_momentum": (None, ["cg", "d", "d", "d"]),

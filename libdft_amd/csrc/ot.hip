@@ -474,6 +474,148 @@ __global__ void k_vjp_out(T *out, cplx<T> *psib, const cplx<T> *psi, const T *rp
   }
 }
 
+// ----------------------------------------- backflow adjoint (reverse mode) --
+// Reverse pass through the backflow potential of src/ot.c:840-991 at the state
+// whose intermediates potential_impl left in the slots:
+//   v_d (clamped to max_veloc), A = U*rho, B_d = U*(rho v_d), C = U*(rho v^2),
+//   t = v^2 A - 2 v.B + C,  Re V += cbf clamp(-(m/2) t),
+//   W_d = v_d A - B_d,  g_d = D_d rho,
+//   Im V = cbf clamp(1/2 sum_d [(g_d / (rho + eps)) W_d + D_d W_d]).
+// Given Vr_bar and Vi_bar, the adjoints of Re V and Im V, the passes below
+// accumulate rho_bar and v_bar; clamped points pass no adjoint.
+struct BfAdj {
+  const void *vrb, *vib;          // adjoints of Re V and Im V
+  const void *rho, *v[3], *A, *W[3];
+  void *tb, *imb;                 // -(m/2) cbf Vr_bar (masked), (cbf/2) Vi_bar (masked)
+  void *Ab, *Bb[3], *gb[3], *vb[3], *rb;
+  const void *uA, *uB[3], *uC;    // U*Abar, U*Bbar_d, U*Cbar (Cbar = tb)
+};
+
+// Pass 1: tb and imb, each zero where the forward clamp (max_bfpot) was
+// active; the values before clamping are rebuilt from v, A, B_d, C and W_d.
+template <typename T>
+__global__ void k_bfadj_a(BfAdj a, const T *__restrict__ B0, const T *__restrict__ B1, const T *__restrict__ B2,
+                          const T *__restrict__ Cf, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *rho = (const T *)a.rho;
+  GRID_STRIDE(c, n) {
+    const T v[3] = {((const T *)a.v[0])[c], ((const T *)a.v[1])[c], ((const T *)a.v[2])[c]};
+    const T A = ((const T *)a.A)[c];
+    T tb = ((const T *)a.vrb)[c] * pw.cbf * ((T)-0.5 * pw.mass);
+    T ib = ((const T *)a.vib)[c] * pw.cbf * (T)0.5;
+    if (pw.clamp_bf) {
+      T t = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * A;
+      t += (T)-2 * v[0] * B0[c];
+      t += (T)-2 * v[1] * B1[c];
+      t += (T)-2 * v[2] * B2[c];
+      t = t + Cf[c];
+      if (fabs(t * ((T)-0.5 * pw.mass)) > pw.bfmax) tb = 0;
+      const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+      const T inv = (T)1 / (rho[c] + pw.div_eps);
+      T im = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T *W = (const T *)a.W[d];
+        im += (T)0.5 * ((rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h * inv) * W[c];
+        im += (T)0.5 * ((W[nb.p[d]] - W[nb.m[d]]) * pw.inv2h);
+      }
+      if (fabs(im) > pw.bfmax) ib = 0;
+    }
+    ((T *)a.tb)[c] = tb;
+    ((T *)a.imb)[c] = ib;
+  }
+}
+
+// Pass 2 (stencils of imb and rho): Wbar_d = imb g_d/(rho+eps) - D_d imb, and
+// the adjoints of t and of the W_d products.
+template <typename T> __global__ void k_bfadj_b(BfAdj a, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *rho = (const T *)a.rho, *imb = (const T *)a.imb;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    const T r = rho[c], tb = ((const T *)a.tb)[c], ib = imb[c], A = ((const T *)a.A)[c];
+    const T inv = (T)1 / (r + pw.div_eps);
+    T Ab = 0, rb = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T v = ((const T *)a.v[d])[c], W = ((const T *)a.W[d])[c];
+      const T g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+      const T Wb = ib * g * inv - (imb[nb.p[d]] - imb[nb.m[d]]) * pw.inv2h;
+      ((T *)a.vb[d])[c] = (T)2 * tb * W + Wb * A;
+      ((T *)a.Bb[d])[c] = (T)-2 * tb * v - Wb;
+      ((T *)a.gb[d])[c] = ib * W * inv;
+      Ab += tb * v * v + Wb * v;
+      rb -= ib * g * W * inv * inv;
+    }
+    ((T *)a.Ab)[c] = Ab;
+    ((T *)a.rb)[c] = rb;
+  }
+}
+
+// Pass 3 (stencil of gbar), after the convolutions uA = U*Abar, uB_d = U*Bbar_d,
+// uC = U*tb: rho_bar and v_bar of the backflow part, v_bar masked by the velocity clamp.
+template <typename T> __global__ void k_bfadj_c(BfAdj a, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    const T r = ((const T *)a.rho)[c], uC = ((const T *)a.uC)[c];
+    T rb = ((const T *)a.rb)[c] + ((const T *)a.uA)[c];
+    T v2 = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T v = ((const T *)a.v[d])[c];
+      v2 += v * v;
+    }
+    rb += uC * v2;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T *gb = (const T *)a.gb[d];
+      const T v = ((const T *)a.v[d])[c], uB = ((const T *)a.uB[d])[c];
+      rb -= (gb[nb.p[d]] - gb[nb.m[d]]) * pw.inv2h;
+      rb += uB * v;
+      T vb = ((const T *)a.vb[d])[c] + uB * r + (T)2 * uC * r * v;
+      if (pw.clamp_v && fabs(v) >= pw.vmax) vb = 0;
+      ((T *)a.vb[d])[c] = vb;
+    }
+    ((T *)a.rb)[c] = rb;
+  }
+}
+
+// psib += 2 rb psi + the transpose of the velocity v_d = Im(psi* D_d psi) / (M (|psi|^2 + eps))
+// applied to vb_d: at the point, v_bar d v/d psi; through D_d psi, -D_d e_d with
+// e_d = vb_d i psi / (M (|psi|^2 + eps)) evaluated at the neighbours.
+template <typename T>
+__global__ void k_bfadj_psi(cplx<T> *psib, const cplx<T> *__restrict__ psi, const T *__restrict__ rb, const T *vb0,
+                            const T *vb1, const T *vb2, PW<T> pw) {
+  const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
+  const T *vb[3] = {vb0, vb1, vb2};
+  GRID_STRIDE(c, n) {
+    const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+    const cplx<T> s = psi[c];
+    const T r = s.x * s.x + s.y * s.y, inv = (T)1 / (r + pw.eps_v);
+    T br = (T)2 * rb[c] * s.x, bi = (T)2 * rb[c] * s.y;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const cplx<T> pp = psi[nb.p[d]], pm = psi[nb.m[d]];
+      const T dr = (pp.x - pm.x) * pw.inv2h, di = (pp.y - pm.y) * pw.inv2h;
+      const T vbd = vb[d][c];
+      const T v = (s.x * di - s.y * dr) * pw.invM * inv;
+      // local: d v / d psi_r = (di/M - 2 psi_r v) / (rho + eps), d v / d psi_i = (-dr/M - 2 psi_i v) / (rho + eps)
+      br += vbd * (di * pw.invM - (T)2 * s.x * v) * inv;
+      bi += vbd * (-dr * pw.invM - (T)2 * s.y * v) * inv;
+      // through D_d psi: -D_d e_d, e_d = vb_d i psi invM / (rho + eps)
+      const T fp = vb[d][nb.p[d]] * pw.invM / (pp.x * pp.x + pp.y * pp.y + pw.eps_v);
+      const T fm = vb[d][nb.m[d]] * pw.invM / (pm.x * pm.x + pm.y * pm.y + pw.eps_v);
+      br -= (-(fp * pp.y) + fm * pm.y) * pw.inv2h;
+      bi -= (fp * pp.x - fm * pm.x) * pw.inv2h;
+    }
+    cplx<T> b = psib[c];
+    b.x += br;
+    b.y += bi;
+    psib[c] = b;
+  }
+}
+
 // External potential of a gaussian well, amp exp(-|r - r0|^2 / 2 w^2), with
 // r_i = (i - n/2) h (the grid convention of the reference drivers), and its
 // derivatives with respect to amp and w.
@@ -519,28 +661,34 @@ __device__ inline void block_sum2(double a, double b, double *pa, double *pb) {
   }
 }
 
-// veff = Re V_OT + V_ext - mu.
-template <typename T> __global__ void k_adj_veff(T *veff, const cplx<T> *vot, Well w, double mu, int64_t n) {
+// veff = Re V_OT + V_ext - mu and veffi = Im V_OT (backflow; may be null).
+template <typename T>
+__global__ void k_adj_veff(T *veff, T *veffi, const cplx<T> *vot, Well w, double mu, int64_t n) {
   GRID_STRIDE(c, n) {
     double v, da, dw;
     well_at(w, c, v, da, dw);
     veff[c] = (T)((double)vot[c].x + v - mu);
+    if (veffi) veffi[c] = vot[c].y;
   }
 }
 
-// e^{-i V dt/2} in FP64 for real V.
-__device__ inline double2 phase_of(double v, double dtr, double dti) {
+// e^{-i V dt/2} in FP64 for complex V = vr + i vi and complex dt = dtr + i dti.
+__device__ inline double2 phase_of(double vr, double vi, double dtr, double dti) {
   double s, c;
-  sincos(-0.5 * v * dtr, &s, &c);
-  const double m = exp(0.5 * v * dti);
+  sincos(-0.5 * (vr * dtr - vi * dti), &s, &c);
+  const double m = exp(0.5 * (vr * dti + vi * dtr));
   return make_double2(m * c, m * s);
+}
+template <typename T> __device__ inline double2 phase_at(const T *veff, const T *veffi, int64_t c, double dtr, double dti) {
+  return phase_of((double)veff[c], veffi ? (double)veffi[c] : 0.0, dtr, dti);
 }
 
 // out = phase(veff) * in
 template <typename T>
-__global__ void k_adj_phase(cplx<T> *out, const cplx<T> *in, const T *veff, double dtr, double dti, int64_t n) {
+__global__ void k_adj_phase(cplx<T> *out, const cplx<T> *in, const T *veff, const T *veffi, double dtr, double dti,
+                            int64_t n) {
   GRID_STRIDE(c, n) {
-    const double2 f = phase_of((double)veff[c], dtr, dti);
+    const double2 f = phase_at(veff, veffi, c, dtr, dti);
     const double ar = in[c].x, ai = in[c].y;
     out[c] = cmk<cplx<T>>((T)(f.x * ar - f.y * ai), (T)(f.x * ai + f.y * ar));
   }
@@ -549,9 +697,9 @@ __global__ void k_adj_phase(cplx<T> *out, const cplx<T> *in, const T *veff, doub
 // Adjoint of psi' = phi u2: u2bar = conj(phi) psibar', phibar = conj(u2) psibar'.
 template <typename T>
 __global__ void k_adj_out(cplx<T> *u2b, cplx<T> *phib, const cplx<T> *psibn, const cplx<T> *u2, const T *veff,
-                          double dtr, double dti, int64_t n) {
+                          const T *veffi, double dtr, double dti, int64_t n) {
   GRID_STRIDE(c, n) {
-    const double2 f = phase_of((double)veff[c], dtr, dti);
+    const double2 f = phase_at(veff, veffi, c, dtr, dti);
     const double br = psibn[c].x, bi = psibn[c].y, ur = u2[c].x, ui = u2[c].y;
     u2b[c] = cmk<cplx<T>>((T)(f.x * br + f.y * bi), (T)(f.x * bi - f.y * br));
     phib[c] = cmk<cplx<T>>((T)(ur * br + ui * bi), (T)(ur * bi - ui * br));
@@ -559,21 +707,28 @@ __global__ void k_adj_out(cplx<T> *u2b, cplx<T> *phib, const cplx<T> *psibn, con
 }
 
 // Adjoint of u1 = phi psi and of phi = e^{-i V dt/2}: psibar = conj(phi) u1bar,
-// phibar += conj(psi) u1bar, Vbar = Re(conj(dphi/dV) phibar) with
-// dphi/dV = -i dt/2 phi; per block: sum Vbar dV/damp and sum Vbar dV/dwidth.
+// phibar += conj(psi) u1bar, Vbar = Re(conj(dphi/dV) phibar) for V's real and
+// imaginary parts, dphi/dVr = -i dt/2 phi and dphi/dVi = dt/2 phi; per block:
+// sum Vbar_r dV/damp and sum Vbar_r dV/dwidth (V_ext is real).
 template <typename T>
-__global__ void k_adj_in(cplx<T> *psib, T *vbar, const cplx<T> *u1b, const cplx<T> *psi, const cplx<T> *phib,
-                         const T *veff, double dtr, double dti, Well w, double *pa, double *pw_, int64_t n) {
+__global__ void k_adj_in(cplx<T> *psib, T *vbar, T *vbari, const cplx<T> *u1b, const cplx<T> *psi, const cplx<T> *phib,
+                         const T *veff, const T *veffi, double dtr, double dti, Well w, double *pa, double *pw_,
+                         int64_t n) {
   double sa = 0.0, sw = 0.0;
   GRID_STRIDE(c, n) {
-    const double2 f = phase_of((double)veff[c], dtr, dti);
+    const double2 f = phase_at(veff, veffi, c, dtr, dti);
     const double br = u1b[c].x, bi = u1b[c].y, pr = psi[c].x, pi = psi[c].y;
     psib[c] = cmk<cplx<T>>((T)(f.x * br + f.y * bi), (T)(f.x * bi - f.y * br));
     const double qr = phib[c].x + (pr * br + pi * bi), qi = phib[c].y + (pr * bi - pi * br);
-    // dphi/dV = (-i dt / 2) phi = ((dti f.x + dtr f.y) + i (dti f.y - dtr f.x)) / 2
+    // dphi/dVr = (-i dt / 2) phi = ((dti f.x + dtr f.y) + i (dti f.y - dtr f.x)) / 2
     const double dr = 0.5 * (dti * f.x + dtr * f.y), di = 0.5 * (dti * f.y - dtr * f.x);
     const double vb = dr * qr + di * qi;
     vbar[c] = (T)vb;
+    if (vbari) {
+      // dphi/dVi = (dt / 2) phi = ((dtr f.x - dti f.y) + i (dti f.x + dtr f.y)) / 2
+      const double er = 0.5 * (dtr * f.x - dti * f.y), ei = 0.5 * (dti * f.x + dtr * f.y);
+      vbari[c] = (T)(er * qr + ei * qi);
+    }
     double v, da, dw;
     well_at(w, c, v, da, dw);
     sa += vb * da;
@@ -600,14 +755,17 @@ __global__ void k_adj_seed(cplx<T> *psib, const cplx<T> *kin, const cplx<T> *psi
   block_sum2(sa, sw, pa, pw_);
 }
 
-// spec *= K (kinetic symbol cx mx^2 + cy my^2 + cz mz^2)
+// spec *= K (kinetic symbol cx (mx - ox)^2 + cy (my - oy)^2 + cz (mz - oz)^2;
+// o: the momentum frame in units of 2 pi / L)
 template <typename T>
-__global__ void k_kin_mul(cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz) {
+__global__ void k_kin_mul(cplx<T> *spec, int nx, int ny, int nz, double cx, double cy, double cz, double ox, double oy,
+                          double oz) {
   const int64_t n = (int64_t)nx * ny * nz;
   GRID_STRIDE(c, n) {
     const int k = (int)(c % nz), j = (int)((c / nz) % ny), i = (int)(c / ((int64_t)nz * ny));
-    const int mx = i < nx / 2 ? i : i - nx, my = j < ny / 2 ? j : j - ny, mz = k < nz / 2 ? k : k - nz;
-    const double K = cx * (double)mx * mx + cy * (double)my * my + cz * (double)mz * mz;
+    const double mx = (double)(i < nx / 2 ? i : i - nx) - ox, my = (double)(j < ny / 2 ? j : j - ny) - oy,
+                 mz = (double)(k < nz / 2 ? k : k - nz) - oz;
+    const double K = cx * mx * mx + cy * my * my + cz * mz * mz;
     spec[c] = cmk<cplx<T>>((T)(K * spec[c].x), (T)(K * spec[c].y));
   }
 }
@@ -812,11 +970,11 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
       }
     if (!in.empty()) r2c3d<T>(p, (int)in.size(), in.data(), out.data(), s);
   }
-  // I2
+  // I2 (J_d and T1 into the slots F2 has consumed; A = U*rho in SA_ stays for the adjoint)
   if (KC) {
     const cplx<T> *in[3] = {CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3)};
     fft::Combine c[4];
-    T *out[4] = {S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ), S<T>(d, SA_)};
+    T *out[4] = {S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4)};
     for (int dd = 0; dd < 3; ++dd) c[dd] = cmb1(dd, K_F, -1, h3N);
     fft::Term t[3] = {{0, K_F, 0, h3N}, {1, K_F, 1, h3N}, {2, K_F, 2, h3N}};
     c[3] = cmb(3, t);
@@ -825,9 +983,9 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
   }
   // P3
   if (KC || BF) {
-    StageCIn ci{rho, S<T>(d, RT), S<T>(d, SA_), S<T>(d, SBX), S<T>(d, SBY), S<T>(d, SBZ), S<T>(d, PX),
+    StageCIn ci{rho, S<T>(d, RT), S<T>(d, Q4), S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, PX),
                 S<T>(d, PY), S<T>(d, PZ), S<T>(d, WX), S<T>(d, WY), S<T>(d, WZ)};
-    StageCOut co{S<T>(d, SC), S<T>(d, VRE), S<T>(d, VIM)};
+    StageCOut co{S<T>(d, TMP5), S<T>(d, VRE), S<T>(d, VIM)};
     if (KC && BF) LAUNCH_PW("ot_stage_kc_terms_bf_div", 17, (k_stageC<T, true, true>), ci, co, pw);
     else if (KC) LAUNCH_PW("ot_stage_kc_terms_bf_div", 12, (k_stageC<T, true, false>), ci, co, pw);
     else LAUNCH_PW("ot_stage_kc_terms_bf_div", 5, (k_stageC<T, false, true>), ci, co, pw);
@@ -835,7 +993,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
   // F3 + I3
   if (LOC || KC) {
     if (KC) {
-      const T *in[1] = {S<T>(d, SC)};
+      const T *in[1] = {S<T>(d, TMP5)};
       cplx<T> *out[1] = {CS<T>(d, SH)};
       r2c3d<T>(p, 1, in, out, s);
     }
@@ -1522,18 +1680,25 @@ bool check_r(dft_ot_functional *otf, rgrid *g, const char *fn) {
 
 
 // -------------------------------------------------------- adjoint sweep --
-// Work space of the reverse pass: real fields, complex fields and half spectra.
+// Work space of the reverse pass: real fields, complex fields and half spectra,
+// and for backflow models the fields of the backflow transpose.
 template <typename T> struct AdjWork {
   int64_t n = 0, nh = 0;
+  bool bf = false;
   T *r[24] = {};
+  T *b[20] = {};
   cplx<T> *c[4] = {};
-  cplx<T> *spec[6] = {};
+  cplx<T> *spec[10] = {};
   double *pa = nullptr, *pw = nullptr;
-  bool alloc(int64_t n_, int64_t nh_) {
+  bool alloc(int64_t n_, int64_t nh_, bool with_bf) {
     n = n_;
     nh = nh_;
+    bf = with_bf;
     for (auto &x : r)
       if (!(x = (T *)dmalloc(sizeof(T) * n))) return false;
+    if (bf)
+      for (auto &x : b)
+        if (!(x = (T *)dmalloc(sizeof(T) * n))) return false;
     for (auto &x : c)
       if (!(x = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n))) return false;
     for (auto &x : spec)
@@ -1544,6 +1709,7 @@ template <typename T> struct AdjWork {
   }
   void release() {
     for (auto &x : r) dfree(x);
+    for (auto &x : b) dfree(x);
     for (auto &x : c) dfree(x);
     for (auto &x : spec) dfree(x);
     dfree(pa);
@@ -1554,22 +1720,27 @@ template <typename T> struct AdjWork {
 // Real fields of the work space.
 enum { AQB, ALV, AHP, ARP, ARBB, AT1B, APB, AJB = APB + 3, AGB = AJB + 3, ARSB = AGB + 3, AGBAR, ACONV = AGBAR + 3,
        AVBAR, AVEFF, ARHO, AGRAD };
+// Backflow fields: Vi_bar, tb, imb, Abar, Bbar_d, gbar_d, vbar_d, rho_bar, U*Abar,
+// U*Bbar_d, U*Cbar, Im V_OT of the step.
+enum { BVBI, BTB, BIMB, BAB, BBB, BGB = BBB + 3, BVB = BGB + 3, BRB = BVB + 3, BUA, BUB, BUC = BUB + 3, BVIM };
 
-// out = sum_m K_m * in_m over real fields (r2c of the inputs, one c2r output per combine).
+// out = sum_m K_m * in_m over real fields (r2c of the inputs, one c2r output
+// per combine; at most 5 of each).
 template <typename T>
 void adj_conv(dft_ot_device *d, AdjWork<T> &w, int nin, const T *const *in, int nout, const fft::Combine *c,
               T *const *out) {
   Plan3D<T> &p = plan_of<T>(d);
   hipStream_t s = lib_stream();
-  cplx<T> *si[3] = {w.spec[0], w.spec[1], w.spec[2]};
+  cplx<T> *si[5] = {w.spec[0], w.spec[1], w.spec[2], w.spec[3], w.spec[4]};
   r2c3d<T>(p, nin, in, si, s);
-  const cplx<T> *ci[3] = {w.spec[0], w.spec[1], w.spec[2]};
-  cplx<T> *tmp[3] = {w.spec[3], w.spec[4], w.spec[5]};
+  const cplx<T> *ci[5] = {w.spec[0], w.spec[1], w.spec[2], w.spec[3], w.spec[4]};
+  cplx<T> *tmp[5] = {w.spec[5], w.spec[6], w.spec[7], w.spec[8], w.spec[9]};
   c2r3d<T>(p, nin, ci, nout, c, kerntab<T>(d), tmp, out, s);
 }
 
-// R = (dV_OT/drho)^T vbar at the state whose intermediates potential_impl left
-// in the slots (rho: the density it used).  psib += 2 R psi.
+// R = (dV_OT/drho)^T vbar (real parts: LJ, local, KC, HD) at the state whose
+// intermediates potential_impl left in the slots (rho: the density it used).
+// psib += 2 R psi, or rout = R when psib is null.
 template <typename T>
 void vjp_ot(dft_ot_functional *otf, const T *vbar, const T *rho, cplx<T> *psib, const cplx<T> *psi, T *rout,
             AdjWork<T> &w) {
@@ -1591,9 +1762,9 @@ void vjp_ot(dft_ot_functional *otf, const T *vbar, const T *rho, cplx<T> *psib, 
   a.lv = w.r[ALV];
   a.hp = w.r[AHP];
   a.rt = S<T>(d, RT);
-  a.t1 = S<T>(d, SA_);
+  a.t1 = S<T>(d, Q4);
   for (int k = 0; k < 3; ++k) {
-    a.J[k] = S<T>(d, SBX + k);
+    a.J[k] = S<T>(d, QX + k);
     a.Pd[k] = S<T>(d, PX + k);
     a.pb[k] = w.r[APB + k];
     a.jb[k] = w.r[AJB + k];
@@ -1631,8 +1802,52 @@ void vjp_ot(dft_ot_functional *otf, const T *vbar, const T *rho, cplx<T> *psib, 
   LAUNCH_PW("ot_adjoint_out", 4, (k_vjp_out<T>), rout, psib, psi, (const T *)w.r[ARP], (const T *)w.r[ACONV], d->n);
 }
 
+// Backflow part: psib += (dV_BF/dpsi)^T (vrb, vib), the adjoints of Re V and
+// Im V, at the state whose intermediates potential_impl left in the slots
+// (v_d, A, B_d, C, W_d; rho: the density it used, mass: the wave function's).
+template <typename T>
+void vjp_bf(dft_ot_functional *otf, const T *vrb, const T *vib, const T *rho, cplx<T> *psib, const cplx<T> *psi,
+            double mass, AdjWork<T> &w) {
+  dft_ot_device *d = otf->device;
+  const double h3N = d->step * d->step * d->step / (double)d->n;
+  PW<T> pw = make_pw<T>(otf, mass, otf->div_epsilon, true);
+  BfAdj a{};
+  a.vrb = vrb;
+  a.vib = vib;
+  a.rho = rho;
+  a.A = S<T>(d, SA_);
+  for (int k = 0; k < 3; ++k) {
+    a.v[k] = S<T>(d, VX + k);
+    a.W[k] = S<T>(d, WX + k);
+    a.Bb[k] = w.b[BBB + k];
+    a.gb[k] = w.b[BGB + k];
+    a.vb[k] = w.b[BVB + k];
+    a.uB[k] = w.b[BUB + k];
+  }
+  a.tb = w.b[BTB];
+  a.imb = w.b[BIMB];
+  a.Ab = w.b[BAB];
+  a.rb = w.b[BRB];
+  a.uA = w.b[BUA];
+  a.uC = w.b[BUC];
+  LAUNCH_PW("ot_adjoint_bf_a", 13, (k_bfadj_a<T>), a, (const T *)S<T>(d, SBX), (const T *)S<T>(d, SBY),
+            (const T *)S<T>(d, SBZ), (const T *)S<T>(d, SC), pw);
+  LAUNCH_PW("ot_adjoint_bf_b", 19, (k_bfadj_b<T>), a, pw);
+  {
+    const T *in[5] = {w.b[BAB], w.b[BBB], w.b[BBB + 1], w.b[BBB + 2], w.b[BTB]};
+    fft::Combine c[5];
+    for (int i = 0; i < 5; ++i) c[i] = cmb1(i, K_U, -1, h3N);
+    T *out[5] = {w.b[BUA], w.b[BUB], w.b[BUB + 1], w.b[BUB + 2], w.b[BUC]};
+    adj_conv<T>(d, w, 5, in, 5, c, out);
+  }
+  LAUNCH_PW("ot_adjoint_bf_c", 19, (k_bfadj_c<T>), a, pw);
+  LAUNCH_PW("ot_adjoint_bf_psi", 8, (k_bfadj_psi<T>), psib, psi, (const T *)w.b[BRB], (const T *)w.b[BVB],
+            (const T *)w.b[BVB + 1], (const T *)w.b[BVB + 2], pw);
+}
+
 // V_OT of psi into the complex field w.c[3] (the unfused path, leaving its
-// intermediates in the slots) and veff = Re V_OT + V_ext - mu.
+// intermediates in the slots), veff = Re V_OT + V_ext - mu and, for backflow,
+// Im V_OT in w.b[BVIM].
 template <typename T>
 void adj_potential(dft_ot_functional *otf, const cplx<T> *psi, double mass, const Well &wl, double mu, AdjWork<T> &w) {
   dft_ot_device *d = otf->device;
@@ -1646,43 +1861,92 @@ void adj_potential(dft_ot_functional *otf, const cplx<T> *psi, double mass, cons
   V.dev = w.c[3];
   V.dev_elems = d->n;
   V.prec = d->prec;
-  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, false};
+  PotParts parts{true, (otf->model & DFT_OT_KC) != 0, w.bf};
   potential_impl<T>(otf, &V, psi, mass, w.r[ARHO], nullptr, nullptr, nullptr, parts, false);
-  hipLaunchKernelGGL(k_adj_veff<T>, dim3(grid_for(d->n)), dim3(kBlock), 0, s, w.r[AVEFF], (const cplx<T> *)w.c[3], wl,
-                     mu, d->n);
+  hipLaunchKernelGGL(k_adj_veff<T>, dim3(grid_for(d->n)), dim3(kBlock), 0, s, w.r[AVEFF], w.bf ? w.b[BVIM] : nullptr,
+                     (const cplx<T> *)w.c[3], wl, mu, d->n);
   DFT_LAUNCH_CHECK();
 }
 
+// rb = h3 Re V, vb_d = -mcb h3 rho W_d: the gradient of the backflow energy
+// with respect to rho (at fixed v) and to v_d.
+template <typename T>
+__global__ void k_bfseed(T *rb, T *vb0, T *vb1, T *vb2, const cplx<T> *V, const T *rho, const T *W0, const T *W1,
+                         const T *W2, double h3, double mcb, int64_t n) {
+  GRID_STRIDE(c, n) {
+    rb[c] = (T)(h3 * (double)V[c].x);
+    const double f = -mcb * h3 * (double)rho[c];
+    vb0[c] = (T)(f * (double)W0[c]);
+    vb1[c] = (T)(f * (double)W1[c]);
+    vb2[c] = (T)(f * (double)W2[c]);
+  }
+}
+
+// psib += dE_BF/dpsi: the backflow energy of src/ot-energy.c:202-288 (velocity
+// with DFT_EPS, no clamps) depends on psi through rho and v:
+// psib += 2 h^3 Re V_BF psi + (dv/dpsi)^T vb, vb_d = -m c_bf h^3 rho W_d.
+template <typename T>
+void seed_bf(dft_ot_functional *otf, const cplx<T> *psi, double mass, cplx<T> *psib, AdjWork<T> &w) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  const double h3 = d->step * d->step * d->step;
+  const double keep[3] = {otf->div_epsilon, otf->max_veloc, otf->max_bfpot};
+  otf->div_epsilon = DFT_EPS;
+  otf->max_veloc = 999.0 / GRID_AUTOMPS;
+  otf->max_bfpot = 10000.0 / GRID_AUTOK;
+  const PW<T> pw = make_pw<T>(otf, mass, DFT_EPS, false);
+  DFT_HIP_CHECK(hipMemsetAsync(w.c[3], 0, sizeof(cplx<T>) * d->n, s));
+  cgrid V{};
+  V.nx = d->nx;
+  V.ny = d->ny;
+  V.nz = d->nz;
+  V.step = d->step;
+  V.dev = w.c[3];
+  V.dev_elems = d->n;
+  V.prec = d->prec;
+  PotParts parts{false, false, true};
+  potential_impl<T>(otf, &V, psi, mass, w.r[ARHO], nullptr, nullptr, nullptr, parts, false);
+  otf->div_epsilon = keep[0];
+  otf->max_veloc = keep[1];
+  otf->max_bfpot = keep[2];
+  LAUNCH_PW("ot_adjoint_bf_seed", 8, (k_bfseed<T>), w.b[BRB], w.b[BVB], w.b[BVB + 1], w.b[BVB + 2],
+            (const cplx<T> *)w.c[3], (const T *)w.r[ARHO], (const T *)S<T>(d, WX), (const T *)S<T>(d, WY),
+            (const T *)S<T>(d, WZ), h3, otf->mass * otf->c_bfpot, d->n);
+  LAUNCH_PW("ot_adjoint_bf_psi", 8, (k_bfadj_psi<T>), psib, psi, (const T *)w.b[BRB], (const T *)w.b[BVB],
+            (const T *)w.b[BVB + 1], (const T *)w.b[BVB + 2], pw);
+}
+
 // One step backwards: psib holds dJ/dpsi_{n+1} on entry and dJ/dpsi_n on exit;
-// th[0..1] accumulate dJ/d(amp, width) through V_ext of this step.
+// th[0..1] accumulate dJ/d(amp, width) through V_ext of this step.  k0: the
+// momentum frame of the kinetic propagator.
 template <typename T>
 void adj_step(dft_ot_functional *otf, const cplx<T> *psi, cplx<T> *psib, double mass, const Well &wl, double mu,
-              double dtr, double dti, AdjWork<T> &w, double th[2]) {
+              double dtr, double dti, const double *k0, AdjWork<T> &w, double th[2]) {
   dft_ot_device *d = otf->device;
   Plan3D<T> &p = plan_of<T>(d);
   hipStream_t s = lib_stream();
   const dim3 g(grid_for(d->n)), b(kBlock);
   adj_potential<T>(otf, psi, mass, wl, mu, w);
+  const T *vi = w.bf ? w.b[BVIM] : nullptr;
   // forward: u1 = phi psi, u2 = T u1
-  hipLaunchKernelGGL(k_adj_phase<T>, g, b, 0, s, w.c[0], psi, (const T *)w.r[AVEFF], dtr, dti, d->n);
+  hipLaunchKernelGGL(k_adj_phase<T>, g, b, 0, s, w.c[0], psi, (const T *)w.r[AVEFF], vi, dtr, dti, d->n);
   DFT_LAUNCH_CHECK();
-  strang<T>(p, w.c[0], nullptr, dtr, dti, mass, nullptr, s);
+  strang<T>(p, w.c[0], nullptr, dtr, dti, mass, nullptr, s, k0);
   // reverse: u2bar, phibar; u1bar = T^dagger u2bar (T at -conj(dt)); psibar, Vbar
   hipLaunchKernelGGL(k_adj_out<T>, g, b, 0, s, w.c[1], w.c[2], (const cplx<T> *)psib, (const cplx<T> *)w.c[0],
-                     (const T *)w.r[AVEFF], dtr, dti, d->n);
+                     (const T *)w.r[AVEFF], vi, dtr, dti, d->n);
   DFT_LAUNCH_CHECK();
-  strang<T>(p, w.c[1], nullptr, -dtr, dti, mass, nullptr, s);
+  strang<T>(p, w.c[1], nullptr, -dtr, dti, mass, nullptr, s, k0);
   hipLaunchKernelGGL(k_adj_in<T>, dim3((unsigned)red_blocks(d->n)), b, 0, s, psib, w.r[AVBAR],
-                     (const cplx<T> *)w.c[1], psi, (const cplx<T> *)w.c[2], (const T *)w.r[AVEFF], dtr, dti, wl, w.pa,
-                     w.pw, d->n);
+                     w.bf ? w.b[BVBI] : nullptr, (const cplx<T> *)w.c[1], psi, (const cplx<T> *)w.c[2],
+                     (const T *)w.r[AVEFF], vi, dtr, dti, wl, w.pa, w.pw, d->n);
   DFT_LAUNCH_CHECK();
   th[0] += sum_partials(w.pa, red_blocks(d->n), s);
   th[1] += sum_partials(w.pw, red_blocks(d->n), s);
   vjp_ot<T>(otf, w.r[AVBAR], w.r[ARHO], psib, psi, nullptr, w);
+  if (w.bf) vjp_bf<T>(otf, w.r[AVBAR], w.b[BVBI], w.r[ARHO], psib, psi, mass, w);
 }
 
-// Gradient of E[psi_K] with respect to (amp, width) of the well, for K single
-// steps of dft_ot_propagate from the state in gwf (left unchanged), by an
 // Progress lines of the long-running sweeps on stderr (dft_set_verbose).
 int g_verbose = 0;
 
@@ -1723,6 +1987,7 @@ int adjoint_sweep(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, do
   __real__ dt = dtr;
   __imag__ dt = dti;
   const double mass = gwf->mass;
+  const double k0[3] = {gwf->grid->kx0, gwf->grid->ky0, gwf->grid->kz0};
   // forward sweep
   DFT_HIP_CHECK(hipMemcpyAsync(cur->grid->dev, gwf->grid->dev, bytes, hipMemcpyDeviceToDevice, s));
   for (int64_t it = 0; it < K; ++it) {
@@ -1745,7 +2010,7 @@ int adjoint_sweep(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu, do
       if (it + 1 < n1) dft_ot_propagate(otf, cur, vext, mu, dt);
     }
     for (int64_t it = n1 - 1; it >= n0; --it)
-      adj_step<T>(otf, seg[it - n0], psib, mass, wl, mu, dtr, dti, w, th);
+      adj_step<T>(otf, seg[it - n0], psib, mass, wl, mu, dtr, dti, k0, w, th);
     if (g_verbose) fprintf(stderr, "libdft: adjoint reverse segment %lld / %lld done\n", (long long)(ncp - j), (long long)ncp);
   }
   cleanup();
@@ -1759,9 +2024,10 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
   dft_ot_device *d = otf->device;
   hipStream_t s = lib_stream();
   const int64_t n = d->n;
+  const bool BF = (otf->model & DFT_OT_BACKFLOW) != 0;
   AdjWork<T> w;
   cplx<T> *psib = nullptr;
-  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP) && (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
+  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP, BF) && (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
   double th[2] = {0.0, 0.0};
   auto seed = [&](wf *cur, cplx<T> *pb, double t2[2]) {
     // the gradient of the total energy with respect to psi_K
@@ -1774,7 +2040,10 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
     const double cx = std::pow(2.0 * M_PI / (d->nx * d->step), 2) / (2.0 * gwf->mass);
     const double cy = std::pow(2.0 * M_PI / (d->ny * d->step), 2) / (2.0 * gwf->mass);
     const double cz = std::pow(2.0 * M_PI / (d->nz * d->step), 2) / (2.0 * gwf->mass);
-    hipLaunchKernelGGL(k_kin_mul<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, w.c[0], d->nx, d->ny, d->nz, cx, cy, cz);
+    const double o[3] = {cur->grid->kx0 * d->nx * d->step / (2.0 * M_PI), cur->grid->ky0 * d->ny * d->step / (2.0 * M_PI),
+                         cur->grid->kz0 * d->nz * d->step / (2.0 * M_PI)};
+    hipLaunchKernelGGL(k_kin_mul<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, w.c[0], d->nx, d->ny, d->nz, cx, cy, cz,
+                       o[0], o[1], o[2]);
     DFT_LAUNCH_CHECK();
     c2c_inv<T>(p, w.c[0], w.c[1], s);
     hipLaunchKernelGGL(k_adj_seed<T>, dim3((unsigned)red_blocks(n)), dim3(kBlock), 0, s, pb, (const cplx<T> *)w.c[1],
@@ -1782,6 +2051,8 @@ int adjoint_well_impl(dft_ot_functional *otf, wf *gwf, const Well &wl, double mu
     DFT_LAUNCH_CHECK();
     t2[0] = h3 * sum_partials(w.pa, red_blocks(n), s);
     t2[1] = h3 * sum_partials(w.pw, red_blocks(n), s);
+    // the backflow energy also depends on psi through the velocity
+    if (BF) seed_bf<T>(otf, psiK, gwf->mass, pb, w);
     *E = grid_wf_energy(cur, nullptr) + dft_ot_energy(otf, cur) + wl.amp * t2[0];
   };
   const int rc = ok ? adjoint_sweep<T>(otf, gwf, wl, mu, dtr, dti, K, every, w, psib, th, seed) : -1;
@@ -1873,7 +2144,8 @@ int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0,
   DFT_LAUNCH_CHECK();
   AdjWork<T> w;
   cplx<T> *psib = nullptr;
-  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP) && (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
+  bool ok = w.alloc(n, (int64_t)d->nx * d->ny * d->PP, (otf->model & DFT_OT_BACKFLOW) != 0) &&
+            (psib = (cplx<T> *)dmalloc(sizeof(cplx<T>) * n));
   double th[2] = {0.0, 0.0}, explicit_term = 0.0;
   auto seed = [&](wf *cur, cplx<T> *pb, double *) {
     const cplx<T> *psiK = (const cplx<T> *)cur->grid->dev;
@@ -1908,7 +2180,7 @@ int adjoint_cavity_impl(dft_ot_functional *otf, wf *gwf, double rho0, double R0,
 template <typename T> bool potential_vjp_impl(dft_ot_functional *otf, T *rho_bar, const T *v_bar, wf *gwf) {
   dft_ot_device *d = otf->device;
   AdjWork<T> w;
-  if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->PP)) {
+  if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->PP, false)) {
     fprintf(stderr, "libdft: dft_ot_potential_vjp: out of device memory\n");
     w.release();
     return false;
@@ -1917,6 +2189,41 @@ template <typename T> bool potential_vjp_impl(dft_ot_functional *otf, T *rho_bar
   adj_potential<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, none, 0.0, w);
   vjp_ot<T>(otf, v_bar, w.r[ARHO], nullptr, nullptr, rho_bar, w);
   DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  w.release();
+  return true;
+}
+
+// re = Re c, im = Im c
+template <typename T> __global__ void k_split(T *re, T *im, const cplx<T> *c, int64_t n) {
+  GRID_STRIDE(i, n) {
+    re[i] = c[i].x;
+    if (im) im[i] = c[i].y;
+  }
+}
+
+// Standalone reverse pass of the complex V_OT with respect to psi (every model):
+// psi_bar = (dV/dpsi)^T v_bar, with Re v_bar the adjoint of Re V and Im v_bar
+// that of Im V (backflow), in the convention dJ = Re sum conj(psi_bar) dpsi.
+template <typename T> bool potential_vjp_psi_impl(dft_ot_functional *otf, cplx<T> *psib, const cplx<T> *vb, wf *gwf) {
+  dft_ot_device *d = otf->device;
+  hipStream_t s = lib_stream();
+  const bool BF = (otf->model & DFT_OT_BACKFLOW) != 0;
+  AdjWork<T> w;
+  if (!w.alloc(d->n, (int64_t)d->nx * d->ny * d->PP, BF)) {
+    fprintf(stderr, "libdft: dft_ot_potential_vjp_psi: out of device memory\n");
+    w.release();
+    return false;
+  }
+  const cplx<T> *psi = (const cplx<T> *)gwf->grid->dev;
+  Well none{0.0, 1.0, 0.0, 0.0, 0.0, d->step, d->nx, d->ny, d->nz};
+  adj_potential<T>(otf, psi, gwf->mass, none, 0.0, w);
+  hipLaunchKernelGGL(k_split<T>, dim3(grid_for(d->n)), dim3(kBlock), 0, s, w.r[AVBAR], BF ? w.b[BVBI] : nullptr, vb,
+                     d->n);
+  DFT_LAUNCH_CHECK();
+  DFT_HIP_CHECK(hipMemsetAsync(psib, 0, sizeof(cplx<T>) * d->n, s));
+  vjp_ot<T>(otf, w.r[AVBAR], w.r[ARHO], psib, psi, nullptr, w);
+  if (BF) vjp_bf<T>(otf, w.r[AVBAR], w.b[BVBI], w.r[ARHO], psib, psi, gwf->mass, w);
+  DFT_HIP_CHECK(hipStreamSynchronize(s));
   w.release();
   return true;
 }
@@ -2223,10 +2530,6 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
   if (!check_wf(otf, gwf, "dft_ot_adjoint_well")) return -1;
-  if (otf->model & DFT_OT_BACKFLOW) {
-    fprintf(stderr, "libdft: dft_ot_adjoint_well: the adjoint covers the PLAIN and KC models (no backflow)\n");
-    return -1;
-  }
   if (nsteps < 1 || every < 1 || width <= 0.0 || !energy || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_well: invalid arguments\n");
     return -1;
@@ -2247,10 +2550,6 @@ void dft_set_verbose(int level) { g_verbose = level; }
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
   if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity")) return -1;
-  if (otf->model & DFT_OT_BACKFLOW) {
-    fprintf(stderr, "libdft: dft_ot_adjoint_cavity: the adjoint covers the PLAIN and KC models (no backflow)\n");
-    return -1;
-  }
   if (nsteps < 1 || every < 1 || width <= 0.0 || rho0 <= 0.0 || !bubble_radius || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_cavity: invalid arguments\n");
     return -1;
@@ -2274,6 +2573,16 @@ int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, w
   }
   bool ok = false;
   DFT_BY_PREC(otf->device->prec, ok = potential_vjp_impl<T>(otf, (T *)rho_bar->dev, (const T *)v_bar->dev, gwf));
+  return ok ? 0 : -1;
+}
+
+int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_bar, wf *gwf) {
+  if (!check_wf(otf, gwf, "dft_ot_potential_vjp_psi") || !check_c(otf, psi_bar, "dft_ot_potential_vjp_psi") ||
+      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi"))
+    return -1;
+  bool ok = false;
+  DFT_BY_PREC(otf->device->prec, ok = potential_vjp_psi_impl<T>(otf, (cplx<T> *)psi_bar->dev,
+                                                                (const cplx<T> *)v_bar->dev, gwf));
   return ok ? 0 : -1;
 }
 
