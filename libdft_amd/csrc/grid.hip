@@ -516,7 +516,10 @@ cgrid *cgrid_clone(cgrid *src, char *id) {
   g_prec = src->prec;
   cgrid *g = cgrid_alloc(src->nx, src->ny, src->nz, src->step, src->value_outside, src->outside_params_ptr, id);
   g_prec = save;
-  if (g) cgrid_set_origin(g, src->x0, src->y0, src->z0);
+  if (g) {
+    cgrid_set_origin(g, src->x0, src->y0, src->z0);
+    cgrid_set_momentum(g, src->kx0, src->ky0, src->kz0);
+  }
   return g;
 }
 
