@@ -215,6 +215,15 @@ REAL_COMPLEX cgrid_integral_of_conjugate_product(cgrid *a, cgrid *b);
 REAL grid_wf_potential_energy(wf *gwf, cgrid *potential);
 REAL grid_wf_norm(wf *gwf);
 void grid_wf_normalize(wf *gwf);
+/* grid_wf_normalize without allocation or host synchronisation (no counterpart
+   in the reference): for callers that cannot allocate, such as code under HIP
+   graph capture or tight stepping loops.  psi *= sqrt(gwf->norm / N), with
+   N = h^3 sum |psi|^2 summed in FP64 on the device, all on the library
+   stream.  Its partial sums live in a fixed device array of 1026 doubles per
+   GPU, held until the process ends and shared by every call on that GPU:
+   calls that overlap on one GPU race.  Returns 0, or -1 (message on stderr)
+   when there is no wave function or no usable device. */
+int grid_wf_normalize_noalloc(wf *gwf);
 /* Kinetic energy h^3/N sum_k ((k - k0)^2/2M)|psi_k|^2, plus h^3 sum |psi|^2 Re V
    when potential is not NULL. */
 REAL grid_wf_energy(wf *gwf, cgrid *potential);

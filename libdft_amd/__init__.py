@@ -105,6 +105,7 @@ _SIGS = {
     "rgrid_integral": ("d", ["rg"]), "rgrid_integral_of_product": ("d", ["rg", "rg"]),
     "grid_wf_potential_energy": ("d", ["wf", "cg"]), "grid_wf_norm": ("d", ["wf"]), "grid_wf_normalize": (None, ["wf"]),
     "grid_wf_energy": ("d", ["wf", "cg"]),
+    "grid_wf_normalize_noalloc": ("i", ["wf"]),
     "grid_wf_propagate": (None, ["wf", "cg", "z"]),
     "grid_wf_propagate_predict": (None, ["wf", "wf", "cg", "z"]),
     "grid_wf_propagate_correct": (None, ["wf", "cg", "z"]),
