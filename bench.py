@@ -116,22 +116,36 @@ class Run:
         self.w.free()
 
 
-def cpu_baseline(n, step, psi0, dt):
-    """One step of the same workload on the FP64 oracle (OpenMP over the host cores).
-    Returns the baseline record and the oracle's total energy after that step."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, step, psi0, dt, timed=3):
+    """The same workload on the FP64 oracle (OpenMP over the host cores): one
+    warm-up step, then `timed` steps timed with the wall clock.  Returns the
+    baseline record and the oracle's total energy after the first step."""
     from oracle import pyoracle as po
     o = po.OracleOT(9, (n, n, n), step, 1, 1)   # kernel sampling cost is setup, not part of a step
     psi = psi0.copy()
     mu = o.bulk("dEdRho", o.p.rho0)
-    t0 = time.perf_counter()
     o.single_step(psi, None, mu, dt)
-    sec = time.perf_counter() - t0
     e_oracle = o.observables(psi)["energy"]
+    t0 = time.perf_counter()
+    for _ in range(timed):
+        o.single_step(psi, None, mu, dt)
+    sec = (time.perf_counter() - t0) / timed
     o.close()
     return ({"value": 1.0 / sec, "unit": "steps/s", "cores": po.num_threads(), "kind": "port",
-             "sample": "1 full-OT single step of this workload (%d^3, FP64) on the C/OpenMP oracle; "
-                       "kernels sampled once per cell (setup cost excluded, step cost unchanged); %.1f s"
-                       % (n, sec)}, e_oracle)
+             "cpu": cpu_model(), "nproc": os.cpu_count(),
+             "sample": "%d full-OT single steps of this workload (%d^3, FP64) after 1 warm-up step, on the "
+                       "C/OpenMP oracle; kernels sampled once per cell (setup cost excluded, step cost "
+                       "unchanged); %.2f s per step" % (timed, n, sec)}, e_oracle)
 
 
 def copy_calibration(n, step, reps=3):
@@ -241,9 +255,9 @@ def size_sweep(steps=10, warmup=3):
     return out
 
 
-def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
+def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2, model=9):
     """The 512^3 released-cavity workload (h = 0.4 A, 17 A cavity with a 0.5 A wall, 2 fs
-    steps; KC functional, the adjoint's scope): bubble radius (missing volume within a
+    steps; full OT): bubble radius (missing volume within a
     25.5 A window, as dft_ot_adjoint_cavity defines it) after `steps` steps and its
     derivative with respect to the initial cavity radius from the checkpointed adjoint sweep
     (ceil(steps/every) + every stored states), against central differences of two forward runs."""
@@ -252,7 +266,7 @@ def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
     R0, wd, dt = 17.0 / L.AUTOANG, 0.5 / L.AUTOANG, 2.0 / L.AUTOFS
     lib = L.lib()
     w = L.Wave((n, n, n), step, prec)
-    f = L.Functional(1, w)
+    f = L.Functional(model, w)
     mu = f.bulk("dEdRho", rho0)
     log("512^3 cavity adjoint: %d steps, checkpoints every %d" % (steps, every))
     lib.dft_set_verbose(1)  # a progress line per checkpoint
@@ -285,7 +299,7 @@ def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2):
     t_fwd = (time.perf_counter() - t0) / 2
     f.free(), w.free()
     states = -(-steps // every) + every
-    return {"workload": "released 17 A cavity, 512^3, h = 0.4 A, KC functional", "dtype": "f32" if prec == L.GRID_FP32
+    return {"workload": "released 17 A cavity, 512^3, h = 0.4 A, model %d" % model, "dtype": "f32" if prec == L.GRID_FP32
             else "f64", "steps": steps, "every": every, "stored_states": states,
             "stored_GB": states * n ** 3 * (8 if prec == L.GRID_FP32 else 16) / 1e9,
             "R_b_A": J * L.AUTOANG, "dRb_dR0_adjoint": g, "dRb_dR0_fd": fd, "rel_err": abs(g / fd - 1),

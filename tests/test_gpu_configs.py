@@ -168,7 +168,14 @@ def test_vortex256_circulation_and_conservation():
         ph = np.angle(psi[ix, iy, n // 2])
         dph = np.angle(np.exp(1j * np.diff(ph)))
         circ = dph.sum() / L.HELIUM_MASS
-        res[(prec, rounded)] = (_energy(lib, f, w), lib.grid_wf_norm(w.h), E0, N0, circ)
+        # the core: centroid of the density deficit within 3 A of the axis in the mid plane
+        rho = np.abs(psi[:, :, n // 2]) ** 2
+        xs = (np.arange(n) - n // 2) * step
+        X, Y = np.meshgrid(xs, xs, indexing="ij")
+        near = X ** 2 + Y ** 2 < (3.0 / L.AUTOANG) ** 2
+        dfc = np.where(near, np.clip(f.p.rho0 - rho, 0.0, None), 0.0)
+        core = (np.sum(dfc * X) / dfc.sum(), np.sum(dfc * Y) / dfc.sum())
+        res[(prec, rounded)] = (_energy(lib, f, w), lib.grid_wf_norm(w.h), E0, N0, circ, core, rho)
         _log("prec %d rounded-start %d: E0 %.10f E %.10f N %.10f circulation %.6f (quantum %.6f)" % (
             prec, rounded, E0 * L.AUTOK, res[(prec, rounded)][0] * L.AUTOK, res[(prec, rounded)][1], circ,
             2 * np.pi / L.HELIUM_MASS))
@@ -176,12 +183,20 @@ def test_vortex256_circulation_and_conservation():
     quantum = 2 * np.pi / L.HELIUM_MASS
     for k in res:
         assert abs(res[k][4] / quantum - 1) < 1e-9
-    E64, N64, E064, N064, _ = res[(L.GRID_FP64, True)]
+    E64, N64, E064, N064, _, c64, rho64 = res[(L.GRID_FP64, True)]
     E64x = res[(L.GRID_FP64, False)][0]
-    E32, N32, _, _, _ = res[(L.GRID_FP32, True)]
-    _log("FP32 vs FP64 energy after 500 steps: %.3e; FP64 sensitivity to rounding psi0 once: %.3e" % (
-        abs(E32 / E64 - 1), abs(E64x / E64 - 1)))
+    E32, N32, _, _, _, c32, rho32 = res[(L.GRID_FP32, True)]
+    dcore = np.hypot(c32[0] - c64[0], c32[1] - c64[1]) * L.AUTOANG
+    drho = np.abs(rho32 - rho64).max() / np.abs(rho64).max()
+    _log("FP32 vs FP64 after 500 steps: energy %.3e, norm %.3e, core position %.3e A (core at %.4f, %.4f A), "
+         "mid-plane density %.3e; FP64 sensitivity to rounding psi0 once: %.3e" % (
+             abs(E32 / E64 - 1), abs(N32 / N64 - 1), dcore, c64[0] * L.AUTOANG, c64[1] * L.AUTOANG, drho,
+             abs(E64x / E64 - 1)))
     assert abs(E32 / E64 - 1) < 1e-6
+    assert abs(N32 / N64 - 1) < 1e-6
+    # the dynamics themselves: where the core is and the density around it
+    assert dcore < 1e-3
+    assert drho < 1e-4
     # the backflow potential's imaginary part changes the norm (1.2e-3 over 500 steps)
     assert abs(N64 / N064 - 1) < 5e-3
 
