@@ -209,6 +209,47 @@ def summarize_kernel_stats(stats_csv, top=16):
     return "\n".join(lines)
 
 
+def summarize_counters(csvs, kernels=None):
+    """Markdown table of per-dispatch counter averages from rocprofv3 --pmc
+    counter_collection.csv files (one pass each), one row per kernel of the
+    256^3 FP32 step, with derived ratios: wave lifetime (SQ_WAVE_CYCLES /
+    SQ_WAVES, in cycles of the SQ clock per wave), the share of wave cycles
+    spent waiting on any instruction dependency (SQ_WAIT_INST_ANY), issuing
+    VALU, VMEM and LDS instructions, SQ_LEVEL_WAVES / SQ_BUSY_CYCLES (the
+    resident-wave level per busy SQ cycle, proportional to occupancy) and
+    the L2 hit rate."""
+    import csv
+    import collections
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in csvs:
+        for r in csv.DictReader(open(path)):
+            vals[_kernel_symbol(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    wanted = kernels or [SYMBOL[k] for k in SYMBOL]
+    cols = [("wave cycles", lambda a: a["SQ_WAVE_CYCLES"] / a["SQ_WAVES"]),
+            ("wait %", lambda a: 100 * a["SQ_WAIT_INST_ANY"] / a["SQ_WAVE_CYCLES"]),
+            ("VALU %", lambda a: 100 * a["SQ_ACTIVE_INST_VALU"] / a["SQ_WAVE_CYCLES"]),
+            ("VMEM %", lambda a: 100 * a["SQ_ACTIVE_INST_VMEM"] / a["SQ_WAVE_CYCLES"]),
+            ("LDS %", lambda a: 100 * a["SQ_ACTIVE_INST_LDS"] / a["SQ_WAVE_CYCLES"]),
+            ("level/busy", lambda a: a["SQ_LEVEL_WAVES"] / a["SQ_BUSY_CYCLES"]),
+            ("VALU insts/wave", lambda a: a["SQ_INSTS_VALU"] / a["SQ_WAVES"]),
+            ("VMEM rd/wave", lambda a: a["SQ_INSTS_VMEM_RD"] / a["SQ_WAVES"]),
+            ("LDS insts/wave", lambda a: a["SQ_INSTS_LDS"] / a["SQ_WAVES"]),
+            ("L2 hit %", lambda a: 100 * a["TCC_HIT_sum"] / (a["TCC_HIT_sum"] + a["TCC_MISS_sum"]))]
+    lines = ["| kernel | " + " | ".join(c for c, _ in cols) + " |", "|---" * (len(cols) + 1) + "|"]
+    for k in wanted:
+        if k not in vals:
+            continue
+        a = {c: float(np.mean(v)) for c, v in vals[k].items()}
+        cells = []
+        for _, f in cols:
+            try:
+                cells.append("%.1f" % f(a))
+            except (KeyError, ZeroDivisionError):
+                cells.append("")
+        lines.append("| `%s` | " % k + " | ".join(cells) + " |")
+    return "\n".join(lines)
+
+
 def measured_traffic(label, n):
     """HBM bytes per dispatch of a profiled launch label from TRAFFIC_FILE (256^3 FP32 only)."""
     sym = SYMBOL.get(label)
@@ -343,7 +384,12 @@ def main():
                     help="print the HBM traffic table from two rocprofv3 --pmc counter_collection.csv files and exit")
     ap.add_argument("--kernel-stats", metavar="STATS_CSV",
                     help="print a rocprofv3 kernel_stats.csv as a table and exit")
+    ap.add_argument("--counters", nargs="+", metavar="COUNTER_CSV",
+                    help="print per-kernel counter ratios from rocprofv3 --pmc counter_collection.csv files and exit")
     args = ap.parse_args()
+    if args.counters:
+        print(summarize_counters(args.counters))
+        return
     if args.traffic:
         print(summarize_traffic(*args.traffic))
         return

@@ -11,6 +11,8 @@
  *   dft_ot_energy_density src/ot-energy.c:38
  *   dft_ot_energy_density_kc src/ot-energy.c:123, _bf src/ot-energy.c:202
  *   dft_ot_backflow_potential src/ot.c:840  dft_ot_init_params       src/ot.c:1002
+ *   dft_ot_add_lennard_jones_potential src/ot.c:524
+ *   dft_ot_add_local_correlation_potential src/ot.c:547
  *   dft_ot_bulk_*         src/helium-ot-bulk.c:21-463
  * and the libgrid subset they and their callers use (libgrid is not part of
  * the reference tree; call sites are cited per function).
@@ -299,6 +301,14 @@ void dft_ot_potential2(dft_ot_functional *otf, cgrid *potential, rgrid *density,
 void dft_ot_backflow_potential(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *veloc_x,
                                rgrid *veloc_y, rgrid *veloc_z, rgrid *workspace1, rgrid *workspace2,
                                rgrid *workspace3, rgrid *workspace4, rgrid *workspace5, rgrid *workspace6);
+/* src/ot.c:524: potential += V_LJ * density.  Unlike the reference, workspace1 does
+ * not receive FFT(density): the workspaces are accepted and not written. */
+void dft_ot_add_lennard_jones_potential(dft_ot_functional *otf, cgrid *potential, rgrid *density,
+                                        rgrid *workspace1, rgrid *workspace2);
+/* src/ot.c:547: potential += the local correlation terms of rhobar = Pi * rho.  rho_tf
+ * (FFT(rho) in the reference) and the workspaces are accepted and unused. */
+void dft_ot_add_local_correlation_potential(dft_ot_functional *otf, cgrid *potential, rgrid *rho, rgrid *rho_tf,
+                                            rgrid *workspace1, rgrid *workspace2);
 void dft_ot_energy_density(dft_ot_functional *otf, rgrid *energy_density, wf *gwf);
 void dft_ot_energy_density_kc(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density);
 void dft_ot_energy_density_bf(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density);

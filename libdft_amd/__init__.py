@@ -114,6 +114,8 @@ _SIGS = {
     "dft_ot_potential": (None, ["otf", "cg", "wf"]),
     "dft_ot_potential2": (None, ["otf", "cg", "rg", "rg", "rg", "rg"]),
     "dft_ot_backflow_potential": (None, ["otf", "cg", "rg", "rg", "rg", "rg"] + ["rg"] * 6),
+    "dft_ot_add_lennard_jones_potential": (None, ["otf", "cg", "rg", "rg", "rg"]),
+    "dft_ot_add_local_correlation_potential": (None, ["otf", "cg", "rg", "rg", "rg", "rg"]),
     "dft_ot_energy_density": (None, ["otf", "rg", "wf"]),
     "dft_ot_energy_density_kc": (None, ["otf", "rg", "wf", "rg"]),
     "dft_ot_energy_density_bf": (None, ["otf", "rg", "wf", "rg"]),

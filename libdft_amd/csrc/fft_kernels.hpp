@@ -57,9 +57,14 @@ __device__ __forceinline__ void split_store(const Regs<T, N, EM> &v, cplx<T> *X,
 }
 
 // Packed pair Z(m) = A(m) + i B(m) from rows 2p, 2p+1 of a half-spectrum field
-// (Hermitian extension for m > N/2).
+// (Hermitian extension for m > N/2), in two parts: load_raw issues the loads of
+// both rows, merge_raw forms Z once they have arrived, so a kernel can issue
+// the next stage's loads before the current stage's transform.
+template <typename T, int N, int EM> struct RawPair {
+  cplx<T> a[RP<N, EM>::E], b[RP<N, EM>::E];
+};
 template <typename T, int N, int EM>
-__device__ __forceinline__ void load_merge(Regs<T, N, EM> &v, const void *in, uint32_t p, int P, int t) {
+__device__ __forceinline__ void load_raw(RawPair<T, N, EM> &r, const void *in, uint32_t p, int P, int t) {
   using C = cplx<T>;
   constexpr int E = RP<N, EM>::E, TPL = RP<N, EM>::TPL;
   // point m = t + TPL*i: row index m for i < E/2, N - m = (N - t) - TPL*i for i >= E/2
@@ -68,20 +73,34 @@ __device__ __forceinline__ void load_merge(Regs<T, N, EM> &v, const void *in, ui
   const uint32_t ma = ra + (uint32_t)(N - t), mb = rb + (uint32_t)(N - t);
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    C a, b;
     if (i < E / 2) {
-      a = lde_once<C>(in, fa, TPL * i);
-      b = lde_once<C>(in, fb, TPL * i);
+      r.a[i] = lde_once<C>(in, fa, TPL * i);
+      r.b[i] = lde_once<C>(in, fb, TPL * i);
     } else {
-      a = lde_once<C>(in, ma, -TPL * i);
-      b = lde_once<C>(in, mb, -TPL * i);
-      if (i > E / 2 || t > 0) {  // m > N/2: conjugate of the mirrored point
-        a.y = -a.y;
-        b.y = -b.y;
-      }
+      r.a[i] = lde_once<C>(in, ma, -TPL * i);
+      r.b[i] = lde_once<C>(in, mb, -TPL * i);
+    }
+  }
+}
+template <typename T, int N, int EM>
+__device__ __forceinline__ void merge_raw(Regs<T, N, EM> &v, const RawPair<T, N, EM> &r, int t) {
+  using C = cplx<T>;
+  constexpr int E = RP<N, EM>::E;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    C a = r.a[i], b = r.b[i];
+    if (i > E / 2 || (i == E / 2 && t > 0)) {  // m > N/2: conjugate of the mirrored point
+      a.y = -a.y;
+      b.y = -b.y;
     }
     v[i] = cmk<C>(a.x - b.y, a.y + b.x);
   }
+}
+template <typename T, int N, int EM>
+__device__ __forceinline__ void load_merge(Regs<T, N, EM> &v, const void *in, uint32_t p, int P, int t) {
+  RawPair<T, N, EM> r;
+  load_raw<T, N, EM>(r, in, p, P, t);
+  merge_raw<T, N, EM>(v, r, t);
 }
 
 // ------------------------------------------------ z pass: real -> half spec --
@@ -92,7 +111,7 @@ __global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, co
   using G = LineCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
   __shared__ C s_x[G::LINES * LP];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   const void *in = b.in[blockIdx.y];
   void *out = b.out[blockIdx.y];
   const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -117,7 +136,7 @@ __global__ void __launch_bounds__(256)
   using G = LineCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
   __shared__ C s_x[G::LINES * LP];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   const void *in = b.in[blockIdx.y];
   void *out = b.out[blockIdx.y];
   const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -159,7 +178,7 @@ __global__ void __launch_bounds__((StrCfg<T, N>::BS))
   using G = StrCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   const void *in = b.in[blockIdx.y];
   void *out = b.out[blockIdx.y];
   const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
@@ -190,7 +209,7 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
   using G = ColCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   const Combine cb = cs.c[blockIdx.y];
   void *out = b.out[blockIdx.y];
   const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
@@ -252,7 +271,7 @@ __global__ void __launch_bounds__(256)
   using G = LineCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? G::LINES * LP : 1];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const uint32_t l0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = l0 < nlines;
@@ -305,18 +324,22 @@ __global__ void __launch_bounds__(256)
 #ifndef DFT_XKIN_MINB
 #define DFT_XKIN_MINB 3
 #endif
+// the same request for the FP64 pass (0: none)
+#ifndef DFT_XKIN_MINB64
+#define DFT_XKIN_MINB64 0
+#endif
 template <typename T, int N>
 using KinCfg = typename std::conditional<(N >= 512), ColCfg<T, N>, ColCfg<T, N, 16, DFT_XKIN_COLS>>::type;
 
 template <typename T, int N>
-__global__ void __launch_bounds__((KinCfg<T, N>::BS), DFT_XKIN_MINB)
+__global__ void __launch_bounds__((KinCfg<T, N>::BS), sizeof(T) == 4 ? DFT_XKIN_MINB : DFT_XKIN_MINB64)
     k_xkin(cplx<T> *psi, int ny, int nz, int ntile, const cplx<T> *__restrict__ g_tw, double cx, double cy, double cz,
            double dtr, double dti, double ox, double oy, double oz) {
   using C = cplx<T>;
   using G = KinCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   __shared__ double2 s_kx[N];
   const int o = blockIdx.x / ntile, c0 = (blockIdx.x - o * ntile) * COLS;
   const int col = threadIdx.x % COLS, t = threadIdx.x / COLS;
@@ -353,7 +376,7 @@ __global__ void __launch_bounds__((KinCfg<T, N>::BS), DFT_XKIN_MINB)
     const double pr = x.x * yr - x.y * yi, pi = x.x * yi + x.y * yr;
     if constexpr (sizeof(T) == 4) {
       const float hr = (float)pr, hi = (float)pi;
-      v[i] = cmul_hl(v[i], make_float2(hr, hi), make_float2((float)(pr - hr), (float)(pi - hi)));
+      v[i] = cmul_hl_phase(v[i], make_float2(hr, hi), make_float2((float)(pr - hr), (float)(pi - hi)));
     } else {
       v[i] = cmul(v[i], cmk<C>(pr, pi));
     }

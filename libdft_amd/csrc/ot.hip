@@ -868,8 +868,11 @@ template <typename T> PW<T> make_pw(dft_ot_functional *otf, double wf_mass, doub
   } while (0)
 
 // Which parts of the potential to evaluate.
+// loc: local correlation (Pi terms), kc, bf; lj: the Lennard-Jones term
+// (-1: with loc, 0 / 1: without / with).
 struct PotParts {
   bool loc, kc, bf;
+  int lj = -1;
 };
 
 // Shared body of dft_ot_potential / potential2 / backflow_potential.
@@ -881,7 +884,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
   Plan3D<T> &p = plan_of<T>(d);
   hipStream_t s = lib_stream();
   const double h3N = d->step * d->step * d->step / (double)d->n;
-  const bool KC = parts.kc, BF = parts.bf, LOC = parts.loc;
+  const bool KC = parts.kc, BF = parts.bf, LOC = parts.loc, LJ = parts.lj < 0 ? LOC : parts.lj != 0;
   fft::KernTab kt = kerntab<T>(d);
   PW<T> pw = make_pw<T>(otf, wf_mass, otf->div_epsilon, true);
 
@@ -991,7 +994,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     else LAUNCH_PW("ot_stage_kc_terms_bf_div", 5, (k_stageC<T, false, true>), ci, co, pw);
   }
   // F3 + I3
-  if (LOC || KC) {
+  if (LOC || KC || LJ) {
     if (KC) {
       const T *in[1] = {S<T>(d, TMP5)};
       cplx<T> *out[1] = {CS<T>(d, SH)};
@@ -1000,10 +1003,8 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     const cplx<T> *in[3] = {CS<T>(d, SP0), CS<T>(d, SP4), CS<T>(d, SH)};
     fft::Term t[3];
     int nt = 0;
-    if (LOC) {
-      t[nt++] = {0, K_LJ, -1, h3N};
-      t[nt++] = {1, K_SA, -1, h3N};
-    }
+    if (LJ) t[nt++] = {0, K_LJ, -1, h3N};
+    if (LOC) t[nt++] = {1, K_SA, -1, h3N};
     if (KC) t[nt++] = {2, K_F, -1, h3N * otf->kc_coef23 * otf->alpha_s / (2.0 * otf->mass)};
     fft::Combine c = cmb(nt, t);
     // (TMP1 as a real field: RB keeps rhobar for the adjoint pass)
@@ -1012,7 +1013,9 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     c2r3d<T>(p, 3, in, 1, &c, kt, tmp, out, s);
   }
   // P4
-  LAUNCH_PW("ot_accumulate_potential", 5 + ((LOC || KC) ? 1 : 0) + (BF ? 1 : 0), (k_final<T>), (cplx<T> *)V->dev, S<T>(d, VRE), (LOC || KC) ? S<T>(d, TMP1) : (T *)nullptr,
+  const bool PW_ = LOC || KC || BF;  // pointwise stages ran (V_re in VRE)
+  LAUNCH_PW("ot_accumulate_potential", 4 + (PW_ ? 1 : 0) + ((LOC || KC || LJ) ? 1 : 0) + (BF ? 1 : 0), (k_final<T>),
+            (cplx<T> *)V->dev, PW_ ? S<T>(d, VRE) : (T *)nullptr, (LOC || KC || LJ) ? S<T>(d, TMP1) : (T *)nullptr,
             BF ? S<T>(d, VIM) : (T *)nullptr, d->n);
   if (write_back_w && BF) {
     // the reference leaves v_d A - B_d in the velocity inputs (src/ot.c:923-931)
@@ -1106,8 +1109,8 @@ void launch_xmid(const fused::MidProg *pg, const fused::MidProg &hp, int ny, int
   DFT_PROF_BEGIN(s);
   bool one = true;
   for (int m = 0; m < hp.nout; ++m) one = one && hp.o[m].nterms == 1;
-  if constexpr ((DFT_XMID_KIND == 1 || DFT_XMID_KIND == 2) && sizeof(T) == 4 ||
-                DFT_XMID_KIND == 3 && (sizeof(T) == 4 || N <= 256)) {
+  if constexpr (((DFT_XMID_KIND == 1 || DFT_XMID_KIND == 2) && sizeof(T) == 4) ||
+                (DFT_XMID_KIND == 3 && (sizeof(T) == 4 || N <= 256))) {
     // per-output blocks unless the program shares inputs between
     // multi-term outputs with derivative symbols (J_d and T1 = F sum_d D_d G_d)
     if (one || !deriv || DFT_XMID_KIND == 2) {
@@ -1135,8 +1138,15 @@ void launch_xmid(const fused::MidProg *pg, const fused::MidProg &hp, int ny, int
 template <typename T, int N, bool KC, bool BF>
 void launch_zAB1(const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw, hipStream_t s) {
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
-  hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), dim3((unsigned)((npairs + LN - 1) / LN)), dim3(256), 0, s, a,
-                     npairs, P, pw, tw);
+  const dim3 grid((unsigned)((npairs + LN - 1) / LN));
+  if constexpr (BF && sizeof(T) == 8 && DFT_ZAB_SPLIT64) {
+    // FP64: the backflow terms, then the local and KC terms (V_re through memory)
+    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 1>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+    DFT_LAUNCH_CHECK();
+    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 2>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+  } else {
+    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+  }
 }
 
 template <typename T, int N>
@@ -2454,6 +2464,32 @@ void dft_ot_potential2(dft_ot_functional *otf, cgrid *potential, rgrid *density,
   DFT_BY_PREC(otf->device->prec,
               potential_impl<T>(otf, potential, nullptr, 0.0, (T *)density->dev, parts.bf ? (T *)vx->dev : nullptr,
                                 parts.bf ? (T *)vy->dev : nullptr, parts.bf ? (T *)vz->dev : nullptr, parts, true));
+}
+
+// src/ot.c:524-531: potential += V_LJ * density.  The reference leaves FFT(density)
+// in workspace1 for dft_ot_add_local_correlation_potential; here the workspaces
+// are accepted and not written (that function transforms its density itself).
+void dft_ot_add_lennard_jones_potential(dft_ot_functional *otf, cgrid *potential, rgrid *density, rgrid *, rgrid *) {
+  if (!check_c(otf, potential, "dft_ot_add_lennard_jones_potential") ||
+      !check_r(otf, density, "dft_ot_add_lennard_jones_potential"))
+    return;
+  PotParts parts{false, false, false, 1};
+  DFT_BY_PREC(otf->device->prec, potential_impl<T>(otf, potential, nullptr, 0.0, (T *)density->dev, nullptr, nullptr,
+                                                   nullptr, parts, false));
+}
+
+// src/ot.c:547-592: potential += c2/2 rhobar^2 + c3/3 rhobar^3 + Pi * ((c2 rhobar +
+// c3 rhobar^2) rho), rhobar = Pi * rho, for the OT exponents (c2_exp = 2, c3_exp
+// = 3).  rho_tf (FFT(rho) in the reference) and the workspaces are accepted and
+// not read or written: rhobar is computed from rho.
+void dft_ot_add_local_correlation_potential(dft_ot_functional *otf, cgrid *potential, rgrid *rho, rgrid *, rgrid *,
+                                            rgrid *) {
+  if (!check_c(otf, potential, "dft_ot_add_local_correlation_potential") ||
+      !check_r(otf, rho, "dft_ot_add_local_correlation_potential"))
+    return;
+  PotParts parts{true, false, false, 0};
+  DFT_BY_PREC(otf->device->prec, potential_impl<T>(otf, potential, nullptr, 0.0, (T *)rho->dev, nullptr, nullptr,
+                                                   nullptr, parts, false));
 }
 
 // src/ot.c:840-991 (the workspace arguments are accepted and unused)

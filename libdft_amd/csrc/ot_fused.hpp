@@ -96,13 +96,17 @@ template <typename T, int N, int TPL> struct PairPos {
   }
 };
 
-// v_d = Im(psi* (psi_{+d} - psi_{-d})/2h) / (M (rho + eps)), clamped when enabled.
+// v_d = Im(psi* (psi_{+d} - psi_{-d})/2h) / (M (rho + eps)), clamped when
+// enabled; f = 1 / (M (rho + eps)) is computed once per point for all axes.
+template <typename T> __device__ __forceinline__ T velocity_factor(const PW<T> &pw, T r) {
+  return pw.invM / (r + pw.eps_v);
+}
 template <typename T, int D, class Q>
-__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T r) {
+__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f) {
   const cplx<T> s = q.template at<cplx<T>>(psi, w, i);
   const cplx<T> a = q.template nb<cplx<T>, D, 1>(psi, w, i), b = q.template nb<cplx<T>, D, -1>(psi, w, i);
   const T dx = (a.x - b.x) * pw.inv2h, dy = (a.y - b.y) * pw.inv2h;
-  T v = (s.x * dy - s.y * dx) * (pw.invM / (r + pw.eps_v));
+  T v = (s.x * dy - s.y * dx) * f;
   if (pw.clamp_v) v = clampv(v, pw.vmax);
   return v;
 }
@@ -125,7 +129,7 @@ using PairSync = std::conditional_t<(rf::RP<N, pair_em<N>()>::TPL <= 64), rf::Wa
   using G = LineCfg<T, N, pair_em<N>()>;                                     \
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;                          \
   __shared__ C s_x[G::LINES * LP];                                           \
-  __shared__ C s_tw[2 * N];                                                  \
+  __shared__ C s_tw[rf::tw_len<T, N>()];                                     \
   const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;                   \
   const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;                  \
   const bool act = p0 < npairs;                                              \
@@ -158,7 +162,7 @@ template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { retur
 // waves per SIMD asked of the register allocator for the FP32 zA and zC
 // kernels (0: no request)
 #ifndef DFT_ZA_MINB
-#define DFT_ZA_MINB 0
+#define DFT_ZA_MINB 5
 #endif
 #ifndef DFT_ZC_MINB
 #define DFT_ZC_MINB 0
@@ -181,8 +185,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZA_MINB : 0)
       q.template put<T>(rho, 1, i, v[i].y);
     }
   }
-  r2c_store<T, N>(v, X, s_tw, t, outs.p[0], p, P, act);
   if constexpr (BF) {
+    // velocity factor per point, kept for the three axes
+    T fv[2][E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      fv[0][i] = velocity_factor(pw, v[i].x);
+      fv[1][i] = velocity_factor(pw, v[i].y);
+    }
+    r2c_store<T, N>(v, X, s_tw, t, outs.p[0], p, P, act);
     T v2[2][E];
 #pragma unroll
     for (int i = 0; i < E; ++i) v2[0][i] = v2[1][i] = 0;
@@ -196,7 +207,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZA_MINB : 0)
         for (int w = 0; w < 2; ++w) {
           const C s = qq.template at<C>(psi, w, i);
           const T r = s.x * s.x + s.y * s.y;
-          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, r);
+          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i]);
           v2[w][i] += vd * vd;
           o[w] = vd * r;
         }
@@ -212,6 +223,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZA_MINB : 0)
       v[i] = cmk<C>(v2[0][i] * q.template at<T>(rho, 0, i), v2[1][i] * q.template at<T>(rho, 1, i));
     }
     r2c_store<T, N>(v, X, s_tw, t, outs.p[4], p, P, act);
+  } else {
+    r2c_store<T, N>(v, X, s_tw, t, outs.p[0], p, P, act);
   }
 }
 
@@ -273,7 +286,7 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
   using G = MidCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   // DFT_XMID_ORDER 0: consecutive tiles (one ky) on one XCD, back to back
   // (shared L2 lines); 1: ky fastest, so that the blocks in flight spread over
   // whole x-planes
@@ -403,6 +416,10 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
 #ifndef DFT_XCONV_MINB
 #define DFT_XCONV_MINB 4
 #endif
+// the same request for the FP64 instantiations (0: none)
+#ifndef DFT_XCONV_MINB64
+#define DFT_XCONV_MINB64 2
+#endif
 // (N = 512 keeps 16 points per thread and 16 columns.  FP64 single-term
 // outputs take DFT_XCONV1_COLS64 columns: 16 measured faster than 32, which
 // fill the LDS with one block per CU; the accumulating form takes at most 32.)
@@ -416,14 +433,14 @@ using XConvCfg = ColCfg<T, N, N >= 512 ? 16 : (ONE ? DFT_XCONV1_EM : DFT_XCONV_E
                                         : (sizeof(T) == 8 && DFT_XCONV_COLS > 32 ? 32 : DFT_XCONV_COLS))>;
 
 template <typename T, int N, bool DERIV, bool ONE>
-__global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), DFT_XCONV_MINB) k_xconv(const MidProg *__restrict__ pgp, int m0, int nl, int ny,
+__global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DFT_XCONV_MINB : DFT_XCONV_MINB64) k_xconv(const MidProg *__restrict__ pgp, int m0, int nl, int ny,
                                                               int P, int PP, const cplx<T> *__restrict__ g_tw) {
   const MidProg &pg = *pgp;
   using C = cplx<T>;
   using G = XConvCfg<T, N, ONE>;
   constexpr int E = G::E, TPL = G::TPL, COLS = G::COLS, LP = G::LP;
   __shared__ C s_x[G::S > 1 ? COLS * LP : 1];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   // outputs m0 .. m0 + nl - 1 of the program
   const int L = rf::xcd_block();
   const int b = L / nl, m = m0 + (L - b * nl);
@@ -526,6 +543,46 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), DFT_XCONV_MINB) k_x
 #ifndef DFT_ZBC_MINB64
 #define DFT_ZBC_MINB64 2
 #endif
+// FP64 backflow zAB as two kernels (1) or one (0), and the waves per SIMD
+// asked for each of the two
+#ifndef DFT_ZAB_SPLIT64
+#define DFT_ZAB_SPLIT64 1
+#endif
+#ifndef DFT_ZAB_SPLIT_MINB64
+#define DFT_ZAB_SPLIT_MINB64 2
+#endif
+// Early loads in zAB / zBC, so that memory round trips overlap the transforms
+// (bit masks, FP32; FP64 holds twice the registers and spills at its
+// occupancy, so it uses them only with DFT_PREFETCH64=1):
+//  1  a stage's stencils (velocity, density gradients) before its spectrum's transform;
+//  2  the density (zAB) / the accumulated fields (zBC) before the last transform;
+//  4  zAB: the KC density gradient along the next axis before the current r2c;
+//  8  the next stage's spectrum before the current stage's transform;
+// zC (bit 1): the fields of each line of the step before the preceding transform.
+#ifndef DFT_ZAB_PF
+#define DFT_ZAB_PF 15
+#endif
+#ifndef DFT_ZBC_PF
+#define DFT_ZBC_PF 11
+#endif
+#ifndef DFT_ZC_PF
+#define DFT_ZC_PF 1
+#endif
+#ifndef DFT_PREFETCH64
+#define DFT_PREFETCH64 0
+#endif
+template <typename T, int MASK> constexpr int prefetch_bits() { return (sizeof(T) == 4 || DFT_PREFETCH64) ? MASK : 0; }
+
+// Pipelined c2r stages: the spectrum of the current stage was loaded into
+// `raw` earlier; the next stage's loads (if any) are issued before this
+// stage's transform.
+template <typename T, int N>
+__device__ __forceinline__ void c2r_pipe(rf::Regs<T, N, pair_em<N>()> &v, fft::RawPair<T, N, pair_em<N>()> &raw,
+                                         cplx<T> *X, const cplx<T> *tw, int t, const void *next, uint32_t p, int P) {
+  fft::merge_raw<T, N, pair_em<N>()>(v, raw, t);
+  if (next) fft::load_raw<T, N, pair_em<N>()>(raw, next, p, P, t);
+  rf::rfft<T, N, pair_em<N>(), +1, PairSync<N>>(v, X, tw, t);
+}
 struct ZABArgs {
   Ptrs8 in;       // rhobar, [rhot], [A, Bx, By, Bz, C]
   Ptrs8 out;      // Q, [G_x, G_y, G_z]
@@ -533,43 +590,91 @@ struct ZABArgs {
   void *rhot, *vre, *vim, *wx, *wy, *wz;
 };
 
-template <typename T, int N, bool KC, bool BF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : DFT_ZAB_MINB64) k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+// PART 0: the whole stage; with backflow it may run as two kernels with fewer
+// live fields, PART 1 (the backflow terms, V_re stored) then PART 2 (the local
+// and KC terms, added to the stored V_re).
+template <typename T, int N, bool KC, bool BF, int PART = 0>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : (PART ? DFT_ZAB_SPLIT_MINB64 : DFT_ZAB_MINB64))
+    k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+  static_assert(PART == 0 || BF, "only the backflow stage splits");
   FUSED_PAIR_SETUP();
+  constexpr int PF = prefetch_bits<T, DFT_ZAB_PF>();
   constexpr int IA = KC ? 2 : 1;  // first backflow input
+  constexpr bool DO_BF = BF && PART != 2, DO_LOCAL = PART != 1;
   const void *rho = a.rho;
   C v[E];
   T vre[2][E];
-  if constexpr (BF) {
+  fft::RawPair<T, N, pair_em<N>()> raw;
+  // c2r of stage input `cur`, then (with bit 8) the loads of `next`
+  auto c2r = [&](const void *cur, const void *next) {
+    if constexpr ((PF & 8) != 0) c2r_pipe<T, N>(v, raw, X, s_tw, t, next, p, P);
+    else load_c2r<T, N>(v, X, s_tw, t, cur, p, P);
+  };
+  if constexpr ((PF & 8) != 0) fft::load_raw<T, N, pair_em<N>()>(raw, a.in.p[DO_BF ? IA : 0], p, P, t);
+  if constexpr (PART == 2) {
+    const auto qv = q.here();
+#pragma unroll
+    for (int i = 0; i < E; ++i)
+#pragma unroll
+      for (int w = 0; w < 2; ++w) vre[w][i] = qv.template at_once<T>(a.vre, w, i);
+  }
+  if constexpr (DO_BF) {
     // A = U*rho; per axis: B_d = U*q_d -> W_d = v_d A - B_d (stored),
     // BF real part sum_d (v_d^2 A - 2 v_d B_d) + C, imaginary part sum_d (g_d/(rho+eps)) W_d / 2
     T A[2][E], tr[2][E], im[2][E];
-    load_c2r<T, N>(v, X, s_tw, t, a.in.p[IA], p, P);
+    c2r(a.in.p[IA], a.in.p[IA + 1]);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       A[0][i] = v[i].x;
       A[1][i] = v[i].y;
     }
     const void *psi = a.psi;
+    // velocity factor and 1 / (rho + eps) per point, kept for the three axes
+    T fv[2][E], fg[2][E];
+    {
+      const auto qr = q.here();
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          const T r = qr.template at<T>(rho, w, i);
+          fv[w][i] = velocity_factor(pw, r);
+          fg[w][i] = (T)1 / (r + pw.div_eps);
+        }
+    }
     auto axis = [&](auto dtag) {
       constexpr int D = decltype(dtag)::value;
-      load_c2r<T, N>(v, X, s_tw, t, a.in.p[IA + 1 + D], p, P);
       void *wd = D == 0 ? a.wx : (D == 1 ? a.wy : a.wz);
+      // stencil positions taken after the transform unless prefetching, so
+      // that the compiler does not hoist those loads into it
+      std::remove_const_t<decltype(q)> qq;
+      if constexpr ((PF & 1) != 0) qq = q.here();
+      // velocity and (D_d rho)/(rho + eps) at point (w, i)
+      auto vg = [&](int w, int i, T &vd, T &gi) {
+        vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i]);
+        gi = grad_d<T, D>(rho, qq, w, i, pw) * fg[w][i];
+      };
+      T vds[2][E], gis[2][E];
+      if constexpr ((PF & 1) != 0)
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+#pragma unroll
+          for (int w = 0; w < 2; ++w) vg(w, i, vds[w][i], gis[w][i]);
+      c2r(a.in.p[IA + 1 + D], a.in.p[IA + 2 + D]);
+      if constexpr ((PF & 1) == 0) qq = q.here();
       T Wv[2][E];
-      const auto qq = q.here();
 #pragma unroll
       for (int i = 0; i < E; ++i) {
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
           const T Bd = part(v[i], w);
-          const T r = qq.template at<T>(rho, w, i);
-          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, r);
-          const T g = grad_d<T, D>(rho, qq, w, i, pw);
-          const T inv = (T)1 / (r + pw.div_eps);
+          T vd, gi;
+          if constexpr ((PF & 1) != 0) vd = vds[w][i], gi = gis[w][i];
+          else vg(w, i, vd, gi);
           const T W = vd * A[w][i] - Bd;
           Wv[w][i] = W;
           const T ti = vd * vd * A[w][i] + (T)-2 * vd * Bd;
-          const T ii = (T)0.5 * (g * inv) * W;
+          const T ii = (T)0.5 * gi * W;
           if constexpr (D == 0) {
             tr[w][i] = ti;
             im[w][i] = ii;
@@ -589,7 +694,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : DFT_ZAB_M
     axis(std::integral_constant<int, 0>{});
     axis(std::integral_constant<int, 1>{});
     axis(std::integral_constant<int, 2>{});
-    load_c2r<T, N>(v, X, s_tw, t, a.in.p[IA + 4], p, P);
+    c2r(a.in.p[IA + 4], DO_LOCAL ? a.in.p[0] : nullptr);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
 #pragma unroll
@@ -606,51 +711,72 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : DFT_ZAB_M
         q.template put<T>(a.vim, 1, i, im[1][i]);
       }
   }
-  // local correlation: rhobar = Pi*rho
-  load_c2r<T, N>(v, X, s_tw, t, a.in.p[0], p, P);
-  const auto ql = q.here();
+  if constexpr (DO_LOCAL) {
+    // local correlation: rhobar = Pi*rho
+    std::remove_const_t<decltype(q)> ql;
+    T rl[2][E];
+    auto dens = [&] {
+      ql = q.here();
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    T Q[2];
+      for (int i = 0; i < E; ++i)
 #pragma unroll
-    for (int w = 0; w < 2; ++w) {
-      const T rb = part(v[i], w), r = ql.template at<T>(rho, w, i);
-      T loc = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
-      if (pw.hdC != (T)0) loc += hd_pot(r, pw);
-      if constexpr (BF) vre[w][i] += loc;
-      else vre[w][i] = loc;
-      Q[w] = (rb * pw.c2 + rb * rb * pw.c3) * r;
-    }
-    v[i] = cmk<C>(Q[0], Q[1]);
-  }
-  r2c_store<T, N>(v, X, s_tw, t, a.out.p[0], p, P, act);
-  if constexpr (KC) {
-    // rhot = F*rho (stored), rho_st = 1 - rhot/rho_0s, G_d = (D_d rho) rho_st
-    load_c2r<T, N>(v, X, s_tw, t, a.in.p[1], p, P);
-    T rst[2][E];
+        for (int w = 0; w < 2; ++w) rl[w][i] = ql.template at<T>(rho, w, i);
+    };
+    if constexpr ((PF & 2) != 0) dens();
+    c2r(a.in.p[0], KC ? a.in.p[1] : nullptr);
+    if constexpr ((PF & 2) == 0) dens();
 #pragma unroll
     for (int i = 0; i < E; ++i) {
+      T Q[2];
 #pragma unroll
-      for (int w = 0; w < 2; ++w) rst[w][i] = part(v[i], w) * (-pw.inv_rho0s) + (T)1;
+      for (int w = 0; w < 2; ++w) {
+        const T rb = part(v[i], w), r = rl[w][i];
+        T loc = rb * rb * (pw.c2 * (T)0.5) + rb * rb * rb * (pw.c3 / (T)3);
+        if (pw.hdC != (T)0) loc += hd_pot(r, pw);
+        if constexpr (BF) vre[w][i] += loc;
+        else vre[w][i] = loc;
+        Q[w] = (rb * pw.c2 + rb * rb * pw.c3) * r;
+      }
+      v[i] = cmk<C>(Q[0], Q[1]);
     }
-    if (act)
+    r2c_store<T, N>(v, X, s_tw, t, a.out.p[0], p, P, act);
+    if constexpr (KC) {
+      // rhot = F*rho (stored), rho_st = 1 - rhot/rho_0s, G_d = (D_d rho) rho_st
+      T gn[2][E];
+      auto grad_next = [&](auto dtag) {
+        constexpr int D = decltype(dtag)::value;
+        const auto qq = q.here();
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+#pragma unroll
+          for (int w = 0; w < 2; ++w) gn[w][i] = grad_d<T, D>(rho, qq, w, i, pw);
+      };
+      if constexpr ((PF & 4) != 0) grad_next(std::integral_constant<int, 0>{});
+      c2r(a.in.p[1], nullptr);
+      T rst[2][E];
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        q.template put<T>(a.rhot, 0, i, v[i].x);
-        q.template put<T>(a.rhot, 1, i, v[i].y);
-      }
-    auto gd = [&](auto dtag) {
-      constexpr int D = decltype(dtag)::value;
-      const auto qq = q.here();
 #pragma unroll
-      for (int i = 0; i < E; ++i) {
-        v[i] = cmk<C>(grad_d<T, D>(rho, qq, 0, i, pw) * rst[0][i], grad_d<T, D>(rho, qq, 1, i, pw) * rst[1][i]);
+        for (int w = 0; w < 2; ++w) rst[w][i] = part(v[i], w) * (-pw.inv_rho0s) + (T)1;
       }
-      r2c_store<T, N>(v, X, s_tw, t, a.out.p[1 + D], p, P, act);
-    };
-    gd(std::integral_constant<int, 0>{});
-    gd(std::integral_constant<int, 1>{});
-    gd(std::integral_constant<int, 2>{});
+      if (act)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          q.template put<T>(a.rhot, 0, i, v[i].x);
+          q.template put<T>(a.rhot, 1, i, v[i].y);
+        }
+      auto gd = [&](auto dtag) {
+        constexpr int D = decltype(dtag)::value;
+        if constexpr ((PF & 4) == 0) grad_next(dtag);
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = cmk<C>(gn[0][i] * rst[0][i], gn[1][i] * rst[1][i]);
+        if constexpr ((PF & 4) != 0 && D < 2) grad_next(std::integral_constant<int, D + 1>{});
+        r2c_store<T, N>(v, X, s_tw, t, a.out.p[1 + D], p, P, act);
+      };
+      gd(std::integral_constant<int, 0>{});
+      gd(std::integral_constant<int, 1>{});
+      gd(std::integral_constant<int, 2>{});
+    }
   }
   if (act) {
 #pragma unroll
@@ -669,29 +795,49 @@ struct ZBCArgs {
   void *vre, *vim;
 };
 
+// Every stage first issues the loads that do not depend on its spectrum (the
+// stencils of rho and rho~, the fields accumulated into), so that their
+// latency overlaps the spectrum's loads and transform.
 template <typename T, int N, bool BF>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : DFT_ZBC_MINB64) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
+  constexpr int PF = prefetch_bits<T, DFT_ZBC_PF>();
   const void *rho = a.rho, *rhot = a.rhot;
   C v[E];
   T H[2][E], t3[2][E];
+  fft::RawPair<T, N, pair_em<N>()> raw;
+  auto c2r = [&](const void *cur, const void *next) {
+    if constexpr ((PF & 8) != 0) c2r_pipe<T, N>(v, raw, X, s_tw, t, next, p, P);
+    else load_c2r<T, N>(v, X, s_tw, t, cur, p, P);
+  };
+  if constexpr ((PF & 8) != 0) fft::load_raw<T, N, pair_em<N>()>(raw, a.in.p[0], p, P, t);
   auto axis = [&](auto dtag) {
     constexpr int D = decltype(dtag)::value;
-    load_c2r<T, N>(v, X, s_tw, t, a.in.p[D], p, P);
-    const auto qq = q.here();
+    T g[2][E], Pd[2][E];
+    auto stencil = [&] {
+      const auto qq = q.here();
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          g[w][i] = grad_d<T, D>(rho, qq, w, i, pw);
+          Pd[w][i] = grad_d<T, D>(rhot, qq, w, i, pw);  // (dF_d * rho) = D_d (F * rho)
+        }
+    };
+    if constexpr ((PF & 1) != 0) stencil();
+    c2r(a.in.p[D], a.in.p[D + 1]);
+    if constexpr ((PF & 1) == 0) stencil();
 #pragma unroll
     for (int i = 0; i < E; ++i) {
 #pragma unroll
       for (int w = 0; w < 2; ++w) {
-        const T g = grad_d<T, D>(rho, qq, w, i, pw);
-        const T Pd = grad_d<T, D>(rhot, qq, w, i, pw);  // (dF_d * rho) = D_d (F * rho)
         const T J = part(v[i], w);
         if constexpr (D == 0) {
-          H[w][i] = g * J;
-          t3[w][i] = Pd * J;
+          H[w][i] = g[w][i] * J;
+          t3[w][i] = Pd[w][i] * J;
         } else {
-          H[w][i] += g * J;
-          t3[w][i] += Pd * J;
+          H[w][i] += g[w][i] * J;
+          t3[w][i] += Pd[w][i] * J;
         }
       }
     }
@@ -699,26 +845,50 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : DFT_ZBC_M
   axis(std::integral_constant<int, 0>{});
   axis(std::integral_constant<int, 1>{});
   axis(std::integral_constant<int, 2>{});
-  load_c2r<T, N>(v, X, s_tw, t, a.in.p[3], p, P);
-  const auto qt = q.here();
-  if (act) {
+  // rho_st, V_re so far, and the backflow imaginary part (which does not
+  // involve T1); with prefetching, all before T1's transform
+  std::remove_const_t<decltype(q)> qt;
+  auto rst_at = [&](int w, int i) { return qt.template at<T>(rhot, w, i) * (-pw.inv_rho0s) + (T)1; };
+  auto vre_at = [&](int w, int i) { return qt.template at_once<T>(a.vre, w, i); };
+  auto kc_at = [&](T J, T rst, int w, int i) { return J * rst * pw.ckc - t3[w][i] * (pw.ckc * pw.k23); };
+  auto bf_im = [&](int w, int i) {
+    if constexpr (BF) {
+      T im = qt.template at_once<T>(a.vim, w, i);
+      im += (T)0.5 * ((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h);
+      im += (T)0.5 * ((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h);
+      im += (T)0.5 * ((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h);
+      if (pw.clamp_bf) im = clampv(im, pw.bfmax);
+      if (act) qt.template put<T>(a.vim, w, i, im * pw.cbf);
+    }
+  };
+  if constexpr ((PF & 2) != 0) {
+    qt = q.here();
+    T rst[2][E], base[2][E];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
+    for (int i = 0; i < E; ++i)
 #pragma unroll
       for (int w = 0; w < 2; ++w) {
-        const T rst = qt.template at<T>(rhot, w, i) * (-pw.inv_rho0s) + (T)1;
-        qt.template put<T>(a.vre, w, i,
-                          qt.template at_once<T>(a.vre, w, i) + (part(v[i], w) * rst * pw.ckc - t3[w][i] * (pw.ckc * pw.k23)));
-        if constexpr (BF) {
-          T im = qt.template at_once<T>(a.vim, w, i);
-          im += (T)0.5 * ((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h);
-          im += (T)0.5 * ((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h);
-          im += (T)0.5 * ((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h);
-          if (pw.clamp_bf) im = clampv(im, pw.bfmax);
-          qt.template put<T>(a.vim, w, i, im * pw.cbf);
-        }
+        rst[w][i] = rst_at(w, i);
+        base[w][i] = vre_at(w, i);
+        bf_im(w, i);
       }
-    }
+    c2r(a.in.p[3], nullptr);
+    if (act)
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) qt.template put<T>(a.vre, w, i, base[w][i] + kc_at(part(v[i], w), rst[w][i], w, i));
+  } else {
+    c2r(a.in.p[3], nullptr);
+    qt = q.here();
+    if (act)
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          qt.template put<T>(a.vre, w, i, vre_at(w, i) + kc_at(part(v[i], w), rst_at(w, i), w, i));
+          bf_im(w, i);
+        }
   }
 #pragma unroll
   for (int i = 0; i < E; ++i) v[i] = cmk<C>(H[0][i], H[1][i]);
@@ -747,7 +917,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZC
   using G = LineCfg<T, N, pair_em<N>()>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
   __shared__ C s_x[G::LINES * LP];
-  __shared__ C s_tw[2 * N];
+  __shared__ C s_tw[rf::tw_len<T, N>()];
   const int ln = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = p0 < npairs;
@@ -755,10 +925,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZC
   C *X = s_x + ln * LP;
   rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
   __syncthreads();
+  constexpr bool PF = prefetch_bits<T, DFT_ZC_PF>() != 0;
   C v[E];
-  load_c2r<T, N>(v, X, s_tw, t, a.in.p[0], p, P);
+  auto c2r_in = [&] { load_c2r<T, N>(v, X, s_tw, t, a.in.p[0], p, P); };
   const uint32_t cb = p * 2u * N + (uint32_t)t;
   if constexpr (!STEP) {
+    c2r_in();
     const uint32_t c0 = cb + rf::opaque_zero();
     if (act) {
 #pragma unroll
@@ -773,23 +945,41 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZC
         }
     }
   } else {
+    // the fields of line w; with prefetching, line 0's are read before the
+    // transform above and line 1's before line 0's transform
+    T bre[E], bim[E], bex[E];
+    C bav[E], bps[E];
+    auto fetch = [&](int w) {
+      const uint32_t c0 = cb + rf::opaque_zero();
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const int d = w * N + TPL * i;
+        bre[i] = rf::lde_once<T>(a.vre, c0, d);
+        bim[i] = a.vim ? rf::lde_once<T>(a.vim, c0, d) : (T)0;
+        bex[i] = a.vext ? rf::lde_once<T>(a.vext, c0, d) : (T)0;
+        if (a.vavg) bav[i] = rf::lde_once<C>(a.vavg, c0, d);
+        bps[i] = rf::lde_once<C>(a.psi_src, c0, d);
+      }
+    };
+    if constexpr (PF) fetch(0);
+    c2r_in();
 #pragma unroll
     for (int w = 0; w < 2; ++w) {
+      if constexpr (!PF) fetch(w);
       const uint32_t c0 = cb + rf::opaque_zero();
       C u[E], ph[E], vs[E];  // ph: the potential of this half step (re-applied by the inverse z pass)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        const int d = w * N + TPL * i;
-        const T re = rf::lde_once<T>(a.vre, c0, d) + part(v[i], w);
-        const T im = a.vim ? rf::lde_once<T>(a.vim, c0, d) : (T)0;
-        C ve = cmk<C>(re - (T)a.mu + (a.vext ? rf::lde_once<T>(a.vext, c0, d) : (T)0), im);
+        const T re = bre[i] + part(v[i], w);
+        const T im = bim[i];
+        C ve = cmk<C>(re - (T)a.mu + bex[i], im);
         if (a.vsave) vs[i] = ve;
         if (a.vavg) {
-          const C va = rf::lde_once<C>(a.vavg, c0, d);
+          const C va = bav[i];
           ve = cmk<C>((va.x + ve.x) * (T)0.5, (va.y + ve.y) * (T)0.5);
         }
         ph[i] = ve;
-        u[i] = fft::apply_vphase<T>(rf::lde_once<C>(a.psi_src, c0, d), ve, a.dtr, a.dti);
+        u[i] = fft::apply_vphase<T>(bps[i], ve, a.dtr, a.dti);
       }
       if (act) {
 #pragma unroll
@@ -797,6 +987,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZC
           rf::ste<C>(a.phase, c0, ph[i], w * N + TPL * i);
           if (a.vsave) rf::ste<C>(a.vsave, c0, vs[i], w * N + TPL * i);
         }
+      }
+      if constexpr (PF) {
+        if (w == 0) fetch(1);
       }
       rf::rfft<T, N, pair_em<N>(), -1, PairSync<N>>(u, X, s_tw, t);
       if (act)

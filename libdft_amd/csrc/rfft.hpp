@@ -228,6 +228,10 @@ __device__ __forceinline__ void rfft(Regs<T, N, EM> &v, cplx<T> *X, const cplx<T
   }
 }
 
+// LDS entries of a twiddle table: the N - R0 stage-ordered twiddles, and for
+// FP32 their rounding residuals at offset N.
+template <typename T, int N> constexpr int tw_len() { return sizeof(T) == 4 ? 2 * N : N; }
+
 // Gather the stage-ordered twiddles of plan RP<N, EM> (see tw_entry) from the
 // natural table g_tw (w^m at m, FP32 residual at N + m) into LDS; callers
 // synchronise before use.

@@ -17,8 +17,10 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# EXTRA_FLAGS: further compiler options, e.g. tuning macros (-DDFT_ZA_MINB=5)
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
-         "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage", "--cuda-device-only"]
+         "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage", "--cuda-device-only"] + \
+    os.environ.get("EXTRA_FLAGS", "").split()
 KEYS = ["VGPRs", "AGPRs", "VGPRs Spill", "ScratchSize [bytes/lane]", "LDS Size [bytes/block]", "Occupancy [waves/SIMD]"]
 
 

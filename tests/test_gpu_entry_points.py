@@ -3,6 +3,7 @@ against the FP64 oracle in both device precisions, through the C ABI:
 
   dft_ot_potential2 (all models; v_d A - B_d left in the velocity inputs,
     src/ot.c:923-931), dft_ot_backflow_potential (src/ot.c:840-991),
+  dft_ot_add_lennard_jones_potential / _local_correlation_potential (src/ot.c:524, 547),
   dft_ot_energy_density_kc / _bf accumulating with a given density
     (src/ot-energy.c:123-288), grid_wf_velocity, rgrid_gradient_*,
   the max_veloc, max_bfpot and c_bfpot branches (src/ot.c:391-395, 911-917)
@@ -117,6 +118,53 @@ def test_backflow_potential_matches_oracle(oracle, prec):
 
 
 # ---------------------------------------------------- _kc / _bf energies --
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+def test_lennard_jones_and_local_correlation_parts(oracle, prec):
+    """dft_ot_add_lennard_jones_potential and dft_ot_add_local_correlation_potential
+    (src/ot.c:524, 547) accumulate; their sum is the plain OT potential of the oracle.
+    The split is pinned by the local part at uniform density, c2/2 rho^2 + c3/3 rho^3
+    + (c2 rho + c3 rho^2) rho (the spherical average integrates to 1), and by the
+    linearity of the Lennard-Jones part in rho."""
+    o = oracle.OracleOT(0, SHAPE, STEP, 2, 8)
+    psi = inputs.smooth_modulated(SHAPE, STEP, o.p.rho0, 0.3, seed=5)
+    rho = np.abs(psi) ** 2
+    base = (1e-5 * inputs.splitmix64_uniform(psi.size, 4).reshape(SHAPE) + 3e-6j).astype(complex)
+    w = L.Wave(SHAPE, STEP, prec)
+    w.set(psi)
+    f = L.Functional(0, w, 2, 8)
+    lib = L.lib()
+    ws = [L.RField(SHAPE, STEP, prec) for _ in range(3)]
+
+    def parts(r):
+        gr = _rfield(r, prec)
+        pl, pc = L.CField(SHAPE, STEP, prec), L.CField(SHAPE, STEP, prec)
+        pl.set(base)
+        pc.set(base)
+        lib.dft_ot_add_lennard_jones_potential(f.h, pl.h, gr.h, ws[0].h, ws[1].h)
+        lib.dft_ot_add_local_correlation_potential(f.h, pc.h, gr.h, ws[0].h, ws[1].h, ws[2].h)
+        out = pl.get() - base, pc.get() - base
+        for x in (gr, pl, pc):
+            x.free()
+        return out
+
+    lj, loc = parts(rho)
+    want = o.potential2(rho, *[np.zeros(SHAPE)] * 3)
+    assert relerr((lj + loc).real, want.real) < TOL[prec] * 10
+    assert np.abs(lj.imag).max() < 1e-10 and np.abs(loc.imag).max() < 1e-10
+    # linearity of the LJ part; the local part is not linear
+    lj2, loc2 = parts(2 * rho)
+    assert relerr(lj2.real, 2 * lj.real) < TOL[prec] * 10
+    assert relerr(loc2.real, 2 * loc.real) > 0.1
+    # uniform density
+    r0 = f.p.rho0
+    _, locu = parts(np.full(SHAPE, r0))
+    c2, c3 = f.p.c2, f.p.c3
+    closed = c2 / 2 * r0 ** 2 + c3 / 3 * r0 ** 3 + (c2 * r0 + c3 * r0 ** 2) * r0
+    assert np.abs(locu.real / closed - 1).max() < TOL[prec] * 10
+    for x in ws + [f, w]:
+        x.free()
+
+
 @pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
 def test_energy_density_kc_bf_accumulate(oracle, prec):
     """_kc and _bf add their parts to the energy density, with the density given

@@ -142,3 +142,32 @@ def test_driver_grid_calls(prec):
     assert abs(got - (a * (1.0 + 0.1 * X - 0.2 * Y * Z)).sum() * h3) < 1e-12 * np.abs(a).sum() * h3 * 10
     for x in (cf, cg, cd, r, fx, fy, fz):
         x.free()
+
+
+@pytest.mark.parametrize("angle", [1e-4, 3e-3, 0.7])
+def test_fp32_phase_factors_keep_the_norm(angle):
+    """A real potential makes e^{-iV dt/2} unitary, so grid_wf_propagate keeps the norm
+    up to unbiased rounding.  The FP32 path applies the factor as its rounded value
+    plus its residual; at angles below ~2.4e-4 the cosine rounds to exactly 1, and a
+    residual added to an exact product is rounded away, always the same way: 1000
+    steps at 1e-4 rad then gained (1 - cos) per half step, ~1e-5 of norm.  A mass of
+    1e12 makes the kinetic factor's angles tiny too.  The state has a random phase per
+    point, so that its norm is spread over all Fourier modes and the unbiased rounding
+    of each transform averages out (a smooth state, carried by a few modes, shows
+    ~1e-6 of rounding noise over these steps)."""
+    shape, step = (32, 32, 32), 0.5 / L.AUTOANG
+    lib = L.lib()
+    rp = np.exp(2j * np.pi * np.random.default_rng(3).random(shape))
+    psi0 = inputs.smooth_modulated(shape, step, 0.0218 * L.AUTOANG ** 3, 0.3, seed=11) * rp
+    dt = 200.0
+    V = 2.0 * angle / dt  # V dt / 2 = angle
+    w = L.Wave(shape, step, L.GRID_FP32, mass=1e12)
+    w.set(psi0.astype(np.complex64).astype(complex))
+    pot = L.CField(shape, step, L.GRID_FP32)
+    pot.set(np.full(shape, V + 0j))
+    n0 = lib.grid_wf_norm(w.h)
+    for _ in range(1000):
+        lib.grid_wf_propagate(w.h, pot.h, L.Complex.of(dt))
+    drift = lib.grid_wf_norm(w.h) / n0 - 1
+    pot.free(), w.free()
+    assert abs(drift) < 1e-6, drift
