@@ -282,6 +282,30 @@ def size_sweep(steps=10, warmup=3):
          2.0 / L.AUTOFS),
     ]
     out = []
+    # config 1's pattern: imaginary-time steps renormalised every step, fused and as the
+    # reference driver does it (the step, then grid_wf_normalize)
+    n, step, prec = 64, 0.5 / L.AUTOANG, L.GRID_FP64
+    log("size sweep: plain OT 64^3 imaginary time, renormalised every step")
+    r = Run(n, step, prec, inputs.noisy_bulk((n, n, n), rho0, 0.01), 0)
+    r.w.h.contents.norm = L.lib().grid_wf_norm(r.w.h)
+    z = L.Complex.of(-1j * 10.0 / L.AUTOFS)
+    t = {}
+    k64 = max(steps, 200)  # 64^3 steps take ~0.1 ms
+    for kind in ("fused", "step_then_normalize"):
+        for it in range(warmup + k64):
+            if it == warmup:
+                L.lib().grid_synchronize()
+                t0 = time.perf_counter()
+            if kind == "fused":
+                L.lib().dft_ot_propagate_normalize(r.f.h, r.w.h, None, r.mu, z)
+            else:
+                L.lib().dft_ot_propagate(r.f.h, r.w.h, None, r.mu, z)
+                L.lib().grid_wf_normalize(r.w.h)
+        L.lib().grid_synchronize()
+        t[kind] = (time.perf_counter() - t0) / k64
+    r.free()
+    out.append({"workload": "plain OT 64^3 imaginary time, renormalised every step", "grid": n, "dtype": "f64",
+                "steps_per_s": 1.0 / t["fused"], "steps_per_s_step_then_normalize": 1.0 / t["step_then_normalize"]})
     for name, n, h_ang, prec, model, make, dt in cases:
         step = h_ang / L.AUTOANG
         log("size sweep: %s (%s)" % (name, "FP32" if prec == L.GRID_FP32 else "FP64"))

@@ -350,6 +350,13 @@ REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z);
 void dft_ot_propagate(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REAL_COMPLEX time);
 /* Predict-correct step of examples/imag_time/imag_time.c:231-239 (gwfp gets the predictor). */
 void dft_ot_propagate_pc(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu, REAL_COMPLEX time);
+/* dft_ot_propagate / dft_ot_propagate_pc followed by grid_wf_normalize (psi *= sqrt(gwf->norm / N)),
+   as examples/imag_time_rot/imag_time.c:198-208 does; the norm is summed in the step's closing
+   z pass and applied on the device, with no host round trip.
+   0 on success, -1 (with a libdft: message) for invalid arguments. */
+int dft_ot_propagate_normalize(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REAL_COMPLEX time);
+int dft_ot_propagate_pc_normalize(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu,
+                                  REAL_COMPLEX time);
 /* KC potential mode: 0 = reference formula (default), 1 = exact dE/drho. */
 void dft_ot_set_kc_mode(dft_ot_functional *otf, int exact);
 /* Energy of the functional, h^3 sum eps_OT, in FP64 (kinetic part excluded). */

@@ -125,6 +125,8 @@ _SIGS = {
     "grid_bench_copy": (None, ["rg", "rg"]),
     "dft_ot_propagate": (None, ["otf", "wf", "rg", "d", "z"]),
     "dft_ot_propagate_pc": (None, ["otf", "wf", "wf", "rg", "d", "z"]),
+    "dft_ot_propagate_normalize": ("i", ["otf", "wf", "rg", "d", "z"]),
+    "dft_ot_propagate_pc_normalize": ("i", ["otf", "wf", "wf", "rg", "d", "z"]),
     "dft_ot_potential_vjp": ("i", ["otf", "rg", "rg", "wf"]),
     "dft_ot_potential_vjp_psi": ("i", ["otf", "cg", "cg", "wf"]),
     "rgrid_value_at_index": ("d", ["rg", "i64", "i64", "i64"]),

@@ -306,6 +306,45 @@ __global__ void __launch_bounds__(256)
   if (DIR > 0 && partial) block_sum_256(acc, partial);
 }
 
+// Total of the per-block partial sums partial[0 .. nb), in every thread of a
+// 256-thread block.
+__device__ __forceinline__ double block_total_256(const double *partial, int nb) {
+  __shared__ double s_tot[4];
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) t += partial[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_tot[threadIdx.x >> 6] = t;
+  __syncthreads();
+  return (s_tot[0] + s_tot[1]) + (s_tot[2] + s_tot[3]);
+}
+
+// psi * f for a real FP64 factor f: in FP32 applied as its rounded value plus
+// its residual.
+template <typename T> __device__ __forceinline__ cplx<T> scale_by(cplx<T> v, double f) {
+  if constexpr (sizeof(T) == 4) {
+    const float h = (float)f, l = (float)(f - (double)h);
+    return cmk<cplx<T>>(__builtin_fmaf(v.x, h, v.x * l), __builtin_fmaf(v.y, h, v.y * l));
+  } else {
+    return cmk<cplx<T>>(v.x * f, v.y * f);
+  }
+}
+
+// Renormalisation fused into a step's closing z pass (grid_wf_normalize): after
+// k_zc2c_pot<+1> wrote its per-block sums of |psi|^2, one block turns them into
+// psi's factor sqrt(target / (h3 sum)), stored after them, and k_scale_psi
+// applies it; no host round trip.
+static __global__ void k_norm_factor(double *partial, int nb, double h3, double target) {
+  const double tot = block_total_256(partial, nb) * h3;
+  if (threadIdx.x == 0) partial[nb] = tot > 0.0 ? sqrt(target / tot) : 1.0;
+}
+template <typename T> __global__ void k_scale_psi(cplx<T> *psi, const double *f, int64_t n) {
+  const double s = *f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    psi[i] = scale_by<T>(psi[i], s);
+}
+
 // Fused x pass of the propagator: forward x FFT, kinetic phase
 // exp(-i k^2 dt / 2M), inverse x FFT.  Field is [nx][ny][nz] complex.
 // The phase factorises as X(kx) YZ(ky, kz); both factors are evaluated in FP64

@@ -44,12 +44,18 @@ int64_t strang_partials(int nx, int ny, int nz, int prec);
 // row pitch `pitch`, in place.
 template <typename T>
 void ypass(const Plan3D<T> &p, int nb, cplx<T> *const *bufs, int dir, int cols, int pitch, hipStream_t s);
+// Renormalisation fused into a step's closing z pass: psi *= sqrt(target /
+// (h3 sum |psi|^2)).  partial: strang_partials() + 1 doubles of device memory.
+struct Renorm {
+  double *partial;
+  double h3, target;
+};
 // Rest of a Strang step after the fused z pass: y, x (with kinetic phase), y,
 // and the inverse z pass multiplying by e^{-iV dt/2} with V = `phase` (the
-// potential the first half step used).
+// potential the first half step used); with rn, that pass also renormalises.
 template <typename T>
 void strang_tail(const Plan3D<T> &p, cplx<T> *psi, const cplx<T> *phase, double dtr, double dti, double mass,
-                 hipStream_t s, const double *k0 = nullptr);
+                 hipStream_t s, const double *k0 = nullptr, const Renorm *rn = nullptr);
 // Full complex forward transform (out of place).
 template <typename T> void c2c_fwd(const Plan3D<T> &p, const cplx<T> *in, cplx<T> *out, hipStream_t s);
 // Its inverse, normalised by 1/N (out of place).

@@ -797,6 +797,7 @@ struct dft_ot_device {
   void *kern[4];             // real half spectra (LJ, SA, F, U)
   void *phase;               // complex: V_eff of the current step, whose e^{-iV dt/2} both z passes apply
   void *vsave;               // complex: predictor potential V_n of a predict-correct step
+  double *norm_part;         // per-block sums |psi|^2 and the factor of a renormalising step
   // x-middle programs of the fused schedule in device memory, and the host
   // copies they were uploaded from (re-uploaded only when one changes)
   fused::MidProg *mid_dev;
@@ -1611,6 +1612,7 @@ void free_device(dft_ot_device *d) {
   for (int i = 0; i < 4; ++i) dfree(d->kern[i]);
   dfree(d->phase);
   dfree(d->vsave);
+  dfree(d->norm_part);
   dfree(d->mid_dev);
   if (d->plan) {
     if (d->prec == GRID_FP32) {
@@ -2341,6 +2343,9 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
   // the x-middle programs of the fused schedule
   if (ok) ok = (d->phase = dmalloc(2 * es * d->n)) != nullptr;
   if (ok) ok = (d->vsave = dmalloc(2 * es * d->n)) != nullptr;
+  if (ok)
+    ok = (d->norm_part = (double *)dmalloc(sizeof(double) * (strang_partials(d->nx, d->ny, d->nz, d->prec) + 1))) !=
+         nullptr;
   if (ok) ok = (d->mid_dev = (fused::MidProg *)dmalloc(3 * sizeof(fused::MidProg))) != nullptr;
   if (ok) {
     if (d->prec == GRID_FP32) {
@@ -2452,6 +2457,52 @@ void dft_ot_propagate_pc(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext,
     strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwf->grid->dev, (const cplx<T> *)d->phase, __real__ time,
                    __imag__ time, gwf->mass, lib_stream(), k0);
   });
+}
+
+// dft_ot_propagate / dft_ot_propagate_pc followed by grid_wf_normalize
+// (examples/imag_time_rot/imag_time.c:198-208), the normalisation fused into
+// the step's closing z pass.
+static int propagate_normalize(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu, REAL_COMPLEX time,
+                               const char *who) {
+  if (!check_wf(otf, gwf, who) || (gwfp && !check_wf(otf, gwfp, who)) || (vext && !check_r(otf, vext, who)))
+    return -1;
+  if (!(gwf->norm > 0.0)) {
+    fprintf(stderr, "libdft: %s: the wave function's norm target is %g\n", who, (double)gwf->norm);
+    return -1;
+  }
+  dft_ot_device *d = otf->device;
+  const void *ve = vext ? vext->dev : nullptr;
+  const double k0[3] = {gwf->grid->kx0, gwf->grid->ky0, gwf->grid->kz0};
+  const Renorm rn{d->norm_part, d->step * d->step * d->step, gwf->norm};
+  DFT_BY_PREC(d->prec, {
+    if (gwfp) {
+      StepIn pred{ve, mu, nullptr, d->vsave, d->phase, gwf->grid->dev, gwfp->grid->dev, __real__ time, __imag__ time};
+      StepIn corr{ve, mu, d->vsave, nullptr, d->phase, gwf->grid->dev, gwf->grid->dev, __real__ time, __imag__ time};
+      fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &pred);
+      strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwfp->grid->dev, (const cplx<T> *)d->phase, __real__ time,
+                     __imag__ time, gwf->mass, lib_stream(), k0);
+      fused_impl<T>(otf, (const cplx<T> *)gwfp->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &corr);
+    } else {
+      StepIn st{ve, mu, nullptr, nullptr, d->phase, gwf->grid->dev, gwf->grid->dev, __real__ time, __imag__ time};
+      fused_impl<T>(otf, (const cplx<T> *)gwf->grid->dev, gwf->mass, (T *)otf->density->dev, nullptr, &st);
+    }
+    strang_tail<T>(plan_of<T>(d), (cplx<T> *)gwf->grid->dev, (const cplx<T> *)d->phase, __real__ time,
+                   __imag__ time, gwf->mass, lib_stream(), k0, &rn);
+  });
+  return 0;
+}
+
+int dft_ot_propagate_normalize(dft_ot_functional *otf, wf *gwf, rgrid *vext, REAL mu, REAL_COMPLEX time) {
+  return propagate_normalize(otf, gwf, nullptr, vext, mu, time, "dft_ot_propagate_normalize");
+}
+
+int dft_ot_propagate_pc_normalize(dft_ot_functional *otf, wf *gwf, wf *gwfp, rgrid *vext, REAL mu,
+                                  REAL_COMPLEX time) {
+  if (!gwfp) {
+    fprintf(stderr, "libdft: dft_ot_propagate_pc_normalize: no predictor wave function\n");
+    return -1;
+  }
+  return propagate_normalize(otf, gwf, gwfp, vext, mu, time, "dft_ot_propagate_pc_normalize");
 }
 
 // src/ot.c:431-510

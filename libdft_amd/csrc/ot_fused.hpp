@@ -549,7 +549,7 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DF
 #define DFT_ZAB_SPLIT64 1
 #endif
 #ifndef DFT_ZAB_SPLIT_MINB64
-#define DFT_ZAB_SPLIT_MINB64 2
+#define DFT_ZAB_SPLIT_MINB64 0
 #endif
 // Early loads in zAB / zBC, so that memory round trips overlap the transforms
 // (bit masks, FP32; FP64 holds twice the registers and spills at its
