@@ -106,7 +106,7 @@ __device__ __forceinline__ void load_merge(Regs<T, N, EM> &v, const void *in, ui
 // ------------------------------------------------ z pass: real -> half spec --
 // Pairs of adjacent real z-lines (2p, 2p+1) -> half spectra rows 2p, 2p+1.
 template <typename T, int N>
-__global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__((LineCfg<T, N>::BS)) k_zr2c(Batch b, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
   using C = cplx<T>;
   using G = LineCfg<T, N>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, co
   const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = p0 < npairs;
   const uint32_t p = act ? p0 : 0;  // out-of-range lanes read pair 0 and store nothing
-  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   const uint32_t e0 = p * 2u * N + (uint32_t)t;
   C v[E];
 #pragma unroll
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(256) k_zr2c(Batch b, int64_t npairs, int P, co
 
 // ------------------------------------------------ z pass: half spec -> real --
 template <typename T, int N>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((LineCfg<T, N>::BS))
     k_zc2r(Batch b, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw, T scale) {
   using C = cplx<T>;
   using G = LineCfg<T, N>;
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t p0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = p0 < npairs;
   const uint32_t p = act ? p0 : 0;
-  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   C v[E];
   load_merge<T, N, G::EM>(v, in, p, P, t);
   __syncthreads();
@@ -250,13 +250,19 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
 
 // ---------------------------------------- propagator passes (complex psi) --
 // Sum of an FP64 value over the block (256 threads) into partial[blockIdx.x].
-__device__ __forceinline__ void block_sum_256(double acc, double *partial) {
-  __shared__ double s_red[4];
+template <int BS> __device__ __forceinline__ void block_sum(double acc, double *partial) {
+  static_assert(BS == 64 || BS == 128 || BS == 256, "block of 1, 2 or 4 wavefronts");
+  __shared__ double s_red[BS / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+  if (threadIdx.x == 0) {
+    double t = s_red[0];
+    if constexpr (BS == 128) t += s_red[1];
+    if constexpr (BS == 256) t = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    partial[blockIdx.x] = t;
+  }
 }
 
 // z pass over a full complex field, multiplying by e^{-iV dt/2} (V = pot,
@@ -264,7 +270,7 @@ __device__ __forceinline__ void block_sum_256(double acc, double *partial) {
 // transform.  After the inverse pass each block writes sum |psi|^2 to
 // partial[block] (FP64) when `partial` is given.
 template <typename T, int N, int DIR>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((LineCfg<T, N>::BS))
     k_zc2c_pot(const cplx<T> *src, cplx<T> *dst, const cplx<T> *__restrict__ pot, int64_t nlines,
                const cplx<T> *__restrict__ g_tw, double dtr, double dti, T scale, double *__restrict__ partial) {
   using C = cplx<T>;
@@ -276,7 +282,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t l0 = (uint32_t)blockIdx.x * G::LINES + ln;
   const bool act = l0 < nlines;
   const uint32_t l = act ? l0 : 0;
-  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   const uint32_t e0 = l * (uint32_t)N + (uint32_t)t;
   C v[E];
 #pragma unroll
@@ -303,7 +309,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < E; ++i) ste<C>(dst, e0, v[i], TPL * i);
   }
-  if (DIR > 0 && partial) block_sum_256(acc, partial);
+  if (DIR > 0 && partial) block_sum<G::BS>(acc, partial);
 }
 
 // Total of the per-block partial sums partial[0 .. nb), in every thread of a

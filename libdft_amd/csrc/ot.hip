@@ -1050,8 +1050,8 @@ template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho,
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (BF) hipLaunchKernelGGL((fused::k_zA<T, N, true>), grid, dim3(256), 0, s, psi, rho, o, npairs, P, pw, tw);
-  else hipLaunchKernelGGL((fused::k_zA<T, N, false>), grid, dim3(256), 0, s, psi, rho, o, npairs, P, pw, tw);
+  if (BF) hipLaunchKernelGGL((fused::k_zA<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, psi, rho, o, npairs, P, pw, tw);
+  else hipLaunchKernelGGL((fused::k_zA<T, N, false>), grid, dim3(fused::PairBS<T, N>), 0, s, psi, rho, o, npairs, P, pw, tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_zA_density_r2c", bytes);
 }
@@ -1142,11 +1142,11 @@ void launch_zAB1(const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   if constexpr (BF && sizeof(T) == 8 && DFT_ZAB_SPLIT64) {
     // FP64: the backflow terms, then the local and KC terms (V_re through memory)
-    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 1>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
     DFT_LAUNCH_CHECK();
-    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 2>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   } else {
-    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   }
 }
 
@@ -1168,8 +1168,8 @@ void launch_zBC(bool BF, const fused::ZBCArgs &a, int64_t npairs, int P, const P
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (BF) hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
-  else hipLaunchKernelGGL((fused::k_zBC<T, N, false>), grid, dim3(256), 0, s, a, npairs, P, pw, tw);
+  if (BF) hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+  else hipLaunchKernelGGL((fused::k_zBC<T, N, false>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_zBC_c2r_kc_terms_r2c", bytes);
 }
@@ -1180,8 +1180,8 @@ void launch_zC(bool step, const fused::ZCArgs &a, int64_t npairs, int P, const c
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (step) hipLaunchKernelGGL((fused::k_zC<T, N, true>), grid, dim3(256), 0, s, a, npairs, P, tw);
-  else hipLaunchKernelGGL((fused::k_zC<T, N, false>), grid, dim3(256), 0, s, a, npairs, P, tw);
+  if (step) hipLaunchKernelGGL((fused::k_zC<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, tw);
+  else hipLaunchKernelGGL((fused::k_zC<T, N, false>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, step ? "ot_zC_c2r_phase_z_fwd" : "ot_zC_c2r_accumulate", bytes);
 }

@@ -123,6 +123,9 @@ template <int N> constexpr int pair_em() { return 4; }
 template <int N>
 using PairSync = std::conditional_t<(rf::RP<N, pair_em<N>()>::TPL <= 64), rf::WaveSync, rf::LineBlockSync>;
 
+// Threads per block of the pair kernels.
+template <typename T, int N> constexpr int PairBS = LineCfg<T, N, pair_em<N>()>::BS;
+
 // Common prologue of the pair kernels.
 #define FUSED_PAIR_SETUP()                                                   \
   using C = cplx<T>;                                                         \
@@ -136,7 +139,7 @@ using PairSync = std::conditional_t<(rf::RP<N, pair_em<N>()>::TPL <= 64), rf::Wa
   /* out-of-range lanes compute on pair 0 and store nothing */               \
   const uint32_t p = act ? p0 : 0;                                           \
   C *X = s_x + ln * LP;                                                      \
-  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);                                  \
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);                                 \
   __syncthreads();                                                           \
   const PairPos<T, N, TPL> q(p, t, pw)
 
@@ -168,7 +171,7 @@ template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { retur
 #define DFT_ZC_MINB 0
 #endif
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZA_MINB : 0)
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB : 0)
     k_zA(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
          const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
@@ -430,7 +433,8 @@ template <typename T, int N, bool ONE>
 using XConvCfg = ColCfg<T, N, N >= 512 ? 16 : (ONE ? DFT_XCONV1_EM : DFT_XCONV_EM),
                         N >= 512 ? 16
                                  : (ONE ? (sizeof(T) == 8 ? DFT_XCONV1_COLS64 : DFT_XCONV1_COLS)
-                                        : (sizeof(T) == 8 && DFT_XCONV_COLS > 32 ? 32 : DFT_XCONV_COLS))>;
+                                        : (sizeof(T) == 8 && DFT_XCONV_COLS > 32 ? 32 : DFT_XCONV_COLS)),
+                        !(ONE && N == 128)>;
 
 template <typename T, int N, bool DERIV, bool ONE>
 __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DFT_XCONV_MINB : DFT_XCONV_MINB64) k_xconv(const MidProg *__restrict__ pgp, int m0, int nl, int ny,
@@ -594,7 +598,7 @@ struct ZABArgs {
 // live fields, PART 1 (the backflow terms, V_re stored) then PART 2 (the local
 // and KC terms, added to the stored V_re).
 template <typename T, int N, bool KC, bool BF, int PART = 0>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZAB_MINB : (PART ? DFT_ZAB_SPLIT_MINB64 : DFT_ZAB_MINB64))
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB : (PART ? DFT_ZAB_SPLIT_MINB64 : DFT_ZAB_MINB64))
     k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   static_assert(PART == 0 || BF, "only the backflow stage splits");
   FUSED_PAIR_SETUP();
@@ -799,7 +803,7 @@ struct ZBCArgs {
 // stencils of rho and rho~, the fields accumulated into), so that their
 // latency overlaps the spectrum's loads and transform.
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZBC_MINB : DFT_ZBC_MINB64) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB : DFT_ZBC_MINB64) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   constexpr int PF = prefetch_bits<T, DFT_ZBC_PF>();
   const void *rho = a.rho, *rhot = a.rhot;
@@ -912,7 +916,7 @@ struct ZCArgs {
 };
 
 template <typename T, int N, bool STEP>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZCArgs a, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZCArgs a, int64_t npairs, int P, const cplx<T> *__restrict__ g_tw) {
   using C = cplx<T>;
   using G = LineCfg<T, N, pair_em<N>()>;
   constexpr int E = G::E, TPL = G::TPL, LP = G::LP;
@@ -923,7 +927,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? DFT_ZC_MINB : 0) k_zC(ZC
   const bool act = p0 < npairs;
   const uint32_t p = act ? p0 : 0;
   C *X = s_x + ln * LP;
-  rf::load_tw<T, N, G::EM, 256>(s_tw, g_tw);
+  rf::load_tw<T, N, G::EM, G::BS>(s_tw, g_tw);
   __syncthreads();
   constexpr bool PF = prefetch_bits<T, DFT_ZC_PF>() != 0;
   C v[E];
