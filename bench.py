@@ -40,7 +40,7 @@ SYMBOL = {"ot_xconv_single": "dft::fused::k_xconv<float, 256, false, true>",
           "ot_xmid_fused": "dft::fused::k_xmid<float, 256, 1, true>",
           "fft_y_c2c_fwd": "dft::fft::k_strided<float, 256, -1>",
           "fft_y_c2c_inv": "dft::fft::k_strided<float, 256, 1>",
-          "ot_zAB_c2r_local_kc_bf_r2c": "dft::fused::k_zAB<float, 256, true, true>",
+          "ot_zAB_c2r_local_kc_bf_r2c": "dft::fused::k_zAB<float, 256, true, true, 0>",
           "ot_zBC_c2r_kc_terms_r2c": "dft::fused::k_zBC<float, 256, true>",
           "ot_zA_density_r2c": "dft::fused::k_zA<float, 256, true>",
           "ot_zC_c2r_phase_z_fwd": "dft::fused::k_zC<float, 256, true>",
@@ -213,11 +213,11 @@ def summarize_counters(csvs, kernels=None):
     """Markdown table of per-dispatch counter averages from rocprofv3 --pmc
     counter_collection.csv files (one pass each), one row per kernel of the
     256^3 FP32 step, with derived ratios: wave lifetime (SQ_WAVE_CYCLES /
-    SQ_WAVES, in cycles of the SQ clock per wave), the share of wave cycles
-    spent waiting on any instruction dependency (SQ_WAIT_INST_ANY), issuing
-    VALU, VMEM and LDS instructions, SQ_LEVEL_WAVES / SQ_BUSY_CYCLES (the
-    resident-wave level per busy SQ cycle, proportional to occupancy) and
-    the L2 hit rate."""
+    SQ_WAVES, in SQ clock cycles per wave), the shares of wave cycles spent
+    waiting for a dependency (SQ_WAIT_ANY: memory or LDS results) and waiting
+    to issue (SQ_WAIT_INST_ANY: barriers, issue stalls), and issuing VALU and
+    LDS instructions, instruction counts per wave, LDS bank-conflict cycles
+    per LDS instruction and the L2 hit rate."""
     import csv
     import collections
     vals = collections.defaultdict(lambda: collections.defaultdict(list))
@@ -226,14 +226,15 @@ def summarize_counters(csvs, kernels=None):
             vals[_kernel_symbol(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
     wanted = kernels or [SYMBOL[k] for k in SYMBOL]
     cols = [("wave cycles", lambda a: a["SQ_WAVE_CYCLES"] / a["SQ_WAVES"]),
-            ("wait %", lambda a: 100 * a["SQ_WAIT_INST_ANY"] / a["SQ_WAVE_CYCLES"]),
+            ("dependency wait %", lambda a: 100 * a["SQ_WAIT_ANY"] / a["SQ_WAVE_CYCLES"]),
+            ("issue wait %", lambda a: 100 * a["SQ_WAIT_INST_ANY"] / a["SQ_WAVE_CYCLES"]),
             ("VALU %", lambda a: 100 * a["SQ_ACTIVE_INST_VALU"] / a["SQ_WAVE_CYCLES"]),
-            ("VMEM %", lambda a: 100 * a["SQ_ACTIVE_INST_VMEM"] / a["SQ_WAVE_CYCLES"]),
             ("LDS %", lambda a: 100 * a["SQ_ACTIVE_INST_LDS"] / a["SQ_WAVE_CYCLES"]),
-            ("level/busy", lambda a: a["SQ_LEVEL_WAVES"] / a["SQ_BUSY_CYCLES"]),
             ("VALU insts/wave", lambda a: a["SQ_INSTS_VALU"] / a["SQ_WAVES"]),
             ("VMEM rd/wave", lambda a: a["SQ_INSTS_VMEM_RD"] / a["SQ_WAVES"]),
+            ("VMEM wr/wave", lambda a: a["SQ_INSTS_VMEM_WR"] / a["SQ_WAVES"]),
             ("LDS insts/wave", lambda a: a["SQ_INSTS_LDS"] / a["SQ_WAVES"]),
+            ("bank conflict/LDS inst", lambda a: a["SQ_LDS_BANK_CONFLICT"] / a["SQ_INSTS_LDS"]),
             ("L2 hit %", lambda a: 100 * a["TCC_HIT_sum"] / (a["TCC_HIT_sum"] + a["TCC_MISS_sum"]))]
     lines = ["| kernel | " + " | ".join(c for c, _ in cols) + " |", "|---" * (len(cols) + 1) + "|"]
     for k in wanted:
