@@ -1168,8 +1168,21 @@ void launch_zBC(bool BF, const fused::ZBCArgs &a, int64_t npairs, int P, const P
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (BF) hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
-  else hipLaunchKernelGGL((fused::k_zBC<T, N, false>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+  constexpr bool SPLIT = sizeof(T) == 8 && DFT_ZBC_SPLIT64;
+  if constexpr (SPLIT) {
+    // FP64: the J_d stages, then T1, the tail and the r2c of H (H and the term-3 sum through memory)
+    if (BF) {
+      hipLaunchKernelGGL((fused::k_zBC<T, N, true, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+      hipLaunchKernelGGL((fused::k_zBC<T, N, true, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+    } else {
+      hipLaunchKernelGGL((fused::k_zBC<T, N, false, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+      hipLaunchKernelGGL((fused::k_zBC<T, N, false, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+    }
+  } else if (BF) {
+    hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+  } else {
+    hipLaunchKernelGGL((fused::k_zBC<T, N, false>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+  }
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_zBC_c2r_kc_terms_r2c", bytes);
 }
@@ -1343,6 +1356,9 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     a.wz = d->slot[WZ];
     a.vre = d->slot[VRE];
     a.vim = d->slot[VIM];
+    // scratch of the split form: the G_d spectra in QX, QY are consumed by the x-middle above
+    a.hbuf = d->slot[QX];
+    a.tbuf = d->slot[QY];
     const double bytes = 4 * H + 3 * R + (BF ? 4 * R : 0) + H + R + (BF ? R : 0);
     DFT_DISPATCH_N(d->nz, (launch_zBC<T, N_>(BF, a, npairs, PP, pw, p.twz, s, bytes)));
     cplx<T> *hb[1] = {CS<T>(d, SH)};

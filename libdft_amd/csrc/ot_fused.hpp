@@ -552,6 +552,12 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DF
 #ifndef DFT_ZAB_SPLIT64
 #define DFT_ZAB_SPLIT64 1
 #endif
+#ifndef DFT_ZBC_SPLIT64
+#define DFT_ZBC_SPLIT64 1
+#endif
+#ifndef DFT_ZBC_SPLIT_MINB64
+#define DFT_ZBC_SPLIT_MINB64 0
+#endif
 #ifndef DFT_ZAB_SPLIT_MINB64
 #define DFT_ZAB_SPLIT_MINB64 0
 #endif
@@ -797,24 +803,29 @@ struct ZBCArgs {
   Ptrs8 out;  // H
   const void *rho, *rhot, *wx, *wy, *wz;
   void *vre, *vim;
+  void *hbuf, *tbuf;  // split form: H and the term-3 sum between the two kernels (real fields)
 };
 
 // Every stage first issues the loads that do not depend on its spectrum (the
 // stencils of rho and rho~, the fields accumulated into), so that their
 // latency overlaps the spectrum's loads and transform.
-template <typename T, int N, bool BF>
-__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB : DFT_ZBC_MINB64) k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+// PART 0: the whole stage; PART 1 (the three J_d stages, H and the term-3
+// sum stored) then PART 2 (T1, the tail and the r2c of H): the split form FP64
+// runs with fewer live fields.
+template <typename T, int N, bool BF, int PART = 0>
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB : (PART ? DFT_ZBC_SPLIT_MINB64 : DFT_ZBC_MINB64))
+    k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   constexpr int PF = prefetch_bits<T, DFT_ZBC_PF>();
   const void *rho = a.rho, *rhot = a.rhot;
   C v[E];
-  T H[2][E], t3[2][E];
+  T H[2][PART == 2 ? 1 : E], t3[2][E];
   fft::RawPair<T, N, pair_em<N>()> raw;
   auto c2r = [&](const void *cur, const void *next) {
     if constexpr ((PF & 8) != 0) c2r_pipe<T, N>(v, raw, X, s_tw, t, next, p, P);
     else load_c2r<T, N>(v, X, s_tw, t, cur, p, P);
   };
-  if constexpr ((PF & 8) != 0) fft::load_raw<T, N, pair_em<N>()>(raw, a.in.p[0], p, P, t);
+  if constexpr ((PF & 8) != 0) fft::load_raw<T, N, pair_em<N>()>(raw, a.in.p[PART == 2 ? 3 : 0], p, P, t);
   auto axis = [&](auto dtag) {
     constexpr int D = decltype(dtag)::value;
     T g[2][E], Pd[2][E];
@@ -846,9 +857,31 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB 
       }
     }
   };
-  axis(std::integral_constant<int, 0>{});
-  axis(std::integral_constant<int, 1>{});
-  axis(std::integral_constant<int, 2>{});
+  if constexpr (PART != 2) {
+    axis(std::integral_constant<int, 0>{});
+    axis(std::integral_constant<int, 1>{});
+    axis(std::integral_constant<int, 2>{});
+  }
+  if constexpr (PART == 1) {
+    // first kernel of the split form: H and the term-3 sum to memory
+    if (act) {
+      const auto qs = q.here();
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          qs.template put<T>(a.hbuf, w, i, H[w][i]);
+          qs.template put<T>(a.tbuf, w, i, t3[w][i]);
+        }
+    }
+    return;
+  } else if constexpr (PART == 2) {
+    const auto qs = q.here();
+#pragma unroll
+    for (int i = 0; i < E; ++i)
+#pragma unroll
+      for (int w = 0; w < 2; ++w) t3[w][i] = qs.template at_once<T>(a.tbuf, w, i);
+  }
   // rho_st, V_re so far, and the backflow imaginary part (which does not
   // involve T1); with prefetching, all before T1's transform
   std::remove_const_t<decltype(q)> qt;
@@ -894,8 +927,14 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB 
           bf_im(w, i);
         }
   }
+  if constexpr (PART == 2) {
+    const auto qs = q.here();
 #pragma unroll
-  for (int i = 0; i < E; ++i) v[i] = cmk<C>(H[0][i], H[1][i]);
+    for (int i = 0; i < E; ++i) v[i] = cmk<C>(qs.template at_once<T>(a.hbuf, 0, i), qs.template at_once<T>(a.hbuf, 1, i));
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = cmk<C>(H[0][i], H[1][i]);
+  }
   r2c_store<T, N>(v, X, s_tw, t, a.out.p[0], p, P, act);
 }
 
