@@ -47,10 +47,13 @@ def test_value_at_index_outside_is_periodic(prec):
 
 
 @pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
-def test_moving_frame(prec):
+@pytest.mark.parametrize("shape", [SHAPE, (32, 32, 16)], ids=["8x16x32", "32x32x16"])
+def test_moving_frame(prec, shape):
     """With k0 = the plane wave's own k, the kinetic symbol (k - k0)^2 / 2M vanishes:
     free propagation leaves the wave unchanged and its kinetic energy is zero.
-    With k0 != k the phase is exp(-i (k - k0)^2 dt / 2M)."""
+    With k0 != k the phase is exp(-i (k - k0)^2 dt / 2M).  (32 x 32 x 16 takes the
+    small-grid block shapes.)"""
+    SHAPE = shape
     x, y, z = np.meshgrid(*[np.arange(n) * STEP for n in SHAPE], indexing="ij")
     kv = np.array([2 * np.pi * 2 / (SHAPE[0] * STEP), 2 * np.pi * -3 / (SHAPE[1] * STEP), 2 * np.pi * 5 / (SHAPE[2] * STEP)])
     psi = np.exp(1j * (kv[0] * x + kv[1] * y + kv[2] * z))

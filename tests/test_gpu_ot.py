@@ -46,8 +46,9 @@ def test_free_plane_wave_phase(prec):
 
 
 @pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
-@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 32, 16), (64, 32, 128)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 32, 16), (64, 32, 128), (32, 32, 16), (64, 64, 32)])
 def test_propagate_matches_oracle(oracle, prec, shape):
+    """(32, 32, 16) and (64, 64, 32) take the small-grid block shapes (64-thread line and column blocks)."""
     step = 0.9
     psi = inputs.smooth_modulated(shape, step, 1.0, 0.4)
     u = inputs.splitmix64_uniform(2 * psi.size, 5).reshape((2,) + shape)
@@ -285,9 +286,11 @@ def test_energy_gradient_fd_128(oracle):
 # ---------------------------------------------------------- fused stepping --
 @pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
 @pytest.mark.parametrize("model", [0, 1, 8, 9], ids=["plain", "kc", "bf", "full"])
-def test_fused_single_step_matches_oracle(oracle, prec, model):
-    """dft_ot_propagate == zero/potential/-mu/propagate of the reference single-step driver."""
-    shape, step = (16, 32, 32), 0.5 / L.AUTOANG
+@pytest.mark.parametrize("shape", [(16, 32, 32), (32, 32, 16)], ids=["16x32x32", "32x32x16"])
+def test_fused_single_step_matches_oracle(oracle, prec, model, shape):
+    """dft_ot_propagate == zero/potential/-mu/propagate of the reference single-step driver
+    (32 x 32 x 16: the small-grid block shapes)."""
+    step = 0.5 / L.AUTOANG
     o = oracle.OracleOT(model, shape, step, 2, 8)
     psi0 = inputs.smooth_modulated(shape, step, o.p.rho0, 0.2)
     vext = inputs.gaussian_well(shape, step)
