@@ -552,6 +552,11 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DF
 #ifndef DFT_ZAB_SPLIT64
 #define DFT_ZAB_SPLIT64 1
 #endif
+// (the same split in FP32, where neither half gets past 2 waves per SIMD without
+// spills, measured slower: zAB 375 -> 386 us at 256^3)
+#ifndef DFT_ZAB_SPLIT32
+#define DFT_ZAB_SPLIT32 0
+#endif
 #ifndef DFT_ZBC_SPLIT64
 #define DFT_ZBC_SPLIT64 1
 #endif
