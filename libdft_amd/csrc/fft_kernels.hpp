@@ -249,7 +249,7 @@ __global__ void __launch_bounds__((ColCfg<T, N>::BS))
 }
 
 // ---------------------------------------- propagator passes (complex psi) --
-// Sum of an FP64 value over the block (256 threads) into partial[blockIdx.x].
+// Sum of an FP64 value over the block (BS threads) into partial[blockIdx.x].
 template <int BS> __device__ __forceinline__ void block_sum(double acc, double *partial) {
   static_assert(BS == 64 || BS == 128 || BS == 256, "block of 1, 2 or 4 wavefronts");
   __shared__ double s_red[BS / 64];

@@ -537,34 +537,35 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DF
 #ifndef DFT_ZBC_MINB
 #define DFT_ZBC_MINB 2
 #endif
-// the same requests for the FP64 instantiations (0: none).  FP64 zBC at 2
-// waves per SIMD spills 240 bytes per lane and still beats 1 wave (256 VGPRs +
-// 56 AGPRs): 192.5 -> 198 steps/s at 256^3; zAB at 2 waves (412 bytes of
-// spills) is slower.
+// the same requests for the single FP64 kernels (0: none), used when the
+// split forms below are off.  FP64 zBC at 2 waves per SIMD spills 240 bytes per
+// lane and still beats 1 wave (256 VGPRs + 56 AGPRs): 192.5 -> 198 steps/s at
+// 256^3; zAB at 2 waves (412 bytes of spills) is slower.
 #ifndef DFT_ZAB_MINB64
 #define DFT_ZAB_MINB64 0
 #endif
 #ifndef DFT_ZBC_MINB64
 #define DFT_ZBC_MINB64 2
 #endif
-// FP64 backflow zAB as two kernels (1) or one (0), and the waves per SIMD
-// asked for each of the two
+// FP64 zAB (backflow) and zBC as two kernels each (1) or one (0), and the
+// waves per SIMD asked for the halves (0: none).  In FP64 neither half spills:
+// 200 -> 229 steps/s at 256^3 with both.
 #ifndef DFT_ZAB_SPLIT64
 #define DFT_ZAB_SPLIT64 1
-#endif
-// (the same split in FP32, where neither half gets past 2 waves per SIMD without
-// spills, measured slower: zAB 375 -> 386 us at 256^3)
-#ifndef DFT_ZAB_SPLIT32
-#define DFT_ZAB_SPLIT32 0
 #endif
 #ifndef DFT_ZBC_SPLIT64
 #define DFT_ZBC_SPLIT64 1
 #endif
+#ifndef DFT_ZAB_SPLIT_MINB64
+#define DFT_ZAB_SPLIT_MINB64 0
+#endif
 #ifndef DFT_ZBC_SPLIT_MINB64
 #define DFT_ZBC_SPLIT_MINB64 0
 #endif
-#ifndef DFT_ZAB_SPLIT_MINB64
-#define DFT_ZAB_SPLIT_MINB64 0
+// (the zAB split in FP32, where neither half gets past 2 waves per SIMD without
+// spills, measured slower: zAB 375 -> 386 us at 256^3)
+#ifndef DFT_ZAB_SPLIT32
+#define DFT_ZAB_SPLIT32 0
 #endif
 // Early loads in zAB / zBC, so that memory round trips overlap the transforms
 // (bit masks, FP32; FP64 holds twice the registers and spills at its
