@@ -265,6 +265,9 @@ template <int BS> __device__ __forceinline__ void block_sum(double acc, double *
   }
 }
 
+#ifndef DFT_ZINV_PF
+#define DFT_ZINV_PF 1
+#endif
 // z pass over a full complex field, multiplying by e^{-iV dt/2} (V = pot,
 // complex) before (DIR<0) or after (DIR>0, together with `scale`) the
 // transform.  After the inverse pass each block writes sum |psi|^2 to
@@ -293,6 +296,15 @@ __global__ void __launch_bounds__((LineCfg<T, N>::BS))
       v[i] = apply_vphase<T>(v[i], lde<C>(pot, e0, TPL * i), dtr, dti);
     }
   }
+  // inverse pass: the potential is read before the transform (DFT_ZINV_PF, FP32),
+  // so that its loads overlap the field's
+  constexpr bool PF = DIR > 0 && sizeof(T) == 4 && DFT_ZINV_PF;
+  C pv[PF ? E : 1];
+  if constexpr (PF) {
+    if (pot)
+#pragma unroll
+      for (int i = 0; i < E; ++i) pv[i] = lde<C>(pot, e0, TPL * i);
+  }
   __syncthreads();
   rf::rfft<T, N, G::EM, DIR, rf::WaveSync>(v, s_x + ln * LP, s_tw, t);
   double acc = 0.0;
@@ -300,7 +312,7 @@ __global__ void __launch_bounds__((LineCfg<T, N>::BS))
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       v[i] = cscale(v[i], scale);
-      if (pot) v[i] = apply_vphase<T>(v[i], lde<C>(pot, e0, TPL * i), dtr, dti);
+      if (pot) v[i] = apply_vphase<T>(v[i], PF ? pv[PF ? i : 0] : lde<C>(pot, e0, TPL * i), dtr, dti);
       acc += (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
     }
     if (!act) acc = 0.0;

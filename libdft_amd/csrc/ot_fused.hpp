@@ -401,9 +401,10 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
 // block (256-byte row segments in FP32), the accumulating form 16.
 // DFT_XCONV1_EM / DFT_XCONV1_COLS set that geometry for the single-term form,
 // DFT_XCONV_EM / DFT_XCONV_COLS for the accumulating one; DFT_XCONV_MINB asks
-// the register allocator for that many waves per SIMD (0: no request).  At 4
-// the accumulating form fits 128 VGPRs with 60 bytes per lane of spills and
-// runs 165 -> 144 us at 256^3 FP32 (152 VGPRs, 3 waves, without the request).
+// the register allocator for that many waves per SIMD (0: no request).
+// Without the next-term prefetch below, 4 fit the accumulating form in 128
+// VGPRs with 60 bytes per lane of spills: 165 -> 144 us at 256^3 FP32 (152
+// VGPRs, 3 waves, without the request); with the prefetch, 2.
 #ifndef DFT_XCONV1_EM
 #define DFT_XCONV1_EM 16
 #endif
@@ -417,7 +418,13 @@ __global__ void __launch_bounds__((MidCfg<T, N>::BS), mid_minb<T>()) k_xmid(cons
 #define DFT_XCONV_COLS 16
 #endif
 #ifndef DFT_XCONV_MINB
-#define DFT_XCONV_MINB 4
+#define DFT_XCONV_MINB 2
+#endif
+// FP32 accumulating outputs read the next term's input and kernel before the
+// current term's transform (+32 VGPRs: with it they run at 2 waves per SIMD
+// without spills, 146 -> 139 us at 256^3; at 3 they spill and take 161 us)
+#ifndef DFT_XCONV_PF
+#define DFT_XCONV_PF 1
 #endif
 // the same request for the FP64 instantiations (0: none)
 #ifndef DFT_XCONV_MINB64
@@ -495,18 +502,34 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DF
   C acc[E];
 #pragma unroll
   for (int i = 0; i < E; ++i) acc[i] = cmk<C>(0, 0);
+  // term q's input and kernel; with DFT_XCONV_PF (FP32) term q + 1's are read
+  // before term q's transform
+  constexpr bool PF = sizeof(T) == 4 && DFT_XCONV_PF;
+  C nv[PF ? E : 1];
+  T nk[PF ? E : 1];
+  auto fetch = [&](int qq, C *fv, T *fk) {
+    const MidTerm tq = mo.t[qq];
+#pragma unroll
+    for (int i = 0; i < E; ++i) fv[i] = rf::gld<C>((const char *)pg.in[tq.in] + i * stC, obC);
+    if (tq.kern >= 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) fk[i] = rf::gld<T>((const char *)pg.kern[tq.kern] + i * stR, obR);
+    } else {
+#pragma unroll
+      for (int i = 0; i < E; ++i) fk[i] = (T)1;
+    }
+  };
+  if constexpr (PF) fetch(0, nv, nk);
   for (int q = 0; q < mo.nterms; ++q) {
     const MidTerm tm = mo.t[q];
     C v[E];
     T k[E];
+    if constexpr (PF) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = rf::gld<C>((const char *)pg.in[tm.in] + i * stC, obC);
-    if (tm.kern >= 0) {
-#pragma unroll
-      for (int i = 0; i < E; ++i) k[i] = rf::gld<T>((const char *)pg.kern[tm.kern] + i * stR, obR);
+      for (int i = 0; i < E; ++i) v[i] = nv[i], k[i] = nk[i];
+      if (q + 1 < mo.nterms) fetch(q + 1, nv, nk);
     } else {
-#pragma unroll
-      for (int i = 0; i < E; ++i) k[i] = (T)1;
+      fetch(q, v, k);
     }
     if (q == 0) __syncthreads();  // twiddles
     rf::rfft<T, N, G::EM, -1, rf::BlockSync>(v, X, s_tw, t);
