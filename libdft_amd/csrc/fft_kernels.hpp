@@ -268,6 +268,9 @@ template <int BS> __device__ __forceinline__ void block_sum(double acc, double *
 #ifndef DFT_ZINV_PF
 #define DFT_ZINV_PF 1
 #endif
+#ifndef DFT_ZINV_PF64
+#define DFT_ZINV_PF64 1
+#endif
 // z pass over a full complex field, multiplying by e^{-iV dt/2} (V = pot,
 // complex) before (DIR<0) or after (DIR>0, together with `scale`) the
 // transform.  After the inverse pass each block writes sum |psi|^2 to
@@ -296,9 +299,9 @@ __global__ void __launch_bounds__((LineCfg<T, N>::BS))
       v[i] = apply_vphase<T>(v[i], lde<C>(pot, e0, TPL * i), dtr, dti);
     }
   }
-  // inverse pass: the potential is read before the transform (DFT_ZINV_PF, FP32),
-  // so that its loads overlap the field's
-  constexpr bool PF = DIR > 0 && sizeof(T) == 4 && DFT_ZINV_PF;
+  // inverse pass: the potential is read before the transform (DFT_ZINV_PF,
+  // DFT_ZINV_PF64), so that its loads overlap the field's
+  constexpr bool PF = DIR > 0 && (sizeof(T) == 4 ? DFT_ZINV_PF : DFT_ZINV_PF64);
   C pv[PF ? E : 1];
   if constexpr (PF) {
     if (pot)
