@@ -20,7 +20,7 @@ LIB_PATH = os.environ.get("LIBDFT_MI355X_SO") or os.path.join(_HERE, "libdft_mi3
 _LIB = None
 
 GRID_FP32, GRID_FP64 = 1, 2
-DFT_OT_PLAIN, DFT_OT_KC, DFT_OT_BACKFLOW = 0, 1, 8
+DFT_OT_PLAIN, DFT_OT_KC, DFT_OT_HD, DFT_OT_HD2, DFT_OT_BACKFLOW, DFT_OT_T0MK = 0, 1, 2, 4, 8, 16
 WF_PERIODIC_BOUNDARY, WF_2ND_ORDER_FFT = 1, 2
 DFT_EPS = 1e-4
 

@@ -102,11 +102,12 @@ template <typename T> __global__ void k_spec_real(const double2 *spec, T *out, i
 }
 
 // P1: psi -> rho and, for backflow, v_d (grid_wf_velocity), q_d = v_d rho,
-// q4 = |v|^2 rho (src/ot.c:385-396, 852-891).
+// q4 = |v|^2 rho (src/ot.c:385-396, 852-891); with the HD / HD2 backflow
+// switch rho~ = G rho (stored in rt) takes rho's place in q_d and q4.
 template <typename T, bool BF>
 __global__ void k_prep(const cplx<T> *__restrict__ psi, T *__restrict__ rho, T *__restrict__ vx, T *__restrict__ vy,
                        T *__restrict__ vz, T *__restrict__ qx, T *__restrict__ qy, T *__restrict__ qz,
-                       T *__restrict__ q4, PW<T> pw) {
+                       T *__restrict__ q4, T *__restrict__ rt, PW<T> pw) {
   const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
   GRID_STRIDE(c, n) {
     const cplx<T> s = psi[c];
@@ -126,10 +127,12 @@ __global__ void k_prep(const cplx<T> *__restrict__ psi, T *__restrict__ rho, T *
       vx[c] = v[0];
       vy[c] = v[1];
       vz[c] = v[2];
-      qx[c] = v[0] * r;
-      qy[c] = v[1] * r;
-      qz[c] = v[2] * r;
-      q4[c] = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * r;
+      const T w = pw.bfhd ? bf_rt(r, pw) : r;
+      if (pw.bfhd) rt[c] = w;
+      qx[c] = v[0] * w;
+      qy[c] = v[1] * w;
+      qz[c] = v[2] * w;
+      q4[c] = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * w;
     }
   }
 }
@@ -137,10 +140,12 @@ __global__ void k_prep(const cplx<T> *__restrict__ psi, T *__restrict__ rho, T *
 // P1 for given density and velocities (dft_ot_potential2, dft_ot_backflow_potential).
 template <typename T>
 __global__ void k_prep_v(const T *__restrict__ rho, T *vx, T *vy, T *vz, T *__restrict__ qx, T *__restrict__ qy,
-                         T *__restrict__ qz, T *__restrict__ q4, PW<T> pw) {
+                         T *__restrict__ qz, T *__restrict__ q4, T *__restrict__ rt, PW<T> pw) {
   const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
   GRID_STRIDE(c, n) {
-    const T r = rho[c];
+    const T r0 = rho[c];
+    const T r = pw.bfhd ? bf_rt(r0, pw) : r0;
+    if (pw.bfhd) rt[c] = r;
     T v[3] = {vx[c], vy[c], vz[c]};
     if (pw.clamp_v) {
 #pragma unroll
@@ -164,7 +169,9 @@ struct StageBOut {
 };
 
 // P2: src/ot.c:547-586 (local terms), 604-656 (G_d), 893-961 (BF real part,
-// W_d and the (d rho/(rho+eps)) W_d half of the imaginary part).
+// W_d and the (d rho/(rho+eps)) W_d half of the imaginary part; with the
+// HD / HD2 backflow switch the real part carries d(G rho)/d rho and the
+// gradient is that of G rho).
 template <typename T, bool LOC, bool KC, bool BF>
 __global__ void k_stageB(StageBIn in, StageBOut out, PW<T> pw) {
   const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
@@ -199,6 +206,7 @@ __global__ void k_stageB(StageBIn in, StageBOut out, PW<T> pw) {
       t += (T)-2 * vy * By;
       t += (T)-2 * vz * Bz;
       t = t + Cc;
+      if (pw.bfhd) t = t * bf_dg(r, pw);
       T re = t * ((T)-0.5 * pw.mass);
       if (pw.clamp_bf) re = clampv(re, pw.bfmax);
       vre += re * pw.cbf;
@@ -206,6 +214,11 @@ __global__ void k_stageB(StageBIn in, StageBOut out, PW<T> pw) {
       ((T *)out.Wx)[c] = Wx;
       ((T *)out.Wy)[c] = Wy;
       ((T *)out.Wz)[c] = Wz;
+      if (pw.bfhd) {
+        const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[d] = (bf_rt(rho[nb.p[d]], pw) - bf_rt(rho[nb.m[d]], pw)) * pw.inv2h;
+      }
       const T inv = (T)1 / (r + pw.div_eps);
       T im = (T)0.5 * (g[0] * inv) * Wx;
       im += (T)0.5 * (g[1] * inv) * Wy;
@@ -224,7 +237,8 @@ struct StageCOut {
 };
 
 // P3: src/ot.c:640-675 (KC terms 1 and 3, H_d = d_d rho J_d summed over d
-// because F*(sum_d H_d) = sum_d F*H_d), 963-990 (BF d_d W_d half of Im V).
+// because F*(sum_d H_d) = sum_d F*H_d), 963-990 (BF d_d W_d half of Im V,
+// times G(rho) with the HD / HD2 backflow switch).
 template <typename T, bool KC, bool BF>
 __global__ void k_stageC(StageCIn in, StageCOut out, PW<T> pw) {
   const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
@@ -248,8 +262,14 @@ __global__ void k_stageC(StageCIn in, StageCOut out, PW<T> pw) {
     if constexpr (BF) {
       const T *W[3] = {(const T *)in.Wx, (const T *)in.Wy, (const T *)in.Wz};
       T im = ((const T *)out.Vim)[c];
+      if (pw.bfhd) {
+        const T gw = bf_g(((const T *)in.rho)[c], pw);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) im += (T)0.5 * ((W[d][nb.p[d]] - W[d][nb.m[d]]) * pw.inv2h);
+        for (int d = 0; d < 3; ++d) im += (T)0.5 * (((W[d][nb.p[d]] - W[d][nb.m[d]]) * pw.inv2h) * gw);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) im += (T)0.5 * ((W[d][nb.p[d]] - W[d][nb.m[d]]) * pw.inv2h);
+      }
       if (pw.clamp_bf) im = clampv(im, pw.bfmax);
       ((T *)out.Vim)[c] = im * pw.cbf;
     }
@@ -297,14 +317,16 @@ __global__ void k_energyB(EnergyIn in, T *eps, T *Gx, T *Gy, T *Gz, PW<T> pw, in
       Gz[c] = (rho[nb.p[2]] - rho[nb.m[2]]) * pw.inv2h * rst;
     }
     if constexpr (BF) {
+      // G rho in place of rho with the HD / HD2 backflow switch (src/ot-energy.c:221-227)
+      const T rw = pw.bfhd ? bf_rt(r, pw) : r;
       const T vx = ((const T *)in.vx)[c], vy = ((const T *)in.vy)[c], vz = ((const T *)in.vz)[c];
       const T v2 = (vx * vx + vy * vy) + vz * vz;
       const T mq = pw.cbf * pw.mass * (T)0.25, mh = pw.cbf * pw.mass * (T)0.5;
-      e += -mq * (((const T *)in.A)[c] * v2 * r);
-      e += mh * (((const T *)in.Bx)[c] * r * vx);
-      e += mh * (((const T *)in.By)[c] * r * vy);
-      e += mh * (((const T *)in.Bz)[c] * r * vz);
-      e += -mq * (((const T *)in.C)[c] * r);
+      e += -mq * (((const T *)in.A)[c] * v2 * rw);
+      e += mh * (((const T *)in.Bx)[c] * rw * vx);
+      e += mh * (((const T *)in.By)[c] * rw * vy);
+      e += mh * (((const T *)in.Bz)[c] * rw * vz);
+      e += -mq * (((const T *)in.C)[c] * rw);
     }
     eps[c] = e;
   }
@@ -778,7 +800,7 @@ enum Kern { K_LJ = 0, K_SA = 1, K_F = 2, K_U = 3 };
 
 // Slot roles (each slot holds a real field or a real-field half spectrum).
 enum Slot {
-  SP0, SP1, SP2, SP3, SP4,                  // spectra
+  SP0, SP1, SP2, SP3, SP4, SP5,             // spectra (SP5: rho~ of the HD / HD2 backflow switch)
   VX, VY, VZ, QX, QY, QZ, Q4,               // velocities, products
   RB, RT, PX, PY, PZ, SA_, SBX, SBY, SBZ, SC,  // first inverse batch
   TMP0, TMP1, TMP2, TMP3, TMP4, TMP5, TMP6, TMP7, TMP8, TMP9,
@@ -856,6 +878,9 @@ template <typename T> PW<T> make_pw(dft_ot_functional *otf, double wf_mass, doub
   pw.hdC = (T)otf->C;
   pw.hdB = (T)otf->beta;
   pw.hdRm = (T)otf->rhom;
+  pw.bfXi = (T)otf->xi;
+  pw.bfRho = (T)otf->rhobf;
+  pw.bfhd = (otf->model & (DFT_OT_HD | DFT_OT_HD2)) && (otf->model & DFT_OT_BACKFLOW);
   return pw;
 }
 
@@ -895,14 +920,15 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
       vx = S<T>(d, VX);
       vy = S<T>(d, VY);
       vz = S<T>(d, VZ);
-      LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+      LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), S<T>(d, SA_), pw);
     } else {
-      LAUNCH_PW("ot_prep_density", 3, (k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
+      LAUNCH_PW("ot_prep_density", 3, (k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
     }
   } else if (BF) {
-    LAUNCH_PW("ot_prep_products", 8, (k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+    LAUNCH_PW("ot_prep_products", 8, (k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), S<T>(d, SA_), pw);
   }
-  // F1
+  // F1 (rho~ of the HD / HD2 backflow switch in SA_ -> SP5)
+  const bool HB = BF && pw.bfhd;
   {
     std::vector<const T *> in{rho};
     std::vector<cplx<T> *> out{CS<T>(d, SP0)};
@@ -910,11 +936,15 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
       for (int i : {QX, QY, QZ, Q4}) in.push_back(S<T>(d, i));
       for (int i : {SP1, SP2, SP3, SP4}) out.push_back(CS<T>(d, i));
     }
+    if (HB) {
+      in.push_back(S<T>(d, SA_));
+      out.push_back(CS<T>(d, SP5));
+    }
     r2c3d<T>(p, (int)in.size(), in.data(), out.data(), s);
   }
   // I1
   {
-    const cplx<T> *in[5] = {CS<T>(d, SP0), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3), CS<T>(d, SP4)};
+    const cplx<T> *in[6] = {CS<T>(d, SP0), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3), CS<T>(d, SP4), CS<T>(d, SP5)};
     std::vector<fft::Combine> c;
     std::vector<T *> out;
     if (LOC) {
@@ -930,7 +960,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
       }
     }
     if (BF) {
-      c.push_back(cmb1(0, K_U, -1, h3N));
+      c.push_back(cmb1(HB ? 5 : 0, K_U, -1, h3N));
       out.push_back(S<T>(d, SA_));
       for (int dd = 0; dd < 3; ++dd) {
         c.push_back(cmb1(1 + dd, K_U, -1, h3N));
@@ -941,7 +971,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     }
     std::vector<cplx<T> *> tmp;
     for (size_t i = 0; i < out.size(); ++i) tmp.push_back(CS<T>(d, TMP0 + (int)i));
-    if (!out.empty()) c2r3d<T>(p, 5, in, (int)out.size(), c.data(), kt, tmp.data(), out.data(), s);
+    if (!out.empty()) c2r3d<T>(p, HB ? 6 : 5, in, (int)out.size(), c.data(), kt, tmp.data(), out.data(), s);
   }
   // P2
   {
@@ -1050,7 +1080,8 @@ template <typename T, int N> void launch_zA(bool BF, const cplx<T> *psi, T *rho,
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   DFT_PROF_BEGIN(s);
-  if (BF) hipLaunchKernelGGL((fused::k_zA<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, psi, rho, o, npairs, P, pw, tw);
+  if (BF && pw.bfhd) hipLaunchKernelGGL((fused::k_zA_bfhd<T, N>), grid, dim3(fused::PairBS<T, N>), 0, s, psi, rho, o, npairs, P, pw, tw);
+  else if (BF) hipLaunchKernelGGL((fused::k_zA<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, psi, rho, o, npairs, P, pw, tw);
   else hipLaunchKernelGGL((fused::k_zA<T, N, false>), grid, dim3(fused::PairBS<T, N>), 0, s, psi, rho, o, npairs, P, pw, tw);
   DFT_LAUNCH_CHECK();
   DFT_PROF_END(s, "ot_zA_density_r2c", bytes);
@@ -1136,15 +1167,24 @@ void launch_xmid(const fused::MidProg *pg, const fused::MidProg &hp, int ny, int
   DFT_PROF_END(s, "ot_xmid_fused", bytes);
 }
 
-template <typename T, int N, bool KC, bool BF>
+// HB: the backflow density switch of HD / HD2 (k_zAB_bfhd)
+template <typename T, int N, bool KC, bool BF, bool HB = false>
 void launch_zAB1(const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw, hipStream_t s) {
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
   if constexpr (BF && (sizeof(T) == 8 ? DFT_ZAB_SPLIT64 : DFT_ZAB_SPLIT32)) {
     // the backflow terms, then the local and KC terms (V_re through memory)
-    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
-    DFT_LAUNCH_CHECK();
-    hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+    if constexpr (HB) {
+      hipLaunchKernelGGL((fused::k_zAB_bfhd<T, N, KC, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+      DFT_LAUNCH_CHECK();
+      hipLaunchKernelGGL((fused::k_zAB_bfhd<T, N, KC, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+    } else {
+      hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+      DFT_LAUNCH_CHECK();
+      hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+    }
+  } else if constexpr (HB) {
+    hipLaunchKernelGGL((fused::k_zAB_bfhd<T, N, KC>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   } else {
     hipLaunchKernelGGL((fused::k_zAB<T, N, KC, BF>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   }
@@ -1154,7 +1194,10 @@ template <typename T, int N>
 void launch_zAB(bool KC, bool BF, const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw,
                 const cplx<T> *tw, hipStream_t s, double bytes) {
   DFT_PROF_BEGIN(s);
-  if (KC && BF) launch_zAB1<T, N, true, true>(a, npairs, P, pw, tw, s);
+  if (BF && pw.bfhd) {
+    if (KC) launch_zAB1<T, N, true, true, true>(a, npairs, P, pw, tw, s);
+    else launch_zAB1<T, N, false, true, true>(a, npairs, P, pw, tw, s);
+  } else if (KC && BF) launch_zAB1<T, N, true, true>(a, npairs, P, pw, tw, s);
   else if (KC) launch_zAB1<T, N, true, false>(a, npairs, P, pw, tw, s);
   else if (BF) launch_zAB1<T, N, false, true>(a, npairs, P, pw, tw, s);
   else launch_zAB1<T, N, false, false>(a, npairs, P, pw, tw, s);
@@ -1171,13 +1214,18 @@ void launch_zBC(bool BF, const fused::ZBCArgs &a, int64_t npairs, int P, const P
   constexpr bool SPLIT = sizeof(T) == 8 && DFT_ZBC_SPLIT64;
   if constexpr (SPLIT) {
     // FP64: the J_d stages, then T1, the tail and the r2c of H (H and the term-3 sum through memory)
-    if (BF) {
+    if (BF && pw.bfhd) {
+      hipLaunchKernelGGL((fused::k_zBC_bfhd<T, N, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+      hipLaunchKernelGGL((fused::k_zBC_bfhd<T, N, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
+    } else if (BF) {
       hipLaunchKernelGGL((fused::k_zBC<T, N, true, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
       hipLaunchKernelGGL((fused::k_zBC<T, N, true, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
     } else {
       hipLaunchKernelGGL((fused::k_zBC<T, N, false, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
       hipLaunchKernelGGL((fused::k_zBC<T, N, false, 2>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
     }
+  } else if (BF && pw.bfhd) {
+    hipLaunchKernelGGL((fused::k_zBC_bfhd<T, N>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   } else if (BF) {
     hipLaunchKernelGGL((fused::k_zBC<T, N, true>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);
   } else {
@@ -1267,17 +1315,18 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     pg.sinz = kt.sinz;
   };
 
-  // ---- stage A: psi -> {rho, q_d, q4}, forward z and y
-  const int nA = BF ? 5 : 1;
+  // ---- stage A: psi -> {rho, q_d, q4, [rho~]}, forward z and y
+  const bool HB = BF && pw.bfhd;
+  const int nA = BF ? (HB ? 6 : 5) : 1;
   fused::Ptrs8 oA{};
   for (int i = 0; i < nA; ++i) oA.p[i] = d->slot[SP0 + i];
   DFT_DISPATCH_N(d->nz, (launch_zA<T, N_>(BF, psi, rho, oA, npairs, PP, pw, p.twz, s, 3 * R + nA * H)));
   {
-    cplx<T> *b[5];
+    cplx<T> *b[6];
     for (int i = 0; i < nA; ++i) b[i] = CS<T>(d, SP0 + i);
     ypass<T>(p, nA, b, -1, P, PP, s);
   }
-  // x middle: Pi*rho, [F*rho], [U*rho, U*q_x, U*q_y, U*q_z, U*q4]
+  // x middle: Pi*rho, [F*rho], [U*rho (U*rho~ with the switch), U*q_x, U*q_y, U*q_z, U*q4]
   int nB = 0;
   {
     fused::MidProg pg = new_prog();
@@ -1293,7 +1342,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     add(0, K_SA);
     if (KC) add(0, K_F);
     if (BF) {
-      add(0, K_U);
+      add(HB ? 5 : 0, K_U);
       for (int i = 1; i <= 4; ++i) add(i, K_U);
     }
     setk(pg);
@@ -1431,15 +1480,16 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
   if (!rho_given) {
     // density (and velocities) from psi
     if (BF)
-      LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+      LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), S<T>(d, SA_), pw);
     else
-      LAUNCH_PW("ot_prep_density", 3, (k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
+      LAUNCH_PW("ot_prep_density", 3, (k_prep<T, false>), psi, rho, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, pw);
   } else if (BF) {
     // velocities from psi (grid_wf_velocity, src/ot-energy.c:231), products with the given density
     LAUNCH_PW("ot_prep_density_velocity", 10, (k_prep<T, true>), psi, S<T>(d, VRE), vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4),
-              pw);
-    LAUNCH_PW("ot_prep_products", 8, (k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), pw);
+              S<T>(d, SA_), pw);
+    LAUNCH_PW("ot_prep_products", 8, (k_prep_v<T>), rho, vx, vy, vz, S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ), S<T>(d, Q4), S<T>(d, SA_), pw);
   }
+  const bool HB = BF && pw.bfhd;  // rho~ of the HD / HD2 backflow switch in SA_ -> SP5
   {
     std::vector<const T *> in{rho};
     std::vector<cplx<T> *> out{CS<T>(d, SP0)};
@@ -1447,10 +1497,14 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
       for (int i : {QX, QY, QZ, Q4}) in.push_back(S<T>(d, i));
       for (int i : {SP1, SP2, SP3, SP4}) out.push_back(CS<T>(d, i));
     }
+    if (HB) {
+      in.push_back(S<T>(d, SA_));
+      out.push_back(CS<T>(d, SP5));
+    }
     r2c3d<T>(p, (int)in.size(), in.data(), out.data(), s);
   }
   {
-    const cplx<T> *in[5] = {CS<T>(d, SP0), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3), CS<T>(d, SP4)};
+    const cplx<T> *in[6] = {CS<T>(d, SP0), CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3), CS<T>(d, SP4), CS<T>(d, SP5)};
     std::vector<fft::Combine> c;
     std::vector<T *> out;
     if (LOC) {
@@ -1464,7 +1518,7 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
       out.push_back(S<T>(d, RT));
     }
     if (BF) {
-      c.push_back(cmb1(0, K_U, -1, h3N));
+      c.push_back(cmb1(HB ? 5 : 0, K_U, -1, h3N));
       out.push_back(S<T>(d, SA_));
       for (int dd = 0; dd < 3; ++dd) {
         c.push_back(cmb1(1 + dd, K_U, -1, h3N));
@@ -1475,7 +1529,7 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
     }
     std::vector<cplx<T> *> tmp;
     for (size_t i = 0; i < out.size(); ++i) tmp.push_back(CS<T>(d, TMP0 + (int)i));
-    c2r3d<T>(p, 5, in, (int)out.size(), c.data(), kt, tmp.data(), out.data(), s);
+    c2r3d<T>(p, HB ? 6 : 5, in, (int)out.size(), c.data(), kt, tmp.data(), out.data(), s);
   }
   {
     EnergyIn ei{rho, S<T>(d, PX), S<T>(d, RB), S<T>(d, RT), vx, vy, vz, S<T>(d, SA_), S<T>(d, SBX), S<T>(d, SBY),
@@ -1661,8 +1715,17 @@ rgrid *spectral_handle(dft_ot_functional *otf, int w, const char *id) {
 // backflow: with HD the reference also switches the backflow to another
 // density dependence (src/ot.c:843-982), which is not built here.
 bool supported_model(INT model) {
-  if (model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2)) return false;
-  return !((model & (DFT_OT_HD | DFT_OT_HD2)) && (model & DFT_OT_BACKFLOW));
+  return !(model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2 | DFT_OT_T0MK));
+}
+
+// The transposed Jacobians of the adjoint entry points cover the backflow
+// without the HD / HD2 density switch.
+bool adjoint_model(dft_ot_functional *otf, const char *fn) {
+  if ((otf->model & (DFT_OT_HD | DFT_OT_HD2)) && (otf->model & DFT_OT_BACKFLOW)) {
+    fprintf(stderr, "libdft: %s: the backflow density switch of HD / HD2 has no adjoint here\n", fn);
+    return false;
+  }
+  return true;
 }
 
 // Argument checks of the entry points: false (with a libdft: message) when
@@ -2261,7 +2324,9 @@ template <typename T> bool potential_vjp_psi_impl(dft_ot_functional *otf, cplx<T
 // ================================================================== C ABI ==
 extern "C" {
 
-// src/ot.c:1002-1271: T = 0 parameter row (1026-1036) and common block (1234-1270).
+// src/ot.c:1002-1271: the T = 0 parameter row (1026-1036), the row of
+// DFT_OT_T0MK (1038-1048: the thermal parametrisation at 0 K, which adds no
+// ideal-gas term) and the common block (1234-1270).
 void dft_ot_init_params(dft_ot_functional *otf, INT model) {
   const double a3 = GRID_AUTOANG * GRID_AUTOANG * GRID_AUTOANG;
   otf->beta = otf->rhom = otf->C = 0.0;
@@ -2274,6 +2339,13 @@ void dft_ot_init_params(dft_ot_functional *otf, INT model) {
   otf->temp = 0.0;
   otf->rho0 = 0.0218360;
   otf->lj_params.h = 2.1903;
+  if (model & DFT_OT_T0MK) {
+    otf->b = -719.2435;
+    otf->c2 = -24258.88;
+    otf->c3 = 1865257.0;
+    otf->rho0 = 0.0218354;
+    otf->lj_params.h = 2.19035;
+  }
   otf->mu0 = 0.0;
   otf->xi = 1E4 / a3;
   otf->rhobf = 0.033 * a3;
@@ -2310,7 +2382,7 @@ void dft_ot_init_params(dft_ot_functional *otf, INT model) {
 // src/ot.c:66-258
 dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_substeps) {
   if (!supported_model(model)) {
-    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC, BACKFLOW, and HD or HD2 without BACKFLOW)\n",
+    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC, BACKFLOW, HD, HD2 and T0MK)\n",
             (long)model);
     return nullptr;
   }
@@ -2632,7 +2704,7 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
 
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_well")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_well") || !adjoint_model(otf, "dft_ot_adjoint_well")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || !energy || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_well: invalid arguments\n");
     return -1;
@@ -2652,7 +2724,7 @@ void dft_set_verbose(int level) { g_verbose = level; }
 
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity") || !adjoint_model(otf, "dft_ot_adjoint_cavity")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || rho0 <= 0.0 || !bubble_radius || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_cavity: invalid arguments\n");
     return -1;
@@ -2681,7 +2753,7 @@ int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, w
 
 int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_bar, wf *gwf) {
   if (!check_wf(otf, gwf, "dft_ot_potential_vjp_psi") || !check_c(otf, psi_bar, "dft_ot_potential_vjp_psi") ||
-      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi"))
+      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi") || !adjoint_model(otf, "dft_ot_potential_vjp_psi"))
     return -1;
   bool ok = false;
   DFT_BY_PREC(otf->device->prec, ok = potential_vjp_psi_impl<T>(otf, (cplx<T> *)psi_bar->dev,

@@ -20,6 +20,8 @@ template <typename T> struct PW {
   T vmax, bfmax;           // clamps
   int clamp_v, clamp_bf;
   T hdC, hdB, hdRm;        // high-density penalty C, beta, rho_m (hdC = 0: off)
+  T bfXi, bfRho;           // backflow density switch xi, rho_bf
+  int bfhd;                // switch on (HD or HD2 with backflow)
 };
 
 struct Nbr {
@@ -65,6 +67,30 @@ template <typename T> __device__ inline T hd_pot(T r, const PW<T> &pw) {
 template <typename T> __device__ inline T hd_dpot(T r, const PW<T> &pw) {
   const HdST<T> q = hd_st(r, pw);
   return (T)2 * pw.hdC * pw.hdB * q.s * q.t * ((T)1 - pw.hdB * r * (q.s - (T)1));
+}
+
+// Backflow density switch of HD / HD2 with backflow (src/ot.c:843-982,
+// src/ot-energy.c:221-227): the backflow takes rho~ = G(rho) rho wherever it
+// uses the density (A, B, C, the energy weight, the gradient half of
+// Im V_BF), multiplies Re V_BF by d(G rho)/d rho and the divergence half of
+// Im V_BF by G.  G is libgrid's (absent) grid_func1, restated from the
+// published form (Ancilotto et al., Int. Rev. Phys. Chem. 36, 621 (2017)):
+//   G = (1 - tanh(x)) / 2 = 1 / (1 + e^{2x}),  x = xi (rho - rho_bf),
+//   G' = -2 xi e / (1 + e)^2 with e = e^{-2|x|},
+// formed from e so that no exponential overflows; G = 1 exactly at liquid
+// densities (x ~ -100).
+template <typename T> __device__ inline T bf_e(T r, const PW<T> &pw) {
+  return exp((T)-2 * fabs(pw.bfXi * (r - pw.bfRho)));
+}
+template <typename T> __device__ inline T bf_g(T r, const PW<T> &pw) {
+  const T e = bf_e(r, pw);
+  return r > pw.bfRho ? e / ((T)1 + e) : (T)1 / ((T)1 + e);
+}
+template <typename T> __device__ inline T bf_rt(T r, const PW<T> &pw) { return bf_g(r, pw) * r; }
+// d(G rho)/d rho = G + rho G'
+template <typename T> __device__ inline T bf_dg(T r, const PW<T> &pw) {
+  const T e = bf_e(r, pw), q = (T)1 + e;
+  return (r > pw.bfRho ? e / q : (T)1 / q) - (T)2 * pw.bfXi * r * e / (q * q);
 }
 
 template <typename T> __device__ inline T clampv(T v, T lim) {

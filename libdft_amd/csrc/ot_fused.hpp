@@ -12,7 +12,8 @@
 // together).
 //
 //   zA    psi -> rho (stored) and, with backflow, q_d = rho v_d, q4 = rho |v|^2;
-//         z r2c of {rho, q_x, q_y, q_z, q4}
+//         z r2c of {rho, q_x, q_y, q_z, q4} (and of rho~ = G rho, with rho~ in
+//         place of rho in q_d and q4, under the HD / HD2 backflow switch)
 //   xmid  x forward FFT, k-space products with real kernel spectra and the
 //         central-difference symbol i sin(k h)/h, x inverse FFT (a small
 //         program: several outputs per input, one multi-term output)
@@ -115,6 +116,11 @@ template <typename T, int D, class Q>
 __device__ __forceinline__ T grad_d(const void *f, const Q &q, int w, int i, const PW<T> &pw) {
   return (q.template nb<T, D, 1>(f, w, i) - q.template nb<T, D, -1>(f, w, i)) * pw.inv2h;
 }
+// central difference of rho~ = G(rho) rho (backflow density switch)
+template <typename T, int D, class Q>
+__device__ __forceinline__ T grad_rt_d(const void *rho, const Q &q, int w, int i, const PW<T> &pw) {
+  return (bf_rt(q.template nb<T, D, 1>(rho, w, i), pw) - bf_rt(q.template nb<T, D, -1>(rho, w, i), pw)) * pw.inv2h;
+}
 
 // Pair kernels use the plan with the fewest points per thread, 4: a line is
 // one wavefront up to N = 256 and two at N = 512 (exchanges then take the
@@ -170,10 +176,11 @@ template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { retur
 #ifndef DFT_ZC_MINB
 #define DFT_ZC_MINB 0
 #endif
-template <typename T, int N, bool BF>
-__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB : 0)
-    k_zA(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
-         const cplx<T> *__restrict__ g_tw) {
+// HB: the backflow density switch of HD / HD2 (q_d = rho~ v_d, q4 = rho~ |v|^2
+// and a sixth output, the transform of rho~ = G rho)
+template <typename T, int N, bool BF, bool HB>
+__device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
+                                        const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   C v[E];
 #pragma unroll
@@ -191,10 +198,15 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB :
   if constexpr (BF) {
     // velocity factor per point, kept for the three axes
     T fv[2][E];
+    T rt[2][HB ? E : 1];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       fv[0][i] = velocity_factor(pw, v[i].x);
       fv[1][i] = velocity_factor(pw, v[i].y);
+      if constexpr (HB) {
+        rt[0][i] = bf_rt(v[i].x, pw);
+        rt[1][i] = bf_rt(v[i].y, pw);
+      }
     }
     r2c_store<T, N>(v, X, s_tw, t, outs.p[0], p, P, act);
     T v2[2][E];
@@ -209,7 +221,8 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB :
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
           const C s = qq.template at<C>(psi, w, i);
-          const T r = s.x * s.x + s.y * s.y;
+          T r = s.x * s.x + s.y * s.y;
+          if constexpr (HB) r = rt[w][i];
           const T vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i]);
           v2[w][i] += vd * vd;
           o[w] = vd * r;
@@ -223,12 +236,30 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB :
     qd(std::integral_constant<int, 2>{});
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      v[i] = cmk<C>(v2[0][i] * q.template at<T>(rho, 0, i), v2[1][i] * q.template at<T>(rho, 1, i));
+      if constexpr (HB) v[i] = cmk<C>(v2[0][i] * rt[0][i], v2[1][i] * rt[1][i]);
+      else v[i] = cmk<C>(v2[0][i] * q.template at<T>(rho, 0, i), v2[1][i] * q.template at<T>(rho, 1, i));
     }
     r2c_store<T, N>(v, X, s_tw, t, outs.p[4], p, P, act);
+    if constexpr (HB) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) v[i] = cmk<C>(rt[0][i], rt[1][i]);
+      r2c_store<T, N>(v, X, s_tw, t, outs.p[5], p, P, act);
+    }
   } else {
     r2c_store<T, N>(v, X, s_tw, t, outs.p[0], p, P, act);
   }
+}
+template <typename T, int N, bool BF>
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB : 0)
+    k_zA(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
+         const cplx<T> *__restrict__ g_tw) {
+  zA_body<T, N, BF, false>(psi, rho, outs, npairs, P, pw, g_tw);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__((PairBS<T, N>), 0)
+    k_zA_bfhd(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
+              const cplx<T> *__restrict__ g_tw) {
+  zA_body<T, N, true, true>(psi, rho, outs, npairs, P, pw, g_tw);
 }
 
 // ------------------------------------------------------------------ xmid --
@@ -632,9 +663,11 @@ struct ZABArgs {
 // PART 0: the whole stage; with backflow it may run as two kernels with fewer
 // live fields, PART 1 (the backflow terms, V_re stored) then PART 2 (the local
 // and KC terms, added to the stored V_re).
-template <typename T, int N, bool KC, bool BF, int PART = 0>
-__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB : (PART ? DFT_ZAB_SPLIT_MINB64 : DFT_ZAB_MINB64))
-    k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+// HB: the backflow density switch of HD / HD2 (A, B_d, C come from rho~; Re
+// V_BF is multiplied by d(G rho)/d rho, the gradient half of Im V_BF takes rho~).
+template <typename T, int N, bool KC, bool BF, int PART, bool HB>
+__device__ __forceinline__ void zAB_body(ZABArgs a, int64_t npairs, int P, PW<T> pw,
+                                         const cplx<T> *__restrict__ g_tw) {
   static_assert(PART == 0 || BF, "only the backflow stage splits");
   FUSED_PAIR_SETUP();
   constexpr int PF = prefetch_bits<T, DFT_ZAB_PF>();
@@ -669,7 +702,7 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB 
     }
     const void *psi = a.psi;
     // velocity factor and 1 / (rho + eps) per point, kept for the three axes
-    T fv[2][E], fg[2][E];
+    T fv[2][E], fg[2][E], dg[2][HB ? E : 1];
     {
       const auto qr = q.here();
 #pragma unroll
@@ -679,6 +712,7 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB 
           const T r = qr.template at<T>(rho, w, i);
           fv[w][i] = velocity_factor(pw, r);
           fg[w][i] = (T)1 / (r + pw.div_eps);
+          if constexpr (HB) dg[w][i] = bf_dg(r, pw);
         }
     }
     auto axis = [&](auto dtag) {
@@ -691,7 +725,8 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB 
       // velocity and (D_d rho)/(rho + eps) at point (w, i)
       auto vg = [&](int w, int i, T &vd, T &gi) {
         vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i]);
-        gi = grad_d<T, D>(rho, qq, w, i, pw) * fg[w][i];
+        if constexpr (HB) gi = grad_rt_d<T, D>(rho, qq, w, i, pw) * fg[w][i];
+        else gi = grad_d<T, D>(rho, qq, w, i, pw) * fg[w][i];
       };
       T vds[2][E], gis[2][E];
       if constexpr ((PF & 1) != 0)
@@ -738,7 +773,9 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB 
     for (int i = 0; i < E; ++i) {
 #pragma unroll
       for (int w = 0; w < 2; ++w) {
-        T re = (tr[w][i] + part(v[i], w)) * ((T)-0.5 * pw.mass);
+        T re = tr[w][i] + part(v[i], w);
+        if constexpr (HB) re = re * dg[w][i];
+        re = re * ((T)-0.5 * pw.mass);
         if (pw.clamp_bf) re = clampv(re, pw.bfmax);
         vre[w][i] = re * pw.cbf;
       }
@@ -825,6 +862,16 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB 
     }
   }
 }
+template <typename T, int N, bool KC, bool BF, int PART = 0>
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB : (PART ? DFT_ZAB_SPLIT_MINB64 : DFT_ZAB_MINB64))
+    k_zAB(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+  zAB_body<T, N, KC, BF, PART, false>(a, npairs, P, pw, g_tw);
+}
+template <typename T, int N, bool KC, int PART = 0>
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZAB_MINB : (PART ? DFT_ZAB_SPLIT_MINB64 : DFT_ZAB_MINB64))
+    k_zAB_bfhd(ZABArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+  zAB_body<T, N, KC, true, PART, true>(a, npairs, P, pw, g_tw);
+}
 
 // ------------------------------------------------------------------- zBC --
 struct ZBCArgs {
@@ -841,9 +888,11 @@ struct ZBCArgs {
 // PART 0: the whole stage; PART 1 (the three J_d stages, H and the term-3
 // sum stored) then PART 2 (T1, the tail and the r2c of H): the split form FP64
 // runs with fewer live fields.
-template <typename T, int N, bool BF, int PART = 0>
-__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB : (PART ? DFT_ZBC_SPLIT_MINB64 : DFT_ZBC_MINB64))
-    k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+// HB: the backflow density switch of HD / HD2 (the divergence half of Im V_BF
+// is multiplied by G(rho)).
+template <typename T, int N, bool BF, int PART, bool HB>
+__device__ __forceinline__ void zBC_body(ZBCArgs a, int64_t npairs, int P, PW<T> pw,
+                                         const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
   constexpr int PF = prefetch_bits<T, DFT_ZBC_PF>();
   const void *rho = a.rho, *rhot = a.rhot;
@@ -920,9 +969,16 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB 
   auto bf_im = [&](int w, int i) {
     if constexpr (BF) {
       T im = qt.template at_once<T>(a.vim, w, i);
-      im += (T)0.5 * ((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h);
-      im += (T)0.5 * ((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h);
-      im += (T)0.5 * ((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h);
+      if constexpr (HB) {
+        const T gw = bf_g(qt.template at<T>(rho, w, i), pw);
+        im += (T)0.5 * (((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h) * gw);
+        im += (T)0.5 * (((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h) * gw);
+        im += (T)0.5 * (((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h) * gw);
+      } else {
+        im += (T)0.5 * ((qt.template nb<T, 0, 1>(a.wx, w, i) - qt.template nb<T, 0, -1>(a.wx, w, i)) * pw.inv2h);
+        im += (T)0.5 * ((qt.template nb<T, 1, 1>(a.wy, w, i) - qt.template nb<T, 1, -1>(a.wy, w, i)) * pw.inv2h);
+        im += (T)0.5 * ((qt.template nb<T, 2, 1>(a.wz, w, i) - qt.template nb<T, 2, -1>(a.wz, w, i)) * pw.inv2h);
+      }
       if (pw.clamp_bf) im = clampv(im, pw.bfmax);
       if (act) qt.template put<T>(a.vim, w, i, im * pw.cbf);
     }
@@ -965,6 +1021,16 @@ __global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB 
     for (int i = 0; i < E; ++i) v[i] = cmk<C>(H[0][i], H[1][i]);
   }
   r2c_store<T, N>(v, X, s_tw, t, a.out.p[0], p, P, act);
+}
+template <typename T, int N, bool BF, int PART = 0>
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB : (PART ? DFT_ZBC_SPLIT_MINB64 : DFT_ZBC_MINB64))
+    k_zBC(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+  zBC_body<T, N, BF, PART, false>(a, npairs, P, pw, g_tw);
+}
+template <typename T, int N, int PART = 0>
+__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZBC_MINB : (PART ? DFT_ZBC_SPLIT_MINB64 : DFT_ZBC_MINB64))
+    k_zBC_bfhd(ZBCArgs a, int64_t npairs, int P, PW<T> pw, const cplx<T> *__restrict__ g_tw) {
+  zBC_body<T, N, true, PART, true>(a, npairs, P, pw, g_tw);
 }
 
 // -------------------------------------------------------------------- zC --

@@ -40,7 +40,10 @@
    5 Im V_BF: (d rho/(rho+eps)) W term negated 6 Im V_BF: 1/2 d W -> d W
    7 W = v A - B -> v A + B                     8 KC term 1: rho_st -> 1
    9 KC term 1 negated                         10 KC term 2 negated
-  11 KC term 3 dropped */
+  11 KC term 3 dropped                        12 HD switch: Re V_BF without d(G rho)/d rho
+  13 HD switch: gradient half of Im V_BF takes rho, not G rho
+  14 HD switch: divergence half of Im V_BF without G
+  15 HD switch: eps_BF with rho, not G rho */
 static int g_plant = 0;
 void orc_set_plant(int which) { g_plant = which; }
 
@@ -237,11 +240,18 @@ static void add_im(double complex *pot, const double *f, long n) {
 
 /* ------------------------------------------------------ parameters (a1) -- */
 
-/* src/ot.c:1002-1271 for model < DFT_OT_T0MK (T = 0 row 1026-1036) and the common block. */
+/* src/ot.c:1002-1271: the T = 0 row (1026-1036) for model < DFT_OT_T0MK, the
+   row of DFT_OT_T0MK (1038-1048; the thermal parametrisation at 0 K, with no
+   ideal-gas term) and the common block. */
 static void init_params(orc_ot *o, int64_t model) {
   double a3 = ORC_AUTOANG * ORC_AUTOANG * ORC_AUTOANG;
   o->b = -718.99; o->c2 = -2.411857e4; o->c2_exp = 2; o->c3 = 1.858496e6; o->c3_exp = 3;
   o->c4 = 0.0; o->temp = 0.0; o->rho0 = 0.0218360; o->lj_h = 2.1903;
+  if (model & ORC_OT_T0MK) {
+    o->b = -719.2435; o->c2 = -24258.88; o->c2_exp = 2; o->c3 = 1865257.0; o->c3_exp = 3;
+    o->c4 = 0.0; o->temp = 0.0; o->rho0 = 0.0218354; o->lj_h = 2.19035;
+  }
+  o->bf_xi = 1e4 / a3; o->bf_rhobf = 0.033 * a3;
   if (model & (ORC_OT_HD | ORC_OT_HD2)) {       /* src/ot.c:1004-1007 */
     o->hd_beta = 40.0 / a3; o->hd_rhom = 0.37 * a3; o->hd_C = 0.1;
   }
@@ -399,6 +409,30 @@ static inline double hd_pot(const orc_ot *o, double r) {
 static inline double hd_eps(const orc_ot *o, double r) { return o->hd_C * r * hd_s(o, r); }
 static inline int has_hd(const orc_ot *o) { return (o->model & (ORC_OT_HD | ORC_OT_HD2)) != 0; }
 
+/* Backflow density switch of HD / HD2 with backflow (src/ot.c:843-982,
+   src/ot-energy.c:221-227).  The reference passes rho through libgrid's
+   grid_func2 (G(rho) rho) wherever the backflow uses the density: in A, B, C,
+   the energy weight and the gradient half of Im V_BF; it multiplies Re V_BF
+   by grid_func3 (d(G rho)/d rho) and the divergence half of Im V_BF by
+   grid_func1 (G).  libgrid is absent, so G is restated from the published
+   form (Ancilotto et al., Int. Rev. Phys. Chem. 36, 621 (2017)):
+   G = (1 - tanh(xi (rho - rho_bf))) / 2 = 1 / (1 + e^{2x}), x = xi (rho - rho_bf),
+   formed from e = e^{-2|x|}; G = 1 to below 1e-40 at liquid densities. */
+static inline int has_bfhd(const orc_ot *o) { return has_hd(o) && (o->model & ORC_OT_BACKFLOW); }
+static inline double bf_g(const orc_ot *o, double r) {
+  double x = o->bf_xi * (r - o->bf_rhobf), e = exp(-2.0 * fabs(x));
+  return x > 0.0 ? e / (1.0 + e) : 1.0 / (1.0 + e);
+}
+static inline double bf_dg(const orc_ot *o, double r) {   /* d(G rho)/d rho = G + rho G' */
+  double x = o->bf_xi * (r - o->bf_rhobf), e = exp(-2.0 * fabs(x));
+  return bf_g(o, r) - 2.0 * o->bf_xi * r * e / ((1.0 + e) * (1.0 + e));
+}
+/* the density the backflow uses: rho, or G(rho) rho with the switch */
+static void bf_density(const orc_ot *o, const double *rho, double *out, long n) {
+  #pragma omp parallel for
+  for (long i = 0; i < n; i++) out[i] = has_bfhd(o) ? bf_g(o, rho[i]) * rho[i] : rho[i];
+}
+
 /* src/ot.c:524-532 and 547-586: LJ + local correlation.  rho_tf is FFT(rho). */
 static void add_lj_local(orc_ot *o, const geom *g, double complex *pot, const double *rho,
                          const double complex *rho_tf, double complex *cw, double *w1, double *w2) {
@@ -472,22 +506,26 @@ static void add_kc(orc_ot *o, const geom *g, double complex *pot, const double *
   }
 }
 
-/* src/ot.c:840-991 (original BF, no HD): adds Re and Im backflow potential.
-   vx, vy, vz are overwritten with v_d A - B_d as in the reference. */
+/* src/ot.c:840-991: adds Re and Im backflow potential, with the density
+   switch of HD / HD2 (rt = G rho; rt = rho without it).  vx, vy, vz are
+   overwritten with v_d A - B_d as in the reference. */
 static void add_bf(orc_ot *o, const geom *g, double complex *pot, const double *rho,
                    double *vx, double *vy, double *vz, double complex *cw,
                    double *A, double *C, double *Bx, double *By, double *Bz, double *w6) {
   long n = g->n;
-  r_fft(g, rho, cw);
+  const int hb = has_bfhd(o);
+  double *rt = rbuf(n);
+  bf_density(o, rho, rt, n);
+  r_fft(g, rt, cw);
   r_convolute(g, cw, cw, o->bf_tf);
   r_inv_norm2(g, cw, A);
   #pragma omp parallel for
-  for (long i = 0; i < n; i++) C[i] = (vz[i] * vz[i] + vy[i] * vy[i] + vx[i] * vx[i]) * rho[i];
+  for (long i = 0; i < n; i++) C[i] = (vz[i] * vz[i] + vy[i] * vy[i] + vx[i] * vx[i]) * rt[i];
   r_fft(g, C, cw); r_convolute(g, cw, cw, o->bf_tf); r_inv_norm2(g, cw, C);
   double *B[3] = {Bx, By, Bz}, *v[3] = {vx, vy, vz};
   for (int d = 0; d < 3; d++) {
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) B[d][i] = v[d][i] * rho[i];
+    for (long i = 0; i < n; i++) B[d][i] = v[d][i] * rt[i];
     r_fft(g, B[d], cw); r_convolute(g, cw, cw, o->bf_tf); r_inv_norm2(g, cw, B[d]);
   }
   /* real part */
@@ -500,6 +538,7 @@ static void add_bf(orc_ot *o, const geom *g, double complex *pot, const double *
     t += mb * vy[i] * By[i];
     t += mb * vz[i] * Bz[i];
     if (g_plant != 2) t = t + C[i];
+    if (hb && g_plant != 12) t = t * bf_dg(o, rho[i]);
     w6[i] = t * (g_plant == 4 ? -o->mass : -0.5 * o->mass);
   }
   if (o->max_bfpot < 10000.0 / ORC_AUTOK) r_clamp(w6, n, o->max_bfpot, -o->max_bfpot, o->max_bfpot, -o->max_bfpot);
@@ -512,18 +551,22 @@ static void add_bf(orc_ot *o, const geom *g, double complex *pot, const double *
     for (long i = 0; i < n; i++) v[d][i] = v[d][i] * A[i] + (g_plant == 7 ? B[d][i] : -B[d][i]);
   }
   for (int d = 0; d < 3; d++) {
-    grad_dir(rho, C, g->nx, g->ny, g->nz, g->step, d);
+    grad_dir(g_plant == 13 ? rho : rt, C, g->nx, g->ny, g->nz, g->step, d);
     #pragma omp parallel for
     for (long i = 0; i < n; i++) w6[i] += (g_plant == 5 ? -0.5 : 0.5) * (C[i] / (rho[i] + o->div_epsilon)) * v[d][i];
   }
   for (int d = 0; d < 3; d++) {
     grad_dir(v[d], C, g->nx, g->ny, g->nz, g->step, d);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w6[i] += (g_plant == 6 ? 1.0 : 0.5) * C[i];
+    for (long i = 0; i < n; i++) {
+      double dw = hb && g_plant != 14 ? C[i] * bf_g(o, rho[i]) : C[i];
+      w6[i] += (g_plant == 6 ? 1.0 : 0.5) * dw;
+    }
   }
   if (o->max_bfpot < 10000.0 / ORC_AUTOK) r_clamp(w6, n, o->max_bfpot, -o->max_bfpot, o->max_bfpot, -o->max_bfpot);
   if (o->c_bfpot != 1.0) for (long i = 0; i < n; i++) w6[i] *= o->c_bfpot;
   add_im(pot, w6, n);
+  free(rt);
 }
 
 /* src/ot.c:431-510 (dft_ot_potential2) with the src/ot.c:314-408 body shared. */
@@ -618,34 +661,37 @@ static void energy_body(orc_ot *o, double *eps, const double *rho, double *const
   }
 
   if (o->model & ORC_OT_BACKFLOW) {
-    double *const *v = vel, *v2 = rbuf(n);
+    /* rho is G rho throughout with the HD / HD2 switch (src/ot-energy.c:221-227) */
+    double *const *v = vel, *v2 = rbuf(n), *rt = rbuf(n);
+    bf_density(o, rho, rt, n);
+    if (g_plant == 15) memcpy(rt, rho, sizeof(double) * (size_t) n);
     #pragma omp parallel for
     for (long i = 0; i < n; i++) v2[i] = (v[0][i] * v[0][i] + v[1][i] * v[1][i]) + v[2][i] * v[2][i];
     double mq = o->c_bfpot * o->mass / (g_plant == 3 || g_plant == 4 ? 2.0 : 4.0), mh = o->c_bfpot * o->mass / 2.0;
     if (g_plant == 1) mh = -mh;
     if (g_plant == 4) mh *= 2.0;
-    r_fft(&g, rho, cw1);
+    r_fft(&g, rt, cw1);
     r_convolute(&g, cw1, o->bf_tf, cw1);
     r_inv_norm2(&g, cw1, w1);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) eps[i] += -mq * (w1[i] * v2[i] * rho[i]);
+    for (long i = 0; i < n; i++) eps[i] += -mq * (w1[i] * v2[i] * rt[i]);
     for (int d = 0; d < 3; d++) {
       #pragma omp parallel for
-      for (long i = 0; i < n; i++) w2[i] = rho[i] * v[d][i];
+      for (long i = 0; i < n; i++) w2[i] = rt[i] * v[d][i];
       r_fft(&g, w2, cw1);
       r_convolute(&g, cw1, o->bf_tf, cw1);
       r_inv_norm2(&g, cw1, w1);
       #pragma omp parallel for
-      for (long i = 0; i < n; i++) eps[i] += mh * (w1[i] * rho[i] * v[d][i]);
+      for (long i = 0; i < n; i++) eps[i] += mh * (w1[i] * rt[i] * v[d][i]);
     }
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) w2[i] = rho[i] * v2[i];
+    for (long i = 0; i < n; i++) w2[i] = rt[i] * v2[i];
     r_fft(&g, w2, cw1);
     r_convolute(&g, cw1, o->bf_tf, cw1);
     r_inv_norm2(&g, cw1, w1);
     #pragma omp parallel for
-    for (long i = 0; i < n; i++) eps[i] += g_plant == 2 ? 0.0 : -mq * (w1[i] * rho[i]);
-    free(v2);
+    for (long i = 0; i < n; i++) eps[i] += g_plant == 2 ? 0.0 : -mq * (w1[i] * rt[i]);
+    free(v2); free(rt);
   }
   free(w1); free(w2); free(cw1); free(cw2);
 }

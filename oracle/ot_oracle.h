@@ -36,6 +36,7 @@
 #define ORC_OT_HD       2
 #define ORC_OT_HD2      4
 #define ORC_OT_BACKFLOW 8
+#define ORC_OT_T0MK     16
 
 /* Unit conversions: CODATA 2018 (libgrid's au.h, which the reference uses, is not available). */
 #define ORC_AUTOANG 0.529177210903
@@ -64,6 +65,8 @@ typedef struct {
   /* high-density penalty (src/ot.c:1004-1011; zero without HD/HD2) and the
      spherical-average radius (h, or 1.065 h for HD2, src/ot.c:164-170) */
   double hd_beta, hd_rhom, hd_C, sa_h;
+  /* backflow density switch of HD / HD2 with backflow (src/ot.c:1235-1236) */
+  double bf_xi, bf_rhobf;
 } orc_ot;
 
 /* Parameters (a1) and kernel spectra (a2). min_sub/max_sub as in dft_ot_alloc. */

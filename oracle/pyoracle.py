@@ -18,7 +18,7 @@ AUTOAMU = 1.0 / 1822.888486209
 AUTOMPS = 2187691.26364
 AUTOBAR = 2.9421015697e8
 HELIUM_MASS = 4.002602 / AUTOAMU
-PLAIN, KC, HD, HD2, BACKFLOW = 0, 1, 2, 4, 8
+PLAIN, KC, HD, HD2, BACKFLOW, T0MK = 0, 1, 2, 4, 8, 16
 
 
 class _OrcOT(C.Structure):
@@ -28,7 +28,7 @@ class _OrcOT(C.Structure):
                     "lj_h lj_sigma lj_epsilon lj_cval g11 g12 g21 g22 a1 a2 "
                     "div_epsilon max_veloc max_bfpot c_bfpot kc_coef23").split()] + \
               [(n, C.c_void_p) for n in "lj_tf sa_tf g_tf gx_tf gy_tf gz_tf bf_tf".split()] + \
-              [(n, C.c_double) for n in "hd_beta hd_rhom hd_C sa_h".split()]
+              [(n, C.c_double) for n in "hd_beta hd_rhom hd_C sa_h bf_xi bf_rhobf".split()]
 
 
 def lib():

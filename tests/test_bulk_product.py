@@ -51,6 +51,27 @@ def test_init_params_table():
     assert p.c_bfpot == 1.0 and abs(p.div_epsilon - 1e-4) < 1e-18
 
 
+def test_init_params_t0mk_row():
+    """DFT_OT_T0MK takes the row of src/ot.c:1038-1048 (the thermal parametrisation
+    at 0 K); the rest of the table is the common block.  The row is a fit with
+    zero pressure at its own rho0 and E/N = -7.16 K there, which pins its digits
+    against one another (P(rho0) = 0 to 1e-5 of the energy density there)."""
+    p = L.host_params(L.DFT_OT_T0MK | L.DFT_OT_KC | L.DFT_OT_BACKFLOW)
+    a3 = L.AUTOANG ** 3
+    assert abs(p.b * L.AUTOK * a3 - (-719.2435)) < 1e-9
+    assert abs(p.c2 * L.AUTOK * a3 ** 2 - (-24258.88)) < 1e-6
+    assert abs(p.c3 * L.AUTOK * a3 ** 3 - 1865257.0) < 1e-3
+    assert abs(p.rho0 / a3 - 0.0218354) < 1e-12 and abs(p.lj_h * L.AUTOANG - 2.19035) < 1e-12
+    assert p.c4 == 0.0 and p.temp == 0.0
+    q = L.host_params(L.DFT_OT_KC | L.DFT_OT_BACKFLOW)
+    for name in ("lj_sigma", "lj_epsilon", "rho_0s", "alpha_s", "l_g", "mass", "div_epsilon"):
+        assert getattr(p, name) == getattr(q, name), name
+    e = _bulk(p, "energy", p.rho0)  # energy per volume
+    assert abs(e / p.rho0 * L.AUTOK + 7.1627) < 1e-3
+    assert abs(_bulk(p, "pressure", p.rho0)) < 1e-5 * abs(e)
+    assert abs(_bulk(p, "density") / p.rho0 - 1) < 1e-5
+
+
 def test_equation_of_state_fixture_product():
     otf = L.host_params(L.DFT_OT_PLAIN)
     rows = _eos_rows()
@@ -65,7 +86,7 @@ def test_equation_of_state_fixture_product():
         assert abs(mu - mu_ref) < 1.2e-4 + 6e-6 * P, (P, mu, mu_ref)
 
 
-@pytest.mark.parametrize("model", [0, 1, 8, 9])
+@pytest.mark.parametrize("model", [0, 1, 8, 9, 16, 25])
 def test_product_bulk_matches_oracle(oracle, model):
     po = oracle
     otf = L.host_params(model)

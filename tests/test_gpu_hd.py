@@ -4,7 +4,8 @@ penalty's onset (0.4 A^-3 at the centre): potential, density and energy
 density, the fused split-operator step, the energy gradient, the transposed
 Jacobian of the potential (dot-product test against central differences) and
 the adjoint sweep (against central differences of the final energy).  HD
-together with BACKFLOW is rejected at allocation.
+together with BACKFLOW (the backflow density switch) is covered by
+test_gpu_hd_backflow.py.
 
 The FP32 fused real-time step lands near 1.4e-5 of the oracle here (bar
 2e-5): at 0.4 A^-3 the penalty is ~0.7 hartree and beta*rho ~ 16 amplifies
@@ -132,13 +133,6 @@ def test_hd_potential_vjp_dot_product(model):
     assert abs(adj / fd - 1) < 1e-7
     for x in (rb, vb, f, w):
         x.free()
-
-
-def test_hd_with_backflow_is_rejected():
-    w = L.Wave(SHAPE, STEP, L.GRID_FP64)
-    assert not L.lib().dft_ot_alloc(2 | 8, w.h, 2, 8)
-    assert not L.lib().dft_ot_alloc(4 | 9, w.h, 2, 8)
-    w.free()
 
 
 @pytest.mark.parametrize("model", [2, 3], ids=["hd", "kc-hd"])

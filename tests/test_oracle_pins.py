@@ -35,9 +35,19 @@ Kinetic correlation, src/ot.c:604-797:
                (c/rho_0s) F*(sum_d D_d rho J_d): it differs from central
                differences of the KC energy by the lattice product-rule error,
                O(h^2), which falls 4x when h halves at a fixed box.
+Backflow density switch of HD / HD2 (src/ot.c:843-982, src/ot-energy.c:221-227):
+rho~ = G(rho) rho replaces rho in A, B, C and the energy weight, Re V_BF
+carries d(rho~)/d rho and Im V_BF = (c_bf/2) sum_d [(D_d rho~ / (rho + eps)) W_d
++ G D_d W_d].  With the switch widened (xi = 300 A^3, rho_bf = 0.026 A^-3, so
+that G runs from ~1 to ~0.05 over the test densities) the same checks apply:
+the pair form with rho~, Re V_BF = dE/drho at fixed v (central differences of
+step 1e-4, no longer exact: E is not quadratic in rho), dE/dv_d = -m c_bf rho~ W_d
+and sum rho Im V_BF = 0.  They pin how G enters; G's own form is libgrid's
+(absent) and is restated from the published one.
 test_checks_catch_planted_mistakes plants one mistake at a time in the oracle
 (flipped -2 v.B, dropped C, wrong m/4, both scaled, each Im term changed, W's
-sign, KC term 1 changed) and shows that at least one check fails for each.
+sign, KC term 1 changed, each use of the switch dropped) and shows that at
+least one check fails for each.
 """
 import numpy as np
 import pytest
@@ -125,6 +135,80 @@ def check_norm_identity(po):
     o8.p.div_epsilon = 0.0
     rho, v = _rho_v(po, SHAPE, o8.step)
     vim = o8.potential2(rho, *v).imag
+    return abs((rho * vim).sum()) / (1e-12 * np.abs(rho * vim).sum())
+
+
+def _hd_models(po, shape=SHAPE, step_ang=STEP_ANG):
+    """HD|BACKFLOW and HD alone, with the backflow switch widened."""
+    out = []
+    for m in (10, 2):
+        o = po.OracleOT(m, shape, step_ang / po.AUTOANG, 2, 8)
+        o.p.bf_xi = 300.0 / po.AUTOANG ** 3
+        o.p.bf_rhobf = 0.026 * po.AUTOANG ** 3
+        out.append(o)
+    return out
+
+
+def _switch(o, rho):
+    """rho~ = G rho, G = (1 - tanh(xi (rho - rho_bf))) / 2."""
+    return 0.5 * (1.0 - np.tanh(o.p.bf_xi * (rho - o.p.bf_rhobf))) * rho
+
+
+def _rho_v_hd(po, shape, seed=5):
+    """Densities from 0.009 to 0.035 A^-3 (across the widened switch) and a smooth velocity."""
+    rho = 0.0218 * po.AUTOANG ** 3 * (1.0 + 0.6 * _smooth_random(shape, seed))
+    v = [1e-4 * _smooth_random(shape, seed + 1 + d) for d in range(3)]
+    return rho, v
+
+
+def check_hd_pair_form(po):
+    shape, step_ang = (8, 8, 8), 0.9
+    o10, o2 = _hd_models(po, shape, step_ang)
+    step = o10.step
+    rho, v = _rho_v_hd(po, shape, seed=11)
+    got = _ebf(po, o10, o2, rho, v)
+    U = np.roll(o10.kernel_map(3, 2, 8), tuple(-n // 2 for n in shape), axis=(0, 1, 2))
+    idx = np.indices(shape).reshape(3, -1)
+    diff = [(idx[a][:, None] - idx[a][None, :]) % shape[a] for a in range(3)]
+    Uij = U[diff[0], diff[1], diff[2]]
+    r, vv = _switch(o10, rho).ravel(), [x.ravel() for x in v]
+    dv2 = sum((vv[d][:, None] - vv[d][None, :]) ** 2 for d in range(3))
+    want = -(o10.p.mass / 4.0) * o10.p.c_bfpot * step ** 6 * np.sum(Uij * r[:, None] * r[None, :] * dv2)
+    return abs(got - want) / (1e-10 * abs(want))
+
+
+def check_hd_density_derivative(po):
+    o10, o2 = _hd_models(po)
+    step = o10.step
+    rho, v = _rho_v_hd(po, SHAPE)
+    eta = 0.1 * o10.p.rho0 * inputs.splitmix64_uniform(rho.size, 7).reshape(SHAPE)
+    t = 1e-4
+    fd = (_ebf(po, o10, o2, rho + t * eta, v) - _ebf(po, o10, o2, rho - t * eta, v)) / (2 * t)
+    vre = (o10.potential2(rho, *v) - o2.potential2(rho, *v)).real
+    an = (vre * eta).sum() * step ** 3
+    return abs(fd - an) / (1e-7 * abs(an))
+
+
+def check_hd_velocity_derivative(po):
+    o10, o2 = _hd_models(po)
+    step = o10.step
+    rho, v = _rho_v_hd(po, SHAPE)
+    zeta = [1e-4 * inputs.splitmix64_uniform(rho.size, 20 + d).reshape(SHAPE) for d in range(3)]
+    t = 1e-1
+    fd = (_ebf(po, o10, o2, rho, [v[d] + t * zeta[d] for d in range(3)])
+          - _ebf(po, o10, o2, rho, [v[d] - t * zeta[d] for d in range(3)])) / (2 * t)
+    W = [a.copy() for a in v]
+    po.lib().orc_ot_potential2(o10._h, po._p(np.zeros(SHAPE, np.complex128)), po._p(rho), *[po._p(a) for a in W])
+    rt = _switch(o10, rho)
+    an = -o10.p.mass * o10.p.c_bfpot * step ** 3 * sum((rt * W[d] * zeta[d]).sum() for d in range(3))
+    return abs(fd - an) / (1e-9 * abs(an))
+
+
+def check_hd_norm_identity(po):
+    o10, _ = _hd_models(po)
+    o10.p.div_epsilon = 0.0
+    rho, v = _rho_v_hd(po, SHAPE)
+    vim = o10.potential2(rho, *v).imag
     return abs((rho * vim).sum()) / (1e-12 * np.abs(rho * vim).sum())
 
 
@@ -221,7 +305,9 @@ def check_kc_response(po):
 CHECKS = {"pair_form": check_pair_form, "density_derivative": check_density_derivative,
           "velocity_derivative": check_velocity_derivative, "norm_identity": check_norm_identity,
           "imag_response": check_imag_response, "galilean": check_galilean, "kc_response": check_kc_response,
-          "kc_convergence": check_kc_convergence}
+          "kc_convergence": check_kc_convergence, "hd_pair_form": check_hd_pair_form,
+          "hd_density_derivative": check_hd_density_derivative,
+          "hd_velocity_derivative": check_hd_velocity_derivative, "hd_norm_identity": check_hd_norm_identity}
 
 
 @pytest.mark.parametrize("name", list(CHECKS))
@@ -232,7 +318,9 @@ def test_pin(oracle, name):
 
 PLANTS = {1: "-2v.B sign", 2: "C dropped", 3: "m/4 in eps only", 4: "Re V and eps both x2",
           5: "Im (d rho/rho) W sign", 6: "Im 1/2 dW -> dW", 7: "W = vA + B", 8: "KC term 1 without rho_st",
-          9: "KC term 1 sign", 10: "KC term 2 sign", 11: "KC term 3 dropped"}
+          9: "KC term 1 sign", 10: "KC term 2 sign", 11: "KC term 3 dropped",
+          12: "switch: Re V without d(G rho)/d rho", 13: "switch: Im gradient of rho",
+          14: "switch: Im divergence without G", 15: "switch: energy with rho"}
 
 
 @pytest.mark.parametrize("plant", list(PLANTS), ids=list(PLANTS.values()))
