@@ -200,6 +200,17 @@ void rgrid_abs_power(rgrid *dst, rgrid *src, REAL p);
 void rgrid_abs_rot(rgrid *rot, rgrid *fx, rgrid *fy, rgrid *fz);
 /* j_d = (hbar / M) Im(psi* d_d psi), central differences; any output may be NULL */
 void grid_wf_probability_flux(wf *gwf, rgrid *jx, rgrid *jy, rgrid *jz);
+/* one component of it (examples/added_mass/added_mass.c:192-196) */
+void grid_wf_probability_flux_x(wf *gwf, rgrid *flux_x);
+void grid_wf_probability_flux_y(wf *gwf, rgrid *flux_y);
+void grid_wf_probability_flux_z(wf *gwf, rgrid *flux_z);
+/* dst = src^p, dst = a b, dst = a / (b + eps), the largest and smallest value
+   (examples/soliton/soliton.c:96, examples/vortex-trace/trace.c:183-185) */
+void rgrid_power(rgrid *dst, rgrid *src, REAL p);
+void rgrid_product(rgrid *dst, rgrid *a, rgrid *b);
+void rgrid_division_eps(rgrid *dst, rgrid *a, rgrid *b, REAL eps);
+REAL rgrid_max(rgrid *grid);
+REAL rgrid_min(rgrid *grid);
 /* h^3 sum f(x, y, z) grid (f evaluated on the host) */
 REAL rgrid_weighted_integral(rgrid *grid, REAL (*func)(void *arg, REAL x, REAL y, REAL z), void *farg);
 

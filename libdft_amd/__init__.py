@@ -142,6 +142,10 @@ _SIGS = {
     "rgrid_abs_power": (None, ["rg", "rg", "d"]),
     "rgrid_abs_rot": (None, ["rg", "rg", "rg", "rg"]),
     "grid_wf_probability_flux": (None, ["wf", "rg", "rg", "rg"]),
+    "grid_wf_probability_flux_x": (None, ["wf", "rg"]), "grid_wf_probability_flux_y": (None, ["wf", "rg"]),
+    "grid_wf_probability_flux_z": (None, ["wf", "rg"]),
+    "rgrid_power": (None, ["rg", "rg", "d"]), "rgrid_product": (None, ["rg", "rg", "rg"]),
+    "rgrid_division_eps": (None, ["rg", "rg", "rg", "d"]), "rgrid_max": ("d", ["rg"]), "rgrid_min": ("d", ["rg"]),
     "dft_ot_adjoint_well": ("i", ["otf", "wf", "d", "d", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_adjoint_cavity": ("i", ["otf", "wf", "d", "d", "d", "d", "z", "i64", "i64", "p", "p"]),
     "dft_ot_dispersion": ("d", ["wf", "otf", "d", "p", "d", "b", "b"]),

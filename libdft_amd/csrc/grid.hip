@@ -298,6 +298,28 @@ __global__ void k_abs_rot(T *rot, const T *fx, const T *fy, const T *fz, int nx,
 }
 
 // dst = |src|^p (real)
+template <typename T> __global__ void k_rpower(T *dst, const T *src, T p, int64_t n) {
+  GRID_STRIDE(i, n) dst[i] = pow(src[i], p);
+}
+template <typename T> __global__ void k_rproduct(T *dst, const T *a, const T *b, int64_t n) {
+  GRID_STRIDE(i, n) dst[i] = a[i] * b[i];
+}
+template <typename T> __global__ void k_rdivision_eps(T *dst, const T *a, const T *b, T eps, int64_t n) {
+  GRID_STRIDE(i, n) dst[i] = a[i] / (b[i] + eps);
+}
+// per-block maximum of sgn * a (sgn = -1: minus the minimum)
+template <typename T> __global__ void k_red_max(const T *a, T sgn, int64_t n, double *partial) {
+  __shared__ double s[kBlock];
+  double m = -INFINITY;
+  GRID_STRIDE(i, n) m = fmax(m, (double)(sgn * a[i]));
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
 template <typename T> __global__ void k_rabs_power(T *dst, const T *src, T p, int64_t n) {
   GRID_STRIDE(c, n) dst[c] = pow(fabs(src[c]), p);
 }
@@ -942,6 +964,46 @@ void rgrid_abs_power(rgrid *dst, rgrid *src, REAL p) {
   DFT_BY_PREC(dst->prec, LAUNCH(k_rabs_power<T>, n, (T *)dst->dev, (const T *)src->dev, (T)p, n));
 }
 
+// dst = src^p (libgrid rgrid_power; examples/soliton/soliton.c:96)
+void rgrid_power(rgrid *dst, rgrid *src, REAL p) {
+  const int64_t n = npts(dst->nx, dst->ny, dst->nz);
+  if (!need_same("rgrid_power", n, npts(src->nx, src->ny, src->nz), dst->prec, src->prec)) return;
+  DFT_BY_PREC(dst->prec, LAUNCH(k_rpower<T>, n, (T *)dst->dev, (const T *)src->dev, (T)p, n));
+}
+
+// dst = a b (libgrid rgrid_product)
+void rgrid_product(rgrid *dst, rgrid *a, rgrid *b) {
+  const int64_t n = npts(dst->nx, dst->ny, dst->nz);
+  for (rgrid *f : {a, b})
+    if (!need_same("rgrid_product", n, npts(f->nx, f->ny, f->nz), dst->prec, f->prec)) return;
+  DFT_BY_PREC(dst->prec, LAUNCH(k_rproduct<T>, n, (T *)dst->dev, (const T *)a->dev, (const T *)b->dev, n));
+}
+
+// dst = a / (b + eps) (libgrid rgrid_division_eps; examples/vortex-trace/trace.c:183)
+void rgrid_division_eps(rgrid *dst, rgrid *a, rgrid *b, REAL eps) {
+  const int64_t n = npts(dst->nx, dst->ny, dst->nz);
+  for (rgrid *f : {a, b})
+    if (!need_same("rgrid_division_eps", n, npts(f->nx, f->ny, f->nz), dst->prec, f->prec)) return;
+  DFT_BY_PREC(dst->prec, LAUNCH(k_rdivision_eps<T>, n, (T *)dst->dev, (const T *)a->dev, (const T *)b->dev, (T)eps, n));
+}
+
+static double rgrid_extreme(rgrid *g, double sgn) {
+  const int64_t n = npts(g->nx, g->ny, g->nz), nb = red_blocks(n);
+  double *part = dft_scratch_partials(nb);
+  DFT_BY_PREC(g->prec, hipLaunchKernelGGL(k_red_max<T>, dim3((unsigned)nb), dim3(kBlock), 0, lib_stream(),
+                                          (const T *)g->dev, (T)sgn, n, part));
+  DFT_LAUNCH_CHECK();
+  std::vector<double> h(nb);
+  DFT_HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, lib_stream()));
+  DFT_HIP_CHECK(hipStreamSynchronize(lib_stream()));
+  double m = -INFINITY;
+  for (double x : h) m = std::fmax(m, x);
+  return sgn * m;
+}
+// largest and smallest value (libgrid rgrid_max / rgrid_min; examples/vortex-trace/trace.c:185)
+REAL rgrid_max(rgrid *g) { return rgrid_extreme(g, 1.0); }
+REAL rgrid_min(rgrid *g) { return rgrid_extreme(g, -1.0); }
+
 // Probability flux j = (hbar / M) Im(psi* grad psi), central differences
 // (examples/gas-bubble/analyze.c:78).  Any of jx, jy, jz may be NULL.
 void grid_wf_probability_flux(wf *w, rgrid *jx, rgrid *jy, rgrid *jz) {
@@ -953,6 +1015,11 @@ void grid_wf_probability_flux(wf *w, rgrid *jx, rgrid *jy, rgrid *jz) {
                               jy ? (T *)jy->dev : nullptr, jz ? (T *)jz->dev : nullptr, (int)g->nx, (int)g->ny,
                               (int)g->nz, (T)(0.5 / g->step), (T)(1.0 / w->mass)));
 }
+
+// one component of the flux (libgrid grid_wf_probability_flux_x/y/z)
+void grid_wf_probability_flux_x(wf *w, rgrid *j) { grid_wf_probability_flux(w, j, nullptr, nullptr); }
+void grid_wf_probability_flux_y(wf *w, rgrid *j) { grid_wf_probability_flux(w, nullptr, j, nullptr); }
+void grid_wf_probability_flux_z(wf *w, rgrid *j) { grid_wf_probability_flux(w, nullptr, nullptr, j); }
 
 // h^3 sum f(x, y, z) g over the grid, f evaluated on the host
 // (examples/gas-bubble/analyze.c:88).

@@ -29,6 +29,9 @@ DRIVERS = [
     ["piston/piston.c"],
     ["shock/shock.c"],
     ["added_mass_viscosity/analyze4.c"],
+    ["soliton/soliton.c"],
+    ["1d-soliton/soliton-1d.c"],
+    ["vortex-trace/trace.c"],
 ]
 
 

@@ -2,7 +2,8 @@
 value_at_index outside the grid (periodic boundary), the momentum frame of the
 propagator and the kinetic energy (cgrid_set_momentum), grid_wf_energy with a
 potential, and the driver helpers (Laplacian, conjugate products, |.|^p, curl,
-region and weighted integrals), each against numpy on the same data.
+power, product, division with eps, max and min, flux components, region and
+weighted integrals), each against numpy on the same data.
 """
 import ctypes as C
 
@@ -135,6 +136,27 @@ def test_driver_grid_calls(prec):
     assert relerr(r.get(), curl) < TOL[prec] * 10
     lib.rgrid_abs_power(r.h, fx.h, 2.5)
     assert relerr(r.get(), np.abs(a) ** 2.5) < TOL[prec] * 10
+    # power, product, division with eps, max / min, one flux component
+    pos = np.abs(a) + 0.1
+    fy.set(pos)
+    lib.rgrid_power(r.h, fy.h, 0.5)
+    assert relerr(r.get(), np.sqrt(pos)) < TOL[prec] * 10
+    lib.rgrid_product(r.h, fx.h, fz.h)
+    assert relerr(r.get(), a * p) < TOL[prec]
+    lib.rgrid_division_eps(r.h, fx.h, fy.h, 0.05)
+    assert relerr(r.get(), a / (pos + 0.05)) < TOL[prec] * 10
+    assert lib.rgrid_max(fx.h) == a.max() and lib.rgrid_min(fx.h) == a.min()
+    w = L.Wave(SHAPE, STEP, prec)
+    w.set(f)
+    jx = L.RField(SHAPE, STEP, prec)
+    lib.grid_wf_probability_flux(w.h, r.h, None, None)
+    lib.grid_wf_probability_flux_x(w.h, jx.h)
+    assert np.array_equal(jx.get(), r.get())
+    lib.grid_wf_probability_flux(w.h, None, None, r.h)
+    lib.grid_wf_probability_flux_z(w.h, jx.h)
+    assert np.array_equal(jx.get(), r.get())
+    w.free(), jx.free()
+    fy.set(b)
     # weighted integral with a host callback
     FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, C.c_double, C.c_double)
     cb = FN(lambda _, x, y, z: 1.0 + 0.1 * x - 0.2 * y * z)
