@@ -513,8 +513,29 @@ struct BfAdj {
   const void *uA, *uB[3], *uC;    // U*Abar, U*Bbar_d, U*Cbar (Cbar = tb)
 };
 
+// d^2(G rho)/d rho^2 = 2 G' + rho G'' of the backflow density switch, with
+// G' = -2 xi e/(1+e)^2 and G'' = -4 xi^2 e (1-e)/(1+e)^3 for x <= 0, its
+// negative for x > 0 (e = e^{-2|x|}).
+template <typename T> __device__ inline T bf_d2(T r, const PW<T> &pw) {
+  const T e = bf_e(r, pw), q = (T)1 + e;
+  const T g1 = (T)-2 * pw.bfXi * e / (q * q);
+  T g2 = (T)-4 * pw.bfXi * pw.bfXi * e * ((T)1 - e) / (q * q * q);
+  if (r > pw.bfRho) g2 = -g2;
+  return (T)2 * g1 + r * g2;
+}
+// G' alone
+template <typename T> __device__ inline T bf_g1(T r, const PW<T> &pw) {
+  const T e = bf_e(r, pw), q = (T)1 + e;
+  return (T)-2 * pw.bfXi * e / (q * q);
+}
+
 // Pass 1: tb and imb, each zero where the forward clamp (max_bfpot) was
 // active; the values before clamping are rebuilt from v, A, B_d, C and W_d.
+// With the HD / HD2 backflow switch (rho~ = G rho; A, B_d, C from rho~;
+// Re V_BF ~ t d(rho~)/d rho; Im V_BF's gradient half from rho~, its divergence
+// half times G) tb also carries d(rho~)/d rho, and rb starts from the terms
+// through the density dependence of those factors: Re_bar t d^2(rho~)/d rho^2
+// and Im_bar G' sum_d D_d W_d (rb = 0 without the switch).
 template <typename T>
 __global__ void k_bfadj_a(BfAdj a, const T *__restrict__ B0, const T *__restrict__ B1, const T *__restrict__ B2,
                           const T *__restrict__ Cf, PW<T> pw) {
@@ -525,31 +546,42 @@ __global__ void k_bfadj_a(BfAdj a, const T *__restrict__ B0, const T *__restrict
     const T A = ((const T *)a.A)[c];
     T tb = ((const T *)a.vrb)[c] * pw.cbf * ((T)-0.5 * pw.mass);
     T ib = ((const T *)a.vib)[c] * pw.cbf * (T)0.5;
-    if (pw.clamp_bf) {
+    T rb = 0;
+    if (pw.clamp_bf || pw.bfhd) {
       T t = (v[2] * v[2] + v[1] * v[1] + v[0] * v[0]) * A;
       t += (T)-2 * v[0] * B0[c];
       t += (T)-2 * v[1] * B1[c];
       t += (T)-2 * v[2] * B2[c];
       t = t + Cf[c];
-      if (fabs(t * ((T)-0.5 * pw.mass)) > pw.bfmax) tb = 0;
+      const T r = rho[c];
+      const T dg = pw.bfhd ? bf_dg(r, pw) : (T)1;
+      if (pw.clamp_bf && fabs(t * dg * ((T)-0.5 * pw.mass)) > pw.bfmax) tb = 0;
       const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
-      const T inv = (T)1 / (rho[c] + pw.div_eps);
-      T im = 0;
+      const T inv = (T)1 / (r + pw.div_eps);
+      T im = 0, div = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const T *W = (const T *)a.W[d];
-        im += (T)0.5 * ((rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h * inv) * W[c];
-        im += (T)0.5 * ((W[nb.p[d]] - W[nb.m[d]]) * pw.inv2h);
+        const T gr = pw.bfhd ? bf_rt(rho[nb.p[d]], pw) - bf_rt(rho[nb.m[d]], pw) : rho[nb.p[d]] - rho[nb.m[d]];
+        im += (T)0.5 * (gr * pw.inv2h * inv) * W[c];
+        div += (T)0.5 * ((W[nb.p[d]] - W[nb.m[d]]) * pw.inv2h);
       }
-      if (fabs(im) > pw.bfmax) ib = 0;
+      im += pw.bfhd ? bf_g(r, pw) * div : div;
+      if (pw.clamp_bf && fabs(im) > pw.bfmax) ib = 0;
+      if (pw.bfhd) {
+        rb = tb * t * bf_d2(r, pw) + ib * (T)2 * bf_g1(r, pw) * div;
+        tb = tb * dg;
+      }
     }
     ((T *)a.tb)[c] = tb;
     ((T *)a.imb)[c] = ib;
+    ((T *)a.rb)[c] = rb;
   }
 }
 
 // Pass 2 (stencils of imb and rho): Wbar_d = imb g_d/(rho+eps) - D_d imb, and
-// the adjoints of t and of the W_d products.
+// the adjoints of t and of the W_d products (g_d the central difference of
+// rho, or of rho~ with the switch, which also weights D_d imb by G).
 template <typename T> __global__ void k_bfadj_b(BfAdj a, PW<T> pw) {
   const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
   const T *rho = (const T *)a.rho, *imb = (const T *)a.imb;
@@ -557,12 +589,19 @@ template <typename T> __global__ void k_bfadj_b(BfAdj a, PW<T> pw) {
     const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
     const T r = rho[c], tb = ((const T *)a.tb)[c], ib = imb[c], A = ((const T *)a.A)[c];
     const T inv = (T)1 / (r + pw.div_eps);
-    T Ab = 0, rb = 0;
+    T Ab = 0, rb = ((const T *)a.rb)[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T v = ((const T *)a.v[d])[c], W = ((const T *)a.W[d])[c];
-      const T g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
-      const T Wb = ib * g * inv - (imb[nb.p[d]] - imb[nb.m[d]]) * pw.inv2h;
+      T g, Wb;
+      if (pw.bfhd) {
+        const T rp = rho[nb.p[d]], rm = rho[nb.m[d]];
+        g = (bf_rt(rp, pw) - bf_rt(rm, pw)) * pw.inv2h;
+        Wb = ib * g * inv - (bf_g(rp, pw) * imb[nb.p[d]] - bf_g(rm, pw) * imb[nb.m[d]]) * pw.inv2h;
+      } else {
+        g = (rho[nb.p[d]] - rho[nb.m[d]]) * pw.inv2h;
+        Wb = ib * g * inv - (imb[nb.p[d]] - imb[nb.m[d]]) * pw.inv2h;
+      }
       ((T *)a.vb[d])[c] = (T)2 * tb * W + Wb * A;
       ((T *)a.Bb[d])[c] = (T)-2 * tb * v - Wb;
       ((T *)a.gb[d])[c] = ib * W * inv;
@@ -575,12 +614,30 @@ template <typename T> __global__ void k_bfadj_b(BfAdj a, PW<T> pw) {
 }
 
 // Pass 3 (stencil of gbar), after the convolutions uA = U*Abar, uB_d = U*Bbar_d,
-// uC = U*tb: rho_bar and v_bar of the backflow part, v_bar masked by the velocity clamp.
+// uC = U*tb: rho_bar and v_bar of the backflow part, v_bar masked by the velocity
+// clamp.  With the switch the terms through A, B_d, C and the gradient of rho~
+// are the adjoint of rho~ and reach rho through d(rho~)/d rho.
 template <typename T> __global__ void k_bfadj_c(BfAdj a, PW<T> pw) {
   const int64_t n = (int64_t)pw.nx * pw.ny * pw.nz;
   GRID_STRIDE(c, n) {
     const Nbr nb = neighbours(c, pw.nx, pw.ny, pw.nz);
-    const T r = ((const T *)a.rho)[c], uC = ((const T *)a.uC)[c];
+    const T r0 = ((const T *)a.rho)[c], uC = ((const T *)a.uC)[c];
+    if (pw.bfhd) {
+      const T r = bf_rt(r0, pw);
+      T rt = ((const T *)a.uA)[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T v = ((const T *)a.v[d])[c], uB = ((const T *)a.uB[d])[c];
+        const T *gb = (const T *)a.gb[d];
+        rt += uC * v * v + uB * v - (gb[nb.p[d]] - gb[nb.m[d]]) * pw.inv2h;
+        T vb = ((const T *)a.vb[d])[c] + uB * r + (T)2 * uC * r * v;
+        if (pw.clamp_v && fabs(v) >= pw.vmax) vb = 0;
+        ((T *)a.vb[d])[c] = vb;
+      }
+      ((T *)a.rb)[c] = ((const T *)a.rb)[c] + bf_dg(r0, pw) * rt;
+      continue;
+    }
+    const T r = r0;
     T rb = ((const T *)a.rb)[c] + ((const T *)a.uA)[c];
     T v2 = 0;
 #pragma unroll
@@ -1718,16 +1775,6 @@ bool supported_model(INT model) {
   return !(model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2 | DFT_OT_T0MK));
 }
 
-// The transposed Jacobians of the adjoint entry points cover the backflow
-// without the HD / HD2 density switch.
-bool adjoint_model(dft_ot_functional *otf, const char *fn) {
-  if ((otf->model & (DFT_OT_HD | DFT_OT_HD2)) && (otf->model & DFT_OT_BACKFLOW)) {
-    fprintf(stderr, "libdft: %s: the backflow density switch of HD / HD2 has no adjoint here\n", fn);
-    return false;
-  }
-  return true;
-}
-
 // Argument checks of the entry points: false (with a libdft: message) when
 // a handle is missing or its shape or precision differs from the functional's.
 bool check_otf(dft_ot_functional *otf, const char *fn) {
@@ -1963,10 +2010,10 @@ void adj_potential(dft_ot_functional *otf, const cplx<T> *psi, double mass, cons
 // with respect to rho (at fixed v) and to v_d.
 template <typename T>
 __global__ void k_bfseed(T *rb, T *vb0, T *vb1, T *vb2, const cplx<T> *V, const T *rho, const T *W0, const T *W1,
-                         const T *W2, double h3, double mcb, int64_t n) {
+                         const T *W2, double h3, double mcb, int64_t n, PW<T> pw) {
   GRID_STRIDE(c, n) {
     rb[c] = (T)(h3 * (double)V[c].x);
-    const double f = -mcb * h3 * (double)rho[c];
+    const double f = -mcb * h3 * (double)(pw.bfhd ? bf_rt(rho[c], pw) : rho[c]);
     vb0[c] = (T)(f * (double)W0[c]);
     vb1[c] = (T)(f * (double)W1[c]);
     vb2[c] = (T)(f * (double)W2[c]);
@@ -1975,7 +2022,8 @@ __global__ void k_bfseed(T *rb, T *vb0, T *vb1, T *vb2, const cplx<T> *V, const 
 
 // psib += dE_BF/dpsi: the backflow energy of src/ot-energy.c:202-288 (velocity
 // with DFT_EPS, no clamps) depends on psi through rho and v:
-// psib += 2 h^3 Re V_BF psi + (dv/dpsi)^T vb, vb_d = -m c_bf h^3 rho W_d.
+// psib += 2 h^3 Re V_BF psi + (dv/dpsi)^T vb, vb_d = -m c_bf h^3 rho W_d
+// (rho~ = G rho in place of rho with the HD / HD2 backflow switch).
 template <typename T>
 void seed_bf(dft_ot_functional *otf, const cplx<T> *psi, double mass, cplx<T> *psib, AdjWork<T> &w) {
   dft_ot_device *d = otf->device;
@@ -2002,7 +2050,7 @@ void seed_bf(dft_ot_functional *otf, const cplx<T> *psi, double mass, cplx<T> *p
   otf->max_bfpot = keep[2];
   LAUNCH_PW("ot_adjoint_bf_seed", 8, (k_bfseed<T>), w.b[BRB], w.b[BVB], w.b[BVB + 1], w.b[BVB + 2],
             (const cplx<T> *)w.c[3], (const T *)w.r[ARHO], (const T *)S<T>(d, WX), (const T *)S<T>(d, WY),
-            (const T *)S<T>(d, WZ), h3, otf->mass * otf->c_bfpot, d->n);
+            (const T *)S<T>(d, WZ), h3, otf->mass * otf->c_bfpot, d->n, pw);
   LAUNCH_PW("ot_adjoint_bf_psi", 8, (k_bfadj_psi<T>), psib, psi, (const T *)w.b[BRB], (const T *)w.b[BVB],
             (const T *)w.b[BVB + 1], (const T *)w.b[BVB + 2], pw);
 }
@@ -2704,7 +2752,7 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
 
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_well") || !adjoint_model(otf, "dft_ot_adjoint_well")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_well")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || !energy || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_well: invalid arguments\n");
     return -1;
@@ -2724,7 +2772,7 @@ void dft_set_verbose(int level) { g_verbose = level; }
 
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity") || !adjoint_model(otf, "dft_ot_adjoint_cavity")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || rho0 <= 0.0 || !bubble_radius || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_cavity: invalid arguments\n");
     return -1;
@@ -2753,7 +2801,7 @@ int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, w
 
 int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_bar, wf *gwf) {
   if (!check_wf(otf, gwf, "dft_ot_potential_vjp_psi") || !check_c(otf, psi_bar, "dft_ot_potential_vjp_psi") ||
-      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi") || !adjoint_model(otf, "dft_ot_potential_vjp_psi"))
+      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi"))
     return -1;
   bool ok = false;
   DFT_BY_PREC(otf->device->prec, ok = potential_vjp_psi_impl<T>(otf, (cplx<T> *)psi_bar->dev,

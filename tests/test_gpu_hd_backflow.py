@@ -18,7 +18,10 @@ the points within ~1e-4 A^-3 of rho_bf, d(G rho)/d rho (up to xi rho / 2 ~ 165)
 moves by about twice that, so Re V there agrees with the oracle to ~2.4e-5 of
 its maximum; the bar is 5x the FP32 one (1e-4) for that case only.
 
-The adjoint entry points refuse the switch; the thermal models above 0 K
+The transposed Jacobian of the switched potential (dft_ot_potential_vjp_psi)
+is checked against central differences of dft_ot_potential (dot-product test)
+and the adjoint sweep through it against central differences of the final
+energy, both with the widened switch in FP64.  The thermal models above 0 K
 (which need libgrid's ideal-gas term) are refused at allocation.
 """
 import numpy as np
@@ -180,21 +183,84 @@ def test_t0mk_matches_oracle(oracle, prec, model):
     pot.free(), eps.free(), f.free(), w.free()
 
 
+def _switch_potential(model, psi, sw):
+    w = L.Wave(SHAPE, STEP, L.GRID_FP64)
+    w.set(psi)
+    f = L.Functional(model, w, 2, 8)
+    _set_switch(sw, f.p)
+    pot = L.CField(SHAPE, STEP, L.GRID_FP64)
+    f.potential(pot)
+    V = pot.get()
+    pot.free(), f.free(), w.free()
+    return V
+
+
+@pytest.mark.parametrize("model", [10, 11], ids=["hd-bf", "kc-hd-bf"])
+def test_switch_potential_vjp_psi_dot_product(model):
+    """<vbar, dV/dpsi . dpsi> (Re and Im of V) by central differences of
+    dft_ot_potential equals Re <psi_bar, dpsi> from dft_ot_potential_vjp_psi."""
+    sw = SWITCH["widened"]
+    rho0 = 0.0218 * L.AUTOANG ** 3
+    psi = _state(rho0)
+    dpsi = inputs.smooth_modulated(SHAPE, STEP, rho0, 0.5, seed=78) * np.exp(1j * 0.7) - np.sqrt(rho0)
+    dpsi *= 1e-3 * np.sqrt(rho0) / np.abs(dpsi).max()
+    u = inputs.splitmix64_uniform(2 * psi.size, 12).reshape((2,) + SHAPE)
+    vbar = u[0] + 1j * u[1]
+    eps = 1e-2
+    dV = _switch_potential(model, psi + eps * dpsi, sw) - _switch_potential(model, psi - eps * dpsi, sw)
+    fd = np.sum(vbar.real * dV.real + vbar.imag * dV.imag) / (2 * eps)
+    w = L.Wave(SHAPE, STEP, L.GRID_FP64)
+    w.set(psi)
+    f = L.Functional(model, w, 2, 8)
+    _set_switch(sw, f.p)
+    vb, pb = L.CField(SHAPE, STEP, L.GRID_FP64), L.CField(SHAPE, STEP, L.GRID_FP64)
+    vb.set(vbar)
+    assert L.lib().dft_ot_potential_vjp_psi(f.h, pb.h, vb.h, w.h) == 0
+    g = pb.get()
+    adj = np.sum(g.real * dpsi.real + g.imag * dpsi.imag)
+    _log("switch psi-vjp model %d: FD %.12e adjoint %.12e rel %.2e" % (model, fd, adj, abs(adj / fd - 1)))
+    assert abs(adj / fd - 1) < 1e-7
+    for x in (pb, vb, f, w):
+        x.free()
+
+
+def _switch_well(model, psi0, amp, width, dt, nsteps, every):
+    w = L.Wave(SHAPE, STEP, L.GRID_FP64)
+    w.set(psi0)
+    f = L.Functional(model, w, 2, 8)
+    _set_switch(SWITCH["widened"], f.p)
+    E, g = f.adjoint_well(amp, width, (0.0, 0.0, 0.0), 0.0, dt, nsteps, every)
+    f.free(), w.free()
+    return E, g
+
+
+def test_switch_adjoint_well_matches_fd():
+    """The checkpointed adjoint sweep through the switch (KC|HD|BF, 6 real-time
+    steps of 0.5 fs, checkpoints every 4): dE/d(well depth, width) against
+    central differences."""
+    model, dt, K, every = 11, 0.5 / L.AUTOFS, 6, 4
+    psi0 = _state(0.0218 * L.AUTOANG ** 3)
+    amp, width = -50.0 / L.AUTOK, 2.0 / L.AUTOANG
+    E, g = _switch_well(model, psi0, amp, width, dt, K, every)
+    da, dw = 1e-4 * abs(amp), 1e-4 * width
+    fa = (_switch_well(model, psi0, amp + da, width, dt, K, every)[0]
+          - _switch_well(model, psi0, amp - da, width, dt, K, every)[0]) / (2 * da)
+    fw = (_switch_well(model, psi0, amp, width + dw, dt, K, every)[0]
+          - _switch_well(model, psi0, amp, width - dw, dt, K, every)[0]) / (2 * dw)
+    _log("switch well: E %.10e; dE/damp %.10e FD %.10e; dE/dwidth %.10e FD %.10e" % (E, g[0], fa, g[1], fw))
+    assert abs(g[0] / fa - 1) < 1e-6
+    assert abs(g[1] / fw - 1) < 1e-6
+
+
 def test_models_accepted_and_refused():
     """HD / HD2 with BACKFLOW and T0MK are accepted; the thermal models above 0 K
-    are refused at allocation; the adjoint entry points refuse the switch."""
+    are refused at allocation."""
     w = L.Wave(SHAPE, STEP, L.GRID_FP64)
     w.set(_state(0.0218 * L.AUTOANG ** 3))
     lib = L.lib()
     for m in (32, 64 | 9, 1 << 18):
         assert not lib.dft_ot_alloc(m, w.h, 2, 8), m
-    f = L.Functional(2 | 9, w, 2, 8)
-    with pytest.raises(RuntimeError):
-        f.adjoint_well(-50.0 / L.AUTOK, 2.0 / L.AUTOANG, (0.0, 0.0, 0.0), 0.0, 0.5 / L.AUTOFS, 2, 1)
-    with pytest.raises(RuntimeError):
-        f.adjoint_cavity(0.0218 * L.AUTOANG ** 3, 5.0 / L.AUTOANG, 0.5 / L.AUTOANG, 0.0, 0.5 / L.AUTOFS, 2, 1)
-    pb, vb = L.CField(SHAPE, STEP, L.GRID_FP64), L.CField(SHAPE, STEP, L.GRID_FP64)
-    assert lib.dft_ot_potential_vjp_psi(f.h, pb.h, vb.h, w.h) == -1
-    f16 = L.Functional(16 | 4 | 8, w, 2, 8)
-    for x in (pb, vb, f16, f, w):
-        x.free()
+    for m in (2 | 9, 16 | 4 | 8, 16 | 1):
+        f = L.Functional(m, w, 2, 8)
+        f.free()
+    w.free()
