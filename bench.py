@@ -265,14 +265,19 @@ def measured_traffic(label, n):
 
 # step traffic in units of one real field R (4N B in FP32, 8N B in FP64):
 # 134 R for the full functional, 34 R for plain OT (algorithmic schedule counts)
-STEP_R = {0: 34, 9: 134}
+# algorithmic bytes per step in real-field units R: plain OT, full OT, and full OT with
+# the HD / HD2 backflow switch (the penalty is pointwise; the switch transforms one more
+# field, rho~ = G rho: its z-pass spectrum write, the y pass over it and the x-middle read)
+STEP_R = {0: 34, 9: 134, 11: 138}
 
 
 def size_sweep(steps=10, warmup=3):
     """Steps/s and the whole-step HBM fraction at each workload size: plain OT at
     64^3 (FP64, h = 0.5 A, noisy bulk), full OT at 128^3 (droplet, h = 0.4 A) in
-    FP32 and FP64, and full OT at 512^3 (bulk with a 17 A cavity, h = 0.4 A) in
-    FP32; real-time single steps (dt = 2 fs) except imaginary time at 64^3."""
+    FP32 and FP64, full OT at 512^3 (bulk with a 17 A cavity, h = 0.4 A) in
+    FP32, and KC|HD|BACKFLOW (the high-density penalty and its backflow switch) on
+    the 256^3 vortex workload in FP32; real-time single steps (dt = 2 fs) except
+    imaginary time at 64^3."""
     rho0 = 0.0218360 * L.AUTOANG ** 3
     cases = [
         ("plain OT 64^3 imaginary time", 64, 0.5, L.GRID_FP64, 0,
@@ -281,6 +286,8 @@ def size_sweep(steps=10, warmup=3):
         ("full OT 128^3 droplet", 128, 0.4, L.GRID_FP64, 9, lambda n, h: inputs.droplet((n, n, n), h), 2.0 / L.AUTOFS),
         ("full OT 512^3 cavity", 512, 0.4, L.GRID_FP32, 9, lambda n, h: inputs.cavity((n, n, n), h, rho0),
          2.0 / L.AUTOFS),
+        ("KC|HD|BACKFLOW 256^3 vortex", 256, 0.3, L.GRID_FP32, 11,
+         lambda n, h: inputs.vortex_line((n, n, n), h, rho0), 2.0 / L.AUTOFS),
     ]
     out = []
     # config 1's pattern: imaginary-time steps renormalised every step, fused and as the

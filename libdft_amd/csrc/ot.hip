@@ -1229,7 +1229,7 @@ template <typename T, int N, bool KC, bool BF, bool HB = false>
 void launch_zAB1(const fused::ZABArgs &a, int64_t npairs, int P, const PW<T> &pw, const cplx<T> *tw, hipStream_t s) {
   constexpr int LN = rf::LineCfg<T, N, fused::pair_em<N>()>::LINES;
   const dim3 grid((unsigned)((npairs + LN - 1) / LN));
-  if constexpr (BF && (sizeof(T) == 8 ? DFT_ZAB_SPLIT64 : DFT_ZAB_SPLIT32)) {
+  if constexpr (BF && (sizeof(T) == 8 ? DFT_ZAB_SPLIT64 : (HB ? DFT_ZAB_SPLIT32_HB : DFT_ZAB_SPLIT32))) {
     // the backflow terms, then the local and KC terms (V_re through memory)
     if constexpr (HB) {
       hipLaunchKernelGGL((fused::k_zAB_bfhd<T, N, KC, 1>), grid, dim3(fused::PairBS<T, N>), 0, s, a, npairs, P, pw, tw);

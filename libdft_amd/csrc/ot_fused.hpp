@@ -621,6 +621,11 @@ __global__ void __launch_bounds__((XConvCfg<T, N, ONE>::BS), sizeof(T) == 4 ? DF
 #ifndef DFT_ZAB_SPLIT32
 #define DFT_ZAB_SPLIT32 0
 #endif
+// the same for the FP32 kernels with the HD / HD2 backflow switch (one wave per
+// SIMD in one kernel)
+#ifndef DFT_ZAB_SPLIT32_HB
+#define DFT_ZAB_SPLIT32_HB 1
+#endif
 // Early loads in zAB / zBC, so that memory round trips overlap the transforms
 // (bit masks, FP32; FP64 holds twice the registers and spills at its
 // occupancy, so it uses them only with DFT_PREFETCH64=1):
@@ -670,7 +675,8 @@ __device__ __forceinline__ void zAB_body(ZABArgs a, int64_t npairs, int P, PW<T>
                                          const cplx<T> *__restrict__ g_tw) {
   static_assert(PART == 0 || BF, "only the backflow stage splits");
   FUSED_PAIR_SETUP();
-  constexpr int PF = prefetch_bits<T, DFT_ZAB_PF>();
+  // (with the switch the stencils' exponentials would be held across the transform)
+  constexpr int PF = prefetch_bits<T, HB ? (DFT_ZAB_PF & ~1) : DFT_ZAB_PF>();
   constexpr int IA = KC ? 2 : 1;  // first backflow input
   constexpr bool DO_BF = BF && PART != 2, DO_LOCAL = PART != 1;
   const void *rho = a.rho;
