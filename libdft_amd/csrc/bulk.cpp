@@ -1,22 +1,42 @@
-// Host FP64 parts of the API: bulk closed forms (src/helium-ot-bulk.c) and the
-// kernel functions (src/common.c, src/ot-private.h).  T = 0 OT models only
-// (c4 = 0), so the ideal-gas terms of the thermal models do not appear.
+// Host FP64 parts of the API: bulk closed forms (src/helium-ot-bulk.c), with the
+// ideal Bose gas term of the thermal models (c4 != 0), and the kernel functions
+// (src/common.c, src/ot-private.h).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
 #include "../../include/libdft_mi355x.h"
+#include "ideal_gas.hpp"
+
+namespace {
+dft::ig::Params ig_params(const dft_ot_functional *otf) {
+  static const struct Tab {
+    double t[3 * dft::ig::kTerms];
+    Tab() { dft::ig::fill_table(t); }
+  } tab;
+  dft::ig::Params p;
+  p.tab = tab.t;
+  p.ckt = otf->c4 * (otf->temp / GRID_AUTOK);
+  p.l3 = dft::ig::lambda3(otf->mass, otf->temp, GRID_AUTOK);
+  p.g1 = dft::ig::series(1.0, tab.t + dft::ig::kTerms);
+  return p;
+}
+}  // namespace
 
 extern "C" {
 
 // src/helium-ot-bulk.c:21-33
 REAL dft_ot_bulk_energy(dft_ot_functional *otf, REAL rho) {
-  return 0.5 * otf->b * rho * rho + 0.5 * otf->c2 * rho * rho * rho + (1.0 / 3.0) * otf->c3 * rho * rho * rho * rho;
+  double e = 0.5 * otf->b * rho * rho + 0.5 * otf->c2 * rho * rho * rho + (1.0 / 3.0) * otf->c3 * rho * rho * rho * rho;
+  if (otf->c4 != 0.0) e += dft::ig::energy(rho, ig_params(otf));
+  return e;
 }
 
 // src/helium-ot-bulk.c:45-57
 REAL dft_ot_bulk_dEdRho(dft_ot_functional *otf, REAL rho) {
-  return otf->b * rho + 1.5 * otf->c2 * rho * rho + (4.0 / 3.0) * otf->c3 * rho * rho * rho;
+  double d = otf->b * rho + 1.5 * otf->c2 * rho * rho + (4.0 / 3.0) * otf->c3 * rho * rho * rho;
+  if (otf->c4 != 0.0) d += dft::ig::potential(rho, ig_params(otf));
+  return d;
 }
 
 // src/helium-ot-bulk.c:143-146

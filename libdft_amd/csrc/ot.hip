@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "ideal_gas.hpp"
 #include "internal.hpp"
 #include "ot_common.hpp"
 #include "ot_fused.hpp"
@@ -287,6 +288,16 @@ __global__ void k_final(cplx<T> *V, const T *__restrict__ vre, const T *__restri
     v.x += add;
     if (vim) v.y += vim[c];
     V[c] = v;
+  }
+}
+
+// Ideal Bose gas of the thermal models (src/ot.c:402-407, src/ot-energy.c:101-104):
+// out += V_ig(rho) (potential, or energy per volume with EPS), in FP64.
+template <typename T, bool EPS>
+__global__ void k_ideal_gas(T *out, const T *__restrict__ rho, ig::Params p, int64_t n) {
+  GRID_STRIDE(c, n) {
+    const double r = (double)rho[c];
+    out[c] += (T)(EPS ? ig::energy(r, p) : ig::potential(r, p));
   }
 }
 
@@ -877,6 +888,8 @@ struct dft_ot_device {
   void *phase;               // complex: V_eff of the current step, whose e^{-iV dt/2} both z passes apply
   void *vsave;               // complex: predictor potential V_n of a predict-correct step
   double *norm_part;         // per-block sums |psi|^2 and the factor of a renormalising step
+  double *igtab;             // pow(k, s) table of the ideal Bose gas (thermal models), or null
+  double ig_g1;              // and its g_{3/2}(1)
   // x-middle programs of the fused schedule in device memory, and the host
   // copies they were uploaded from (re-uploaded only when one changes)
   fused::MidProg *mid_dev;
@@ -885,6 +898,18 @@ struct dft_ot_device {
 };
 
 namespace {
+
+// The thermal models above 0 K add the ideal Bose gas term (src/ot.c:402,
+// src/ot-energy.c:102: model >= DFT_OT_T400MK, GP / DR / ZERO excluded).
+bool has_ig(const dft_ot_functional *otf) { return otf->model >= 32 && otf->model < 524288; }
+ig::Params ig_params(const dft_ot_functional *otf) {
+  ig::Params p;
+  p.tab = otf->device->igtab;
+  p.ckt = otf->c4 * (otf->temp / GRID_AUTOK);
+  p.l3 = ig::lambda3(otf->mass, otf->temp, GRID_AUTOK);
+  p.g1 = otf->device->ig_g1;
+  return p;
+}
 
 template <typename T> Plan3D<T> &plan_of(dft_ot_device *d) { return *(Plan3D<T> *)d->plan; }
 template <typename T> T *S(dft_ot_device *d, int i) { return (T *)d->slot[i]; }
@@ -1099,6 +1124,12 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
     T *out[1] = {S<T>(d, TMP1)};
     cplx<T> *tmp[1] = {CS<T>(d, TMP0)};
     c2r3d<T>(p, 3, in, 1, &c, kt, tmp, out, s);
+  }
+  // ideal Bose gas of the thermal models, with the whole potential (dft_ot_potential,
+  // dft_ot_potential2: src/ot.c:402-407, 505-510)
+  if (LOC && parts.lj < 0 && has_ig(otf)) {
+    const ig::Params igp = ig_params(otf);
+    LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, false>), S<T>(d, VRE), (const T *)rho, igp, d->n);
   }
   // P4
   const bool PW_ = LOC || KC || BF;  // pointwise stages ran (V_re in VRE)
@@ -1475,6 +1506,11 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     StageCOut co{nullptr, S<T>(d, VRE), S<T>(d, VIM)};
     LAUNCH_PW("ot_stage_kc_terms_bf_div", 5, (k_stageC<T, false, true>), ci, co, pw);
   }
+  // ---- ideal Bose gas of the thermal models: pointwise, into V_re
+  if (has_ig(otf)) {
+    const ig::Params igp = ig_params(otf);
+    LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, false>), (T *)d->slot[VRE], (const T *)rho, igp, n);
+  }
   // ---- stage C: linear terms summed in k space, z^-1, accumulate or start the step
   {
     fused::MidProg pg = new_prog();
@@ -1602,6 +1638,10 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
     else if (KC) EB(false, true, false);
     else if (BF) EB(false, false, true);
 #undef EB
+    if (LOC && has_ig(otf)) {  // src/ot-energy.c:101-104
+      const ig::Params igp = ig_params(otf);
+      LAUNCH_PW("ot_ideal_gas_energy", 3, (k_ideal_gas<T, true>), eps, (const T *)rho, igp, d->n);
+    }
     if (KC) {
       const T *in3[3] = {S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ)};
       cplx<T> *sp[3] = {CS<T>(d, SP1), CS<T>(d, SP2), CS<T>(d, SP3)};
@@ -1740,6 +1780,7 @@ void free_device(dft_ot_device *d) {
   dfree(d->phase);
   dfree(d->vsave);
   dfree(d->norm_part);
+  dfree(d->igtab);
   dfree(d->mid_dev);
   if (d->plan) {
     if (d->prec == GRID_FP32) {
@@ -1772,7 +1813,18 @@ rgrid *spectral_handle(dft_ot_functional *otf, int w, const char *id) {
 // backflow: with HD the reference also switches the backflow to another
 // density dependence (src/ot.c:843-982), which is not built here.
 bool supported_model(INT model) {
-  return !(model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2 | DFT_OT_T0MK));
+  // PLAIN, KC, HD, HD2, BACKFLOW and the parameter rows T0MK ... T3000MK (bits 16 ... 262144)
+  return !(model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2 | ((INT)524288 - 16)));
+}
+
+// The entry points whose derivative does not cover the ideal Bose gas term
+// (its fugacity is a bracketed search, not a differentiable function of rho).
+bool no_ideal_gas(dft_ot_functional *otf, const char *fn) {
+  if (has_ig(otf)) {
+    fprintf(stderr, "libdft: %s: not available for the thermal models above 0 K\n", fn);
+    return false;
+  }
+  return true;
 }
 
 // Argument checks of the entry points: false (with a libdft: message) when
@@ -2372,9 +2424,10 @@ template <typename T> bool potential_vjp_psi_impl(dft_ot_functional *otf, cplx<T
 // ================================================================== C ABI ==
 extern "C" {
 
-// src/ot.c:1002-1271: the T = 0 parameter row (1026-1036), the row of
+// src/ot.c:1002-1271: the T = 0 parameter row (1026-1036), the rows of
 // DFT_OT_T0MK (1038-1048: the thermal parametrisation at 0 K, which adds no
-// ideal-gas term) and the common block (1234-1270).
+// ideal-gas term) and of T400MK ... T3000MK (1050-1216), and the common block
+// (1234-1270).
 void dft_ot_init_params(dft_ot_functional *otf, INT model) {
   const double a3 = GRID_AUTOANG * GRID_AUTOANG * GRID_AUTOANG;
   otf->beta = otf->rhom = otf->C = 0.0;
@@ -2387,13 +2440,37 @@ void dft_ot_init_params(dft_ot_functional *otf, INT model) {
   otf->temp = 0.0;
   otf->rho0 = 0.0218360;
   otf->lj_params.h = 2.1903;
-  if (model & DFT_OT_T0MK) {
-    otf->b = -719.2435;
-    otf->c2 = -24258.88;
-    otf->c3 = 1865257.0;
-    otf->rho0 = 0.0218354;
-    otf->lj_params.h = 2.19035;
-  }
+  // rows applied in the reference's order, each whose bit is set (the last one wins)
+  static const struct {
+    INT bit;
+    double b, c2, c3, c4, temp, rho0, h;
+  } rows[] = {
+      {DFT_OT_T0MK, -719.2435, -24258.88, 1865257.0, 0.0, 0.0, 0.0218354, 2.19035},  // src/ot.c:1038-1048
+      {32, -714.2174, -24566.29, 1873203.0, 0.98004, 0.4, 0.0218351, 2.18982},      // 1050-1060
+      {64, -705.1319, -25124.17, 1887707.0, 0.99915, 0.6, 0.0218346, 2.18887},
+      {128, -690.4745, -26027.12, 1911283.0, 0.99548, 0.8, 0.0218331, 2.18735},
+      {256, -646.5135, -28582.81, 1973737.0, 0.99666, 1.2, 0.0218298, 2.18287},
+      {512, -625.8123, -29434.03, 1984068.0, 0.99829, 1.4, 0.0218332, 2.18080},
+      {1024, -605.9788, -30025.96, 1980898.0, 1.00087, 1.6, 0.0218453, 2.17885},
+      {2048, -593.8289, -29807.56, 1945685.0, 1.00443, 1.8, 0.0218703, 2.17766},
+      {4096, -600.8313, -27850.96, 1847407.0, 1.00919, 2.0, 0.0219153, 2.17834},
+      {8192, -620.9129, -25418.15, 1747494.0, 1.01156, 2.1, 0.0219500, 2.18032},
+      {16384, -619.2016, -25096.68, 1720802.0, 1.01436, 2.2, 0.0219859, 2.18015},
+      {32768, -609.0757, -26009.98, 1747943.0, 1.02130, 2.4, 0.0218748, 2.17915},
+      {65536, -634.0664, -23790.66, 1670707.0, 1.02770, 2.6, 0.0217135, 2.18162},
+      {131072, -663.9942, -21046.37, 1574611.0, 1.03429, 2.8, 0.0215090, 2.18463},
+      {262144, -673.6543, -20022.76, 1535887.0, 1.04271, 3.0, 0.0212593, 2.18562},  // 1206-1216
+  };
+  for (const auto &r : rows)
+    if (model & r.bit) {
+      otf->b = r.b;
+      otf->c2 = r.c2;
+      otf->c3 = r.c3;
+      otf->c4 = r.c4;
+      otf->temp = r.temp;
+      otf->rho0 = r.rho0;
+      otf->lj_params.h = r.h;
+    }
   otf->mu0 = 0.0;
   otf->xi = 1E4 / a3;
   otf->rhobf = 0.033 * a3;
@@ -2430,7 +2507,7 @@ void dft_ot_init_params(dft_ot_functional *otf, INT model) {
 // src/ot.c:66-258
 dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_substeps) {
   if (!supported_model(model)) {
-    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC, BACKFLOW, HD, HD2 and T0MK)\n",
+    fprintf(stderr, "libdft: dft_ot_alloc: model %ld not supported (PLAIN, KC, BACKFLOW, HD, HD2 and T0MK ... T3000MK)\n",
             (long)model);
     return nullptr;
   }
@@ -2483,6 +2560,13 @@ dft_ot_functional *dft_ot_alloc(INT model, wf *gwf, INT min_substeps, INT max_su
     ok = (d->norm_part = (double *)dmalloc(sizeof(double) * (strang_partials(d->nx, d->ny, d->nz, d->prec) + 1))) !=
          nullptr;
   if (ok) ok = (d->mid_dev = (fused::MidProg *)dmalloc(3 * sizeof(fused::MidProg))) != nullptr;
+  if (ok && has_ig(otf)) {
+    double tab[3 * ig::kTerms];
+    ig::fill_table(tab);
+    d->ig_g1 = ig::series(1.0, tab + ig::kTerms);
+    ok = (d->igtab = (double *)dmalloc(sizeof(tab))) != nullptr;
+    if (ok) DFT_HIP_CHECK(hipMemcpy(d->igtab, tab, sizeof(tab), hipMemcpyHostToDevice));
+  }
   if (ok) {
     if (d->prec == GRID_FP32) {
       auto *p = new Plan3D<float>();
@@ -2737,7 +2821,9 @@ REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf) {
 }
 
 void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density) {
-  if (!check_r(otf, grad, "dft_ot_energy_gradient") || !check_r(otf, density, "dft_ot_energy_gradient")) return;
+  if (!check_r(otf, grad, "dft_ot_energy_gradient") || !check_r(otf, density, "dft_ot_energy_gradient") ||
+      !no_ideal_gas(otf, "dft_ot_energy_gradient"))
+    return;
   DFT_BY_PREC(otf->device->prec, gradient_impl<T>(otf, (T *)grad->dev, (const T *)density->dev));
 }
 
@@ -2752,7 +2838,7 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
 
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_well")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_well") || !no_ideal_gas(otf, "dft_ot_adjoint_well")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || !energy || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_well: invalid arguments\n");
     return -1;
@@ -2772,7 +2858,7 @@ void dft_set_verbose(int level) { g_verbose = level; }
 
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity") || !no_ideal_gas(otf, "dft_ot_adjoint_cavity")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || rho0 <= 0.0 || !bubble_radius || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_cavity: invalid arguments\n");
     return -1;
@@ -2788,7 +2874,7 @@ int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radiu
 
 int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf) {
   if (!check_wf(otf, gwf, "dft_ot_potential_vjp") || !check_r(otf, rho_bar, "dft_ot_potential_vjp") ||
-      !check_r(otf, v_bar, "dft_ot_potential_vjp"))
+      !check_r(otf, v_bar, "dft_ot_potential_vjp") || !no_ideal_gas(otf, "dft_ot_potential_vjp"))
     return -1;
   if (otf->model & DFT_OT_BACKFLOW) {
     fprintf(stderr, "libdft: dft_ot_potential_vjp: V_OT of a backflow model depends on the phase of psi; use dft_ot_potential_vjp_psi\n");
@@ -2801,7 +2887,7 @@ int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, w
 
 int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_bar, wf *gwf) {
   if (!check_wf(otf, gwf, "dft_ot_potential_vjp_psi") || !check_c(otf, psi_bar, "dft_ot_potential_vjp_psi") ||
-      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi"))
+      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi") || !no_ideal_gas(otf, "dft_ot_potential_vjp_psi"))
     return -1;
   bool ok = false;
   DFT_BY_PREC(otf->device->prec, ok = potential_vjp_psi_impl<T>(otf, (cplx<T> *)psi_bar->dev,

@@ -241,16 +241,36 @@ static void add_im(double complex *pot, const double *f, long n) {
 /* ------------------------------------------------------ parameters (a1) -- */
 
 /* src/ot.c:1002-1271: the T = 0 row (1026-1036) for model < DFT_OT_T0MK, the
-   row of DFT_OT_T0MK (1038-1048; the thermal parametrisation at 0 K, with no
-   ideal-gas term) and the common block. */
+   rows of DFT_OT_T0MK (1038-1048; the thermal parametrisation at 0 K, with no
+   ideal-gas term) and of the thermal models T400MK ... T3000MK (1050-1216),
+   and the common block. */
 static void init_params(orc_ot *o, int64_t model) {
   double a3 = ORC_AUTOANG * ORC_AUTOANG * ORC_AUTOANG;
   o->b = -718.99; o->c2 = -2.411857e4; o->c2_exp = 2; o->c3 = 1.858496e6; o->c3_exp = 3;
   o->c4 = 0.0; o->temp = 0.0; o->rho0 = 0.0218360; o->lj_h = 2.1903;
-  if (model & ORC_OT_T0MK) {
-    o->b = -719.2435; o->c2 = -24258.88; o->c2_exp = 2; o->c3 = 1865257.0; o->c3_exp = 3;
-    o->c4 = 0.0; o->temp = 0.0; o->rho0 = 0.0218354; o->lj_h = 2.19035;
-  }
+  /* rows applied in the reference's order, each whose bit is set (the last one wins) */
+  static const struct { int64_t bit; double b, c2, c3, c4, temp, rho0, h; } rows[] = {
+    {16, -719.2435, -24258.88, 1865257.0, 0.0, 0.0, 0.0218354, 2.19035},        /* src/ot.c:1038-1048 */
+    {32, -714.2174, -24566.29, 1873203.0, 0.98004, 0.4, 0.0218351, 2.18982},    /* 1050-1060 */
+    {64, -705.1319, -25124.17, 1887707.0, 0.99915, 0.6, 0.0218346, 2.18887},
+    {128, -690.4745, -26027.12, 1911283.0, 0.99548, 0.8, 0.0218331, 2.18735},
+    {256, -646.5135, -28582.81, 1973737.0, 0.99666, 1.2, 0.0218298, 2.18287},
+    {512, -625.8123, -29434.03, 1984068.0, 0.99829, 1.4, 0.0218332, 2.18080},
+    {1024, -605.9788, -30025.96, 1980898.0, 1.00087, 1.6, 0.0218453, 2.17885},
+    {2048, -593.8289, -29807.56, 1945685.0, 1.00443, 1.8, 0.0218703, 2.17766},
+    {4096, -600.8313, -27850.96, 1847407.0, 1.00919, 2.0, 0.0219153, 2.17834},
+    {8192, -620.9129, -25418.15, 1747494.0, 1.01156, 2.1, 0.0219500, 2.18032},
+    {16384, -619.2016, -25096.68, 1720802.0, 1.01436, 2.2, 0.0219859, 2.18015},
+    {32768, -609.0757, -26009.98, 1747943.0, 1.02130, 2.4, 0.0218748, 2.17915},
+    {65536, -634.0664, -23790.66, 1670707.0, 1.02770, 2.6, 0.0217135, 2.18162},
+    {131072, -663.9942, -21046.37, 1574611.0, 1.03429, 2.8, 0.0215090, 2.18463},
+    {262144, -673.6543, -20022.76, 1535887.0, 1.04271, 3.0, 0.0212593, 2.18562},   /* 1206-1216 */
+  };
+  for (size_t i = 0; i < sizeof rows / sizeof rows[0]; i++)
+    if (model & rows[i].bit) {
+      o->b = rows[i].b; o->c2 = rows[i].c2; o->c2_exp = 2; o->c3 = rows[i].c3; o->c3_exp = 3;
+      o->c4 = rows[i].c4; o->temp = rows[i].temp; o->rho0 = rows[i].rho0; o->lj_h = rows[i].h;
+    }
   o->bf_xi = 1e4 / a3; o->bf_rhobf = 0.033 * a3;
   if (model & (ORC_OT_HD | ORC_OT_HD2)) {       /* src/ot.c:1004-1007 */
     o->hd_beta = 40.0 / a3; o->hd_rhom = 0.37 * a3; o->hd_C = 0.1;
@@ -427,6 +447,53 @@ static inline double bf_dg(const orc_ot *o, double r) {   /* d(G rho)/d rho = G 
   double x = o->bf_xi * (r - o->bf_rhobf), e = exp(-2.0 * fabs(x));
   return bf_g(o, r) - 2.0 * o->bf_xi * r * e / ((1.0 + e) * (1.0 + e));
 }
+/* Ideal Bose gas of the thermal models, as the reference's src/common.c states it
+   (the bulk closed forms call it, src/helium-ot-bulk.c:30-31, 54-55; libgrid's
+   grid_func6a/6b, which the potential and energy call with the same mass,
+   temperature and c4, src/ot.c:804-808, src/ot-energy.c:101-104, are absent):
+     lambda^3 = (2 pi / (M k_B T))^{3/2}                      (common.c:253-260)
+     g_s(z) = sum_{k=1}^{256} z^k / k^s                      (common.c:293-306)
+     z(x): the golden-section minimum of (x - g_{3/2}(z))^2 on [0, 1] to a
+           bracket of 1e-6, or 1 when x >= g_{3/2}(1)       (common.c:400-437)
+     eps = c4 k_B T (rho ln z - g_{5/2}(z) / lambda^3)        (common.c:489-497)
+     V = c4 k_B T (ln z + x / g_{1/2}(z) - g_{3/2}(z) / g_{1/2}(z)), x = rho lambda^3,
+         or -c4 k_B (T / lambda^3) g_{3/2}(1) when x >= g_{3/2}(1)  (common.c:510-530). */
+static int has_ig(const orc_ot *o) { return o->model >= ORC_OT_T400MK && o->model < ORC_GP; }
+static double ig_series(double z, double s) {
+  double v = 0.0, zk = 1.0;
+  for (int k = 1; k <= 256; k++) {
+    zk *= z;
+    v += zk / pow((double) k, s);
+  }
+  return v;
+}
+static double ig_lambda3(const orc_ot *o) {
+  double l = sqrt(2.0 * M_PI / (o->mass * (o->temp / ORC_AUTOK)));
+  return l * l * l;
+}
+static double ig_z(double x) {
+  const double gr = (sqrt(5.0) + 1.0) / 2.0;
+  if (x >= ig_series(1.0, 1.5)) return 1.0;
+  double a = 0.0, b = 1.0, c = b - (b - a) / gr, d = a + (b - a) / gr;
+  while (fabs(c - d) > 1e-6) {
+    double tc = x - ig_series(c, 1.5), td = x - ig_series(d, 1.5);
+    if (tc * tc < td * td) b = d; else a = c;
+    c = b - (b - a) / gr;
+    d = a + (b - a) / gr;
+  }
+  return (b + a) / 2.0;
+}
+double orc_ig_energy(const orc_ot *o, double rho) {
+  double l3 = ig_lambda3(o), z = ig_z(rho * l3);
+  return o->c4 * (o->temp / ORC_AUTOK) * (rho * log(z) - ig_series(z, 2.5) / l3);
+}
+double orc_ig_dEdRho(const orc_ot *o, double rho) {
+  double l3 = ig_lambda3(o), x = rho * l3, kt = o->temp / ORC_AUTOK, g1 = ig_series(1.0, 1.5);
+  if (x >= g1) return -o->c4 * kt / l3 * g1;
+  double z = ig_z(x), g12 = ig_series(z, 0.5), g32 = ig_series(z, 1.5);
+  return o->c4 * kt * (log(z) + x / g12 - g32 / g12);
+}
+
 /* the density the backflow uses: rho, or G(rho) rho with the switch */
 static void bf_density(const orc_ot *o, const double *rho, double *out, long n) {
   #pragma omp parallel for
@@ -589,6 +656,9 @@ static void potential_body(orc_ot *o, double complex *pot, const double *rho, do
     }
     add_bf(o, &g, pot, rho, vx, vy, vz, cw1, w[0], w[1], w[2], w[3], w[4], w[5]);
   }
+  if (has_ig(o))                               /* src/ot.c:402-407 */
+    #pragma omp parallel for
+    for (long i = 0; i < n; i++) pot[i] += orc_ig_dEdRho(o, rho[i]);
   free(rho_tf); free(cw1); free(cw2);
   for (int i = 0; i < 8; i++) free(w[i]);
 }
@@ -633,6 +703,9 @@ static void energy_body(orc_ot *o, double *eps, const double *rho, double *const
   for (long i = 0; i < n; i++) eps[i] += (o->c3 / 3.0) * (w1[i] * w1[i] * w1[i] * rho[i]);
   if (has_hd(o))                               /* src/ot-energy.c:96-99 */
     for (long i = 0; i < n; i++) eps[i] += hd_eps(o, rho[i]);
+  if (has_ig(o))                               /* src/ot-energy.c:101-104 */
+    #pragma omp parallel for
+    for (long i = 0; i < n; i++) eps[i] += orc_ig_energy(o, rho[i]);
 
   if (o->model & ORC_OT_KC) {
     double *gr[3], *w[3];
@@ -915,11 +988,15 @@ void orc_observables(orc_ot *o, const double complex *psi, const double *vext, d
 
 /* src/helium-ot-bulk.c:21-33 */
 double orc_bulk_energy(const orc_ot *o, double rho) {
-  return 0.5 * o->b * rho * rho + 0.5 * o->c2 * rho * rho * rho + (1.0 / 3.0) * o->c3 * rho * rho * rho * rho;
+  double e = 0.5 * o->b * rho * rho + 0.5 * o->c2 * rho * rho * rho + (1.0 / 3.0) * o->c3 * rho * rho * rho * rho;
+  if (o->c4 != 0.0) e += orc_ig_energy(o, rho);
+  return e;
 }
 /* src/helium-ot-bulk.c:45-57 */
 double orc_bulk_dEdRho(const orc_ot *o, double rho) {
-  return o->b * rho + 1.5 * o->c2 * rho * rho + (4.0 / 3.0) * o->c3 * rho * rho * rho;
+  double d = o->b * rho + 1.5 * o->c2 * rho * rho + (4.0 / 3.0) * o->c3 * rho * rho * rho;
+  if (o->c4 != 0.0) d += orc_ig_dEdRho(o, rho);
+  return d;
 }
 /* src/helium-ot-bulk.c:143-146 */
 double orc_bulk_pressure(const orc_ot *o, double rho) { return rho * orc_bulk_dEdRho(o, rho) - orc_bulk_energy(o, rho); }

@@ -37,6 +37,8 @@
 #define ORC_OT_HD2      4
 #define ORC_OT_BACKFLOW 8
 #define ORC_OT_T0MK     16
+#define ORC_OT_T400MK   32      /* the thermal rows: one bit each, 32 ... 262144 */
+#define ORC_GP          524288
 
 /* Unit conversions: CODATA 2018 (libgrid's au.h, which the reference uses, is not available). */
 #define ORC_AUTOANG 0.529177210903
@@ -112,6 +114,10 @@ double orc_integral(const double *f, int64_t n, double step);
 void orc_kernel_map(orc_ot *otf, int which, double *out, int min_sub, int max_sub);
 
 /* Bulk closed forms, src/helium-ot-bulk.c:21-258, 379-463. */
+/* Ideal Bose gas of the thermal models (tests): energy per volume and its
+   density derivative at density rho. */
+double orc_ig_energy(const orc_ot *otf, double rho);
+double orc_ig_dEdRho(const orc_ot *otf, double rho);
 double orc_bulk_energy(const orc_ot *otf, double rho);
 double orc_bulk_dEdRho(const orc_ot *otf, double rho);
 double orc_bulk_pressure(const orc_ot *otf, double rho);

@@ -41,6 +41,7 @@ def lib():
         P, D, I64, I = C.c_void_p, C.c_double, C.c_int64, C.c_int
         sig = {
             "orc_ot_alloc": (C.POINTER(_OrcOT), [I64, I, I, I, D, I, I]),
+            "orc_ig_energy": (D, [P, D]), "orc_ig_dEdRho": (D, [P, D]),
             "orc_ot_free": (None, [P]),
             "orc_ot_set_kc_mode": (None, [P, I]),
             "orc_ot_potential": (None, [P, P, P, D, P]),

@@ -72,6 +72,22 @@ def test_init_params_t0mk_row():
     assert abs(_bulk(p, "density") / p.rho0 - 1) < 1e-5
 
 
+def test_init_params_thermal_rows():
+    """The rows of T400MK ... T3000MK (src/ot.c:1050-1216) and their temperatures.
+    Up to 1.6 K each row has zero pressure at its own rho0 (to 1e-4 of the energy
+    density) once the ideal Bose gas is included; above, the saturated gas term
+    leaves 1e-3 (2 K) to 1e-2 (3 K) of it, so only the row's digits are checked."""
+    want = {32: (-714.2174, 0.98004, 0.4, 0.0218351), 256: (-646.5135, 0.99666, 1.2, 0.0218298),
+            4096: (-600.8313, 1.00919, 2.0, 0.0219153), 262144: (-673.6543, 1.04271, 3.0, 0.0212593)}
+    a3 = L.AUTOANG ** 3
+    for bit, (b, c4, temp, rho0) in want.items():
+        p = L.host_params(bit | L.DFT_OT_KC)
+        assert abs(p.b * L.AUTOK * a3 - b) < 1e-9 and p.c4 == c4 and p.temp == temp
+        assert abs(p.rho0 / a3 - rho0) < 1e-12
+        if temp <= 1.6:
+            assert abs(_bulk(p, "pressure", p.rho0)) < 1e-4 * abs(_bulk(p, "energy", p.rho0)), bit
+
+
 def test_equation_of_state_fixture_product():
     otf = L.host_params(L.DFT_OT_PLAIN)
     rows = _eos_rows()
@@ -86,7 +102,7 @@ def test_equation_of_state_fixture_product():
         assert abs(mu - mu_ref) < 1.2e-4 + 6e-6 * P, (P, mu, mu_ref)
 
 
-@pytest.mark.parametrize("model", [0, 1, 8, 9, 16, 25])
+@pytest.mark.parametrize("model", [0, 1, 8, 9, 16, 25, 256 | 9, 262144])
 def test_product_bulk_matches_oracle(oracle, model):
     po = oracle
     otf = L.host_params(model)

@@ -21,8 +21,7 @@ its maximum; the bar is 5x the FP32 one (1e-4) for that case only.
 The transposed Jacobian of the switched potential (dft_ot_potential_vjp_psi)
 is checked against central differences of dft_ot_potential (dot-product test)
 and the adjoint sweep through it against central differences of the final
-energy, both with the widened switch in FP64.  The thermal models above 0 K
-(which need libgrid's ideal-gas term) are refused at allocation.
+energy, both with the widened switch in FP64.
 """
 import numpy as np
 import pytest
@@ -253,12 +252,12 @@ def test_switch_adjoint_well_matches_fd():
 
 
 def test_models_accepted_and_refused():
-    """HD / HD2 with BACKFLOW and T0MK are accepted; the thermal models above 0 K
-    are refused at allocation."""
+    """HD / HD2 with BACKFLOW and T0MK are accepted; GP, DR and ZERO are refused at
+    allocation."""
     w = L.Wave(SHAPE, STEP, L.GRID_FP64)
     w.set(_state(0.0218 * L.AUTOANG ** 3))
     lib = L.lib()
-    for m in (32, 64 | 9, 1 << 18):
+    for m in (524288, 1048576 | 9, 2097152):
         assert not lib.dft_ot_alloc(m, w.h, 2, 8), m
     for m in (2 | 9, 16 | 4 | 8, 16 | 1):
         f = L.Functional(m, w, 2, 8)

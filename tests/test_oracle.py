@@ -293,3 +293,91 @@ def test_kinetic_energy_plane_wave(oracle):
     psi = np.exp(1j * k * z)[None, None, :] * np.ones(shape)
     ek = po.kinetic_energy(psi, step)
     assert abs(ek - k * k / (2 * po.HELIUM_MASS) * po.norm(psi, step)) < 1e-12 * ek
+
+
+# ------------------------------------------- ideal Bose gas (thermal models) --
+# The thermal rows T400MK ... T3000MK add the ideal Bose gas of src/common.c:482-530
+# (the oracle restates it; checks fixed by the mathematics of the 256-term
+# polylog series and of the gas, not by the formulas themselves).
+T1200 = 256
+
+
+def _series(z, s, terms=256):
+    k = np.arange(1, terms + 1, dtype=float)
+    return float(np.sum(z ** k / k ** s))
+
+
+def _fugacity(x):
+    """z with g_{3/2}(z) = x for the 256-term series, by bisection to 1e-16."""
+    a, b = 0.0, 1.0
+    for _ in range(80):
+        c = 0.5 * (a + b)
+        if _series(c, 1.5) < x:
+            a = c
+        else:
+            b = c
+    return 0.5 * (a + b)
+
+
+def _lambda3_si(temp, mass_amu=4.0026):
+    """Thermal wavelength cubed (bohr^3) from SI constants: h / sqrt(2 pi m k_B T)."""
+    h, kb, amu, bohr = 6.62607015e-34, 1.380649e-23, 1.66053906660e-27, 0.529177210903e-10
+    lam = h / np.sqrt(2 * np.pi * mass_amu * amu * kb * temp) / bohr
+    return lam ** 3
+
+
+def test_ideal_gas_against_the_exact_fugacity(oracle):
+    """Below saturation the potential is c4 k_B T ln z(x), x = rho lambda^3, to second
+    order in the search bracket (1e-8 here), and the energy per volume
+    c4 k_B T (rho ln z - g_{5/2}(z) / lambda^3) to first order (2e-5); above it both
+    are the saturated constants -c4 k_B T g_{3/2}(1) / lambda^3 and
+    -c4 k_B T g_{5/2}(1) / lambda^3 of the truncated series; lambda^3 is the SI one."""
+    po = oracle
+    o = po.OracleOT(T1200, (8, 8, 8), 1.0, 1, 1)
+    lib = po.lib()
+    kt = o.p.c4 * o.p.temp / po.AUTOK
+    l3 = _lambda3_si(o.p.temp)
+    g1 = _series(1.0, 1.5)
+    V_sat = lib.orc_ig_dEdRho(o._h, 10.0 / l3)
+    assert abs(V_sat / (-kt * g1 / l3) - 1) < 1e-6  # lambda^3 from the SI constants
+    assert abs(lib.orc_ig_energy(o._h, 10.0 / l3) / (-kt * _series(1.0, 2.5) / l3) - 1) < 1e-6
+    for x in (0.05, 0.3, 1.0, 2.0, 2.4):
+        z = _fugacity(x)
+        rho = x / l3
+        assert abs(lib.orc_ig_dEdRho(o._h, rho) / (kt * np.log(z)) - 1) < 1e-8, x
+        e = kt * (rho * np.log(z) - _series(z, 2.5) / l3)
+        assert abs(lib.orc_ig_energy(o._h, rho) / e - 1) < 2e-5, x
+
+
+def test_ideal_gas_classical_limit_and_derivative(oracle):
+    """For x << 1, g_s(z) = z + z^2 / 2^s + ..., so ln z = ln x - x / 2^{3/2} + O(x^2):
+    the classical gas plus its first quantum correction.  The potential is the
+    derivative of the energy per volume (central differences, 1e-5)."""
+    po = oracle
+    o = po.OracleOT(T1200, (8, 8, 8), 1.0, 1, 1)
+    lib = po.lib()
+    kt = o.p.c4 * o.p.temp / po.AUTOK
+    l3 = _lambda3_si(o.p.temp)
+    x = 2e-3
+    want = kt * (np.log(x) - x / 2 ** 1.5)
+    assert abs(lib.orc_ig_dEdRho(o._h, x / l3) - want) < 1e-5 * abs(want)
+    for x in (0.1, 0.8, 2.0):
+        r, h = x / l3, 1e-4 * x / l3
+        fd = (lib.orc_ig_energy(o._h, r + h) - lib.orc_ig_energy(o._h, r - h)) / (2 * h)
+        assert abs(lib.orc_ig_dEdRho(o._h, r) / fd - 1) < 1e-5, x
+
+
+@pytest.mark.parametrize("model", [T1200 | 9, 262144], ids=["t1200-full", "t3000"])
+def test_uniform_density_invariant_thermal(oracle, model):
+    """Uniform density: V_OT = dE/drho and eps = E of the bulk closed forms, which
+    include the ideal Bose gas (src/helium-ot-bulk.c:30-31, 54-55), at the row's
+    rho0 (saturated at these temperatures) and at rho0 / 10 (not saturated)."""
+    po = oracle
+    shape = (8, 8, 8)
+    o = po.OracleOT(model, shape, _ang(0.5, po), 2, 8)
+    for rho in (o.p.rho0, 0.1 * o.p.rho0):
+        psi = np.full(shape, np.sqrt(rho), np.complex128)
+        mu = o.bulk("dEdRho", rho)
+        assert np.abs(o.potential(psi) - mu).max() < 1e-12 * abs(mu)
+        eb = o.bulk("energy", rho)
+        assert np.abs(o.energy_density(psi) - eb).max() < 1e-12 * abs(eb)
