@@ -290,7 +290,7 @@ def size_sweep(steps=10, warmup=3):
          lambda n, h: inputs.vortex_line((n, n, n), h, rho0), 2.0 / L.AUTOFS),
     ]
     out = []
-    # config 1's pattern: imaginary-time steps renormalised every step, fused and as the
+    # the plain-OT 64^3 workload's pattern: imaginary-time steps renormalised every step, fused and as the
     # reference driver does it (the step, then grid_wf_normalize)
     n, step, prec = 64, 0.5 / L.AUTOANG, L.GRID_FP64
     log("size sweep: plain OT 64^3 imaginary time, renormalised every step")

@@ -153,7 +153,7 @@ def test_kernel_spectra_match_oracle(oracle):
 
 
 @pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
-def test_uniform_invariant_config_sizes(prec):
+def test_uniform_invariant_full_128(prec):
     """psi = sqrt(rho0): V = dE/drho(rho0), eps = E_bulk(rho0) at 128^3 full OT."""
     shape, step = (128, 128, 128), 0.4 / L.AUTOANG
     w = L.Wave(shape, step, prec)

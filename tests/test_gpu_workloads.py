@@ -202,7 +202,7 @@ def test_vortex256_circulation_and_conservation():
 
 
 def test_dftrun_driver_runs():
-    """The C driver (host C on the C ABI) runs the config-1 workload and reports JSON lines."""
+    """The C driver (host C on the C ABI) runs the plain-OT 64^3 imaginary-time workload and reports JSON lines."""
     exe = os.path.join(ROOT, "libdft_amd", "dftrun")
     out = subprocess.run([exe, "bulk64", "--steps", "20", "--report", "10"], capture_output=True, text=True,
                          timeout=300, check=True).stdout

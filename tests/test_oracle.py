@@ -266,7 +266,8 @@ def test_norm_conservation_real_potential(oracle):
 
 
 def test_imaginary_time_converges_to_bulk(oracle):
-    """Config-1 pattern at 16^3: noisy bulk relaxes to E/N = E_bulk(rho)/rho, mu = dE/drho."""
+    """Imaginary-time relaxation of a noisy bulk at 16^3 (renormalised every step) to
+    E/N = E_bulk(rho)/rho, mu = dE/drho."""
     po = oracle
     shape, step = (16, 16, 16), _ang(0.5, po)
     o = po.OracleOT(0, shape, step, 2, 8)
