@@ -390,8 +390,8 @@ REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf);
 /* Hand-written backward of E[rho] = h^3 sum eps (LJ, c2/c3, KC parts of
    src/ot-energy.c:38-191): grad = dE/drho, the exact derivative of the
    discrete energy.  For PLAIN it equals dft_ot_potential's output; for KC it
-   differs from it (src/ot.c:660-675 omits 1/rho_0s on terms 2 and 3).  The
-   thermal models above 0 K are refused (message; grad untouched). */
+   differs from it (src/ot.c:660-675 omits 1/rho_0s on terms 2 and 3).  With a
+   thermal model the ideal Bose gas adds c4 k_B T ln z below saturation. */
 void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density);
 /* E[rho] of the same parts, FP64 reduction. */
 REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density);
@@ -401,15 +401,14 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density);
    HD/HD2 penalty) at the state gwf:
    rho_bar = (dV_OT/drho)^T v_bar, the transpose of the potential's Jacobian
    applied to v_bar.  Returns 0, or -1 (message on stderr) for a backflow
-   model, a thermal model above 0 K, mismatched grids or lack of device memory. */
+   model, mismatched grids or lack of device memory. */
 int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf);
 /* Reverse pass of dft_ot_potential with respect to the wave function, for every
    model (backflow's V depends on psi's phase through the velocity,
    src/ot.c:385-396, 840-991): psi_bar = (dV_OT/dpsi)^T v_bar, where Re v_bar is
    the adjoint of Re V and Im v_bar that of Im V, in the convention
    dJ = Re sum conj(psi_bar) dpsi.  psi_bar is overwritten.  Returns 0, or -1
-   (message on stderr) for a thermal model above 0 K, mismatched grids or lack
-   of device memory. */
+   (message on stderr) for mismatched grids or lack of device memory. */
 int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_bar, wf *gwf);
 /* Gradient of the total energy E[psi_K] = E_kin + E_OT + h^3 sum rho V_ext
    after nsteps steps of dft_ot_propagate(otf, ., V_ext, mu, time) from gwf
@@ -417,9 +416,9 @@ int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_ba
    r_i = (i - n/2) step - r0, with respect to (amp, width): grad[0..1].  An
    adjoint sweep that keeps every `every`-th state and recomputes each segment
    in the reverse pass (memory: nsteps/every + every states).  Every model
-   (PLAIN, KC, BACKFLOW, HD/HD2 and its backflow switch, T0MK); returns 0, or -1
-   (message on stderr) for invalid arguments, lack of device memory or a
-   thermal model above 0 K. */
+   (PLAIN, KC, BACKFLOW, HD/HD2 and its backflow switch, the thermal models);
+   returns 0, or -1 (message on stderr) for invalid arguments or lack of device
+   memory. */
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad);
 /* Released cavity: writes psi_0 = sqrt(rho0 (1 + tanh((r - radius)/width)) / 2)
@@ -429,9 +428,8 @@ int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, R
    W = (1 - tanh((r - 1.5 radius) / (4 width))) / 2 (the missing volume of the
    whole box is the conserved N / rho0), of the final state and its derivative
    with respect to the initial cavity radius, from the same
-   checkpointed adjoint sweep as dft_ot_adjoint_well.  The models of
-   dft_ot_adjoint_well; returns 0, or -1 (message on stderr) for invalid
-   arguments, lack of device memory or a thermal model above 0 K. */
+   checkpointed adjoint sweep as dft_ot_adjoint_well.  Every model; returns 0,
+   or -1 (message on stderr) for invalid arguments or lack of device memory. */
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad);
 /* Progress lines of the adjoint sweeps on stderr (level > 0), one per checkpoint. */

@@ -79,6 +79,23 @@ DFT_IG_FN double potential(double rho, const Params &p) {
   return p.ckt * (std::log(z) + x / g12 - g32 / g12);
 }
 
+// Derivatives for the reverse passes.  The search returns a z that is constant
+// between the points where its branch decisions change (steps of ~1e-6 in z),
+// so, between them, d(energy)/d rho = c4 k_B T ln z and d(potential)/d rho =
+// c4 k_B T lambda^3 / g_{1/2}(z); both are 0 at saturation.  (The steps
+// themselves move the energy and the potential by about (1e-6 / z)^2 of their
+// values: below 1e-10 for z > 0.1.)
+DFT_IG_FN double energy_derivative(double rho, const Params &p) {
+  const double x = rho * p.l3;
+  if (x >= p.g1) return 0.0;
+  return p.ckt * std::log(fugacity(x, p.tab + kTerms, p.g1));
+}
+DFT_IG_FN double potential_derivative(double rho, const Params &p) {
+  const double x = rho * p.l3;
+  if (x >= p.g1) return 0.0;
+  return p.ckt * p.l3 / series(fugacity(x, p.tab + kTerms, p.g1), p.tab);
+}
+
 // lambda^3 of mass M (a.u.) at T kelvin (hbar = 1, k_B = 1 / autok)
 inline double lambda3(double mass, double temp, double autok) {
   const double l = std::sqrt(2.0 * M_PI / (mass * (temp / autok)));

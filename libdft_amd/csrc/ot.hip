@@ -291,13 +291,21 @@ __global__ void k_final(cplx<T> *V, const T *__restrict__ vre, const T *__restri
   }
 }
 
-// Ideal Bose gas of the thermal models (src/ot.c:402-407, src/ot-energy.c:101-104):
-// out += V_ig(rho) (potential, or energy per volume with EPS), in FP64.
-template <typename T, bool EPS>
-__global__ void k_ideal_gas(T *out, const T *__restrict__ rho, ig::Params p, int64_t n) {
+// Ideal Bose gas of the thermal models (src/ot.c:402-407, src/ot-energy.c:101-104),
+// in FP64: out += the potential (MODE 0), the energy per volume (1), or its
+// derivative for the energy gradient (2); with MODE 3, out += vbar dV/drho
+// (the transposed Jacobian of the potential).
+template <typename T, int MODE>
+__global__ void k_ideal_gas(T *out, const T *__restrict__ rho, const T *__restrict__ vbar, ig::Params p,
+                            int64_t n) {
   GRID_STRIDE(c, n) {
     const double r = (double)rho[c];
-    out[c] += (T)(EPS ? ig::energy(r, p) : ig::potential(r, p));
+    double v;
+    if constexpr (MODE == 0) v = ig::potential(r, p);
+    else if constexpr (MODE == 1) v = ig::energy(r, p);
+    else if constexpr (MODE == 2) v = ig::energy_derivative(r, p);
+    else v = (double)vbar[c] * ig::potential_derivative(r, p);
+    out[c] += (T)v;
   }
 }
 
@@ -1129,7 +1137,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
   // dft_ot_potential2: src/ot.c:402-407, 505-510)
   if (LOC && parts.lj < 0 && has_ig(otf)) {
     const ig::Params igp = ig_params(otf);
-    LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, false>), S<T>(d, VRE), (const T *)rho, igp, d->n);
+    LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, 0>), S<T>(d, VRE), (const T *)rho, (const T *)nullptr, igp, d->n);
   }
   // P4
   const bool PW_ = LOC || KC || BF;  // pointwise stages ran (V_re in VRE)
@@ -1509,7 +1517,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
   // ---- ideal Bose gas of the thermal models: pointwise, into V_re
   if (has_ig(otf)) {
     const ig::Params igp = ig_params(otf);
-    LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, false>), (T *)d->slot[VRE], (const T *)rho, igp, n);
+    LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, 0>), (T *)d->slot[VRE], (const T *)rho, (const T *)nullptr, igp, n);
   }
   // ---- stage C: linear terms summed in k space, z^-1, accumulate or start the step
   {
@@ -1640,7 +1648,7 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
 #undef EB
     if (LOC && has_ig(otf)) {  // src/ot-energy.c:101-104
       const ig::Params igp = ig_params(otf);
-      LAUNCH_PW("ot_ideal_gas_energy", 3, (k_ideal_gas<T, true>), eps, (const T *)rho, igp, d->n);
+      LAUNCH_PW("ot_ideal_gas_energy", 3, (k_ideal_gas<T, 1>), eps, (const T *)rho, (const T *)nullptr, igp, d->n);
     }
     if (KC) {
       const T *in3[3] = {S<T>(d, QX), S<T>(d, QY), S<T>(d, QZ)};
@@ -1719,6 +1727,10 @@ template <typename T> void gradient_impl(dft_ot_functional *otf, T *grad, const 
   }
   LAUNCH_PW("ot_grad_final", KC ? 6 : 3, (k_grad_final<T>), grad, S<T>(d, VRE), S<T>(d, RB),
             KC ? S<T>(d, WX) : (T *)nullptr, S<T>(d, WY), S<T>(d, WZ), pw);
+  if (has_ig(otf)) {
+    const ig::Params igp = ig_params(otf);
+    LAUNCH_PW("ot_ideal_gas_grad", 3, (k_ideal_gas<T, 2>), grad, rho, (const T *)nullptr, igp, d->n);
+  }
 }
 
 // Build the kernel spectra (FP64 on the device), src/ot.c:138-217.
@@ -1815,16 +1827,6 @@ rgrid *spectral_handle(dft_ot_functional *otf, int w, const char *id) {
 bool supported_model(INT model) {
   // PLAIN, KC, HD, HD2, BACKFLOW and the parameter rows T0MK ... T3000MK (bits 16 ... 262144)
   return !(model & ~(INT)(DFT_OT_KC | DFT_OT_BACKFLOW | DFT_OT_HD | DFT_OT_HD2 | ((INT)524288 - 16)));
-}
-
-// The entry points whose derivative does not cover the ideal Bose gas term
-// (its fugacity is a bracketed search, not a differentiable function of rho).
-bool no_ideal_gas(dft_ot_functional *otf, const char *fn) {
-  if (has_ig(otf)) {
-    fprintf(stderr, "libdft: %s: not available for the thermal models above 0 K\n", fn);
-    return false;
-  }
-  return true;
 }
 
 // Argument checks of the entry points: false (with a libdft: message) when
@@ -1988,6 +1990,10 @@ void vjp_ot(dft_ot_functional *otf, const T *vbar, const T *rho, cplx<T> *psib, 
     fft::Combine c = cmb1(0, K_SA, -1, h3N);
     T *out[1] = {w.r[ACONV]};
     adj_conv<T>(d, w, 1, in, 1, &c, out);
+  }
+  if (has_ig(otf)) {
+    const ig::Params igp = ig_params(otf);
+    LAUNCH_PW("ot_adjoint_ideal_gas", 3, (k_ideal_gas<T, 3>), w.r[ARP], rho, vbar, igp, d->n);
   }
   LAUNCH_PW("ot_adjoint_out", 4, (k_vjp_out<T>), rout, psib, psi, (const T *)w.r[ARP], (const T *)w.r[ACONV], d->n);
 }
@@ -2821,8 +2827,7 @@ REAL dft_ot_energy(dft_ot_functional *otf, wf *gwf) {
 }
 
 void dft_ot_energy_gradient(dft_ot_functional *otf, rgrid *grad, rgrid *density) {
-  if (!check_r(otf, grad, "dft_ot_energy_gradient") || !check_r(otf, density, "dft_ot_energy_gradient") ||
-      !no_ideal_gas(otf, "dft_ot_energy_gradient"))
+  if (!check_r(otf, grad, "dft_ot_energy_gradient") || !check_r(otf, density, "dft_ot_energy_gradient"))
     return;
   DFT_BY_PREC(otf->device->prec, gradient_impl<T>(otf, (T *)grad->dev, (const T *)density->dev));
 }
@@ -2838,7 +2843,7 @@ REAL dft_ot_energy_rho(dft_ot_functional *otf, rgrid *density) {
 
 int dft_ot_adjoint_well(dft_ot_functional *otf, wf *gwf, REAL amp, REAL width, REAL x0, REAL y0, REAL z0, REAL mu,
                         REAL_COMPLEX time, INT nsteps, INT every, REAL *energy, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_well") || !no_ideal_gas(otf, "dft_ot_adjoint_well")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_well")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || !energy || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_well: invalid arguments\n");
     return -1;
@@ -2858,7 +2863,7 @@ void dft_set_verbose(int level) { g_verbose = level; }
 
 int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radius, REAL width, REAL mu,
                           REAL_COMPLEX time, INT nsteps, INT every, REAL *bubble_radius, REAL *grad) {
-  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity") || !no_ideal_gas(otf, "dft_ot_adjoint_cavity")) return -1;
+  if (!check_wf(otf, gwf, "dft_ot_adjoint_cavity")) return -1;
   if (nsteps < 1 || every < 1 || width <= 0.0 || rho0 <= 0.0 || !bubble_radius || !grad) {
     fprintf(stderr, "libdft: dft_ot_adjoint_cavity: invalid arguments\n");
     return -1;
@@ -2874,7 +2879,7 @@ int dft_ot_adjoint_cavity(dft_ot_functional *otf, wf *gwf, REAL rho0, REAL radiu
 
 int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, wf *gwf) {
   if (!check_wf(otf, gwf, "dft_ot_potential_vjp") || !check_r(otf, rho_bar, "dft_ot_potential_vjp") ||
-      !check_r(otf, v_bar, "dft_ot_potential_vjp") || !no_ideal_gas(otf, "dft_ot_potential_vjp"))
+      !check_r(otf, v_bar, "dft_ot_potential_vjp"))
     return -1;
   if (otf->model & DFT_OT_BACKFLOW) {
     fprintf(stderr, "libdft: dft_ot_potential_vjp: V_OT of a backflow model depends on the phase of psi; use dft_ot_potential_vjp_psi\n");
@@ -2887,7 +2892,7 @@ int dft_ot_potential_vjp(dft_ot_functional *otf, rgrid *rho_bar, rgrid *v_bar, w
 
 int dft_ot_potential_vjp_psi(dft_ot_functional *otf, cgrid *psi_bar, cgrid *v_bar, wf *gwf) {
   if (!check_wf(otf, gwf, "dft_ot_potential_vjp_psi") || !check_c(otf, psi_bar, "dft_ot_potential_vjp_psi") ||
-      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi") || !no_ideal_gas(otf, "dft_ot_potential_vjp_psi"))
+      !check_c(otf, v_bar, "dft_ot_potential_vjp_psi"))
     return -1;
   bool ok = false;
   DFT_BY_PREC(otf->device->prec, ok = potential_vjp_psi_impl<T>(otf, (cplx<T> *)psi_bar->dev,

@@ -487,6 +487,14 @@ double orc_ig_energy(const orc_ot *o, double rho) {
   double l3 = ig_lambda3(o), z = ig_z(rho * l3);
   return o->c4 * (o->temp / ORC_AUTOK) * (rho * log(z) - ig_series(z, 2.5) / l3);
 }
+/* the derivative of that energy per volume: the search's z is constant between
+   the steps of its branch decisions, so it is c4 k_B T ln z below saturation and
+   0 above (unlike the potential, which keeps the reference's saturated constant) */
+double orc_ig_dEdRho_energy(const orc_ot *o, double rho) {
+  double l3 = ig_lambda3(o), x = rho * l3;
+  if (x >= ig_series(1.0, 1.5)) return 0.0;
+  return o->c4 * (o->temp / ORC_AUTOK) * log(ig_z(x));
+}
 double orc_ig_dEdRho(const orc_ot *o, double rho) {
   double l3 = ig_lambda3(o), x = rho * l3, kt = o->temp / ORC_AUTOK, g1 = ig_series(1.0, 1.5);
   if (x >= g1) return -o->c4 * kt / l3 * g1;
@@ -835,6 +843,8 @@ void orc_ot_energy_gradient(orc_ot *o, double *grad, const double *rho) {
     for (long i = 0; i < n; i++) grad[i] += (c / o->rho_0s) * w1[i];
     free(rho_st); free(gd); free(J); free(H); free(M);
   }
+  if (has_ig(o))   /* the ideal gas: c4 k_B T ln z below saturation, 0 above */
+    for (long i = 0; i < n; i++) grad[i] += orc_ig_dEdRho_energy(o, rho[i]);
   free(rho_tf); free(cw); free(w1); free(w2); free(pot);
 }
 

@@ -118,6 +118,7 @@ void orc_kernel_map(orc_ot *otf, int which, double *out, int min_sub, int max_su
    density derivative at density rho. */
 double orc_ig_energy(const orc_ot *otf, double rho);
 double orc_ig_dEdRho(const orc_ot *otf, double rho);
+double orc_ig_dEdRho_energy(const orc_ot *otf, double rho);
 double orc_bulk_energy(const orc_ot *otf, double rho);
 double orc_bulk_dEdRho(const orc_ot *otf, double rho);
 double orc_bulk_pressure(const orc_ot *otf, double rho);
