@@ -328,9 +328,10 @@ def _lambda3_si(temp, mass_amu=4.0026):
 
 
 def test_ideal_gas_against_the_exact_fugacity(oracle):
-    """Below saturation the potential is c4 k_B T ln z(x), x = rho lambda^3, to second
-    order in the search bracket (1e-8 here), and the energy per volume
-    c4 k_B T (rho ln z - g_{5/2}(z) / lambda^3) to first order (2e-5); above it both
+    """Below saturation the potential is c4 k_B T ln z(x), x = rho lambda^3, and the
+    energy per volume c4 k_B T (rho ln z - g_{5/2}(z) / lambda^3), both to second
+    order in the search bracket (the energy is stationary in z at the root; the
+    potential's correction terms cancel the first order): 1e-9 and 1e-8; above it both
     are the saturated constants -c4 k_B T g_{3/2}(1) / lambda^3 and
     -c4 k_B T g_{5/2}(1) / lambda^3 of the truncated series; lambda^3 is the SI one."""
     po = oracle
@@ -347,7 +348,7 @@ def test_ideal_gas_against_the_exact_fugacity(oracle):
         rho = x / l3
         assert abs(lib.orc_ig_dEdRho(o._h, rho) / (kt * np.log(z)) - 1) < 1e-8, x
         e = kt * (rho * np.log(z) - _series(z, 2.5) / l3)
-        assert abs(lib.orc_ig_energy(o._h, rho) / e - 1) < 2e-5, x
+        assert abs(lib.orc_ig_energy(o._h, rho) / e - 1) < 1e-9, x
 
 
 def test_ideal_gas_classical_limit_and_derivative(oracle):
