@@ -27,7 +27,9 @@ PRECS = [L.GRID_FP64, L.GRID_FP32]
 PNAME = {L.GRID_FP64: "fp64", L.GRID_FP32: "fp32"}
 TOL = {L.GRID_FP64: 1e-10, L.GRID_FP32: 2e-5}
 SHAPE, STEP = (16, 16, 32), 0.6 / L.AUTOANG
-MODELS = {"t1200-full": 256 | 9, "t3000": 262144, "t400-kc": 32 | 1}
+# (t1200-hd-full: with the HD penalty and its backflow switch too; the state reaches 0.041 A^-3,
+# across the switch, whose FP32 bar is 5x the usual one: see test_gpu_hd_backflow.py)
+MODELS = {"t1200-full": 256 | 9, "t3000": 262144, "t400-kc": 32 | 1, "t1200-hd-full": 256 | 2 | 9}
 
 
 def relerr(a, b):
@@ -56,12 +58,13 @@ def test_thermal_matches_oracle(oracle, prec, name):
     _log("%s prec %d: Re V %.2e, Im V %.2e, eps %.2e" % (name, prec, relerr(V.real, Vo.real),
                                                          relerr(V.imag, Vo.imag) if model & 8 else 0.0,
                                                          relerr(e, eo)))
-    assert relerr(V.real, Vo.real) < TOL[prec]
+    tol = TOL[prec] * (5 if prec == L.GRID_FP32 and model & 2 and model & 8 else 1)
+    assert relerr(V.real, Vo.real) < tol
     if model & 8:
-        assert relerr(V.imag, Vo.imag) < TOL[prec] * 10
-    assert relerr(e, eo) < TOL[prec]
+        assert relerr(V.imag, Vo.imag) < tol * 10
+    assert relerr(e, eo) < tol
     E, Eo = f.energy(), eo.sum() * STEP ** 3
-    assert abs(E - Eo) < TOL[prec] * np.abs(eo).sum() * STEP ** 3
+    assert abs(E - Eo) < tol * np.abs(eo).sum() * STEP ** 3
     mu0 = o.bulk("dEdRho", o.p.rho0)
     for dt in (0.5 / L.AUTOFS, -1j * 0.5 / L.AUTOFS):
         want = psi.copy()
@@ -70,7 +73,7 @@ def test_thermal_matches_oracle(oracle, prec, name):
         L.lib().dft_ot_propagate(f.h, w.h, None, mu0, L.Complex.of(dt))
         err = relerr(w.get(), want)
         _log("  step dt %s: rel %.2e" % (dt, err))
-        assert err < TOL[prec]
+        assert err < tol
     pot.free(), eps.free(), f.free(), w.free()
 
 
