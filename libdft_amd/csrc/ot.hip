@@ -910,6 +910,8 @@ namespace {
 // The thermal models above 0 K add the ideal Bose gas term (src/ot.c:402,
 // src/ot-energy.c:102: model >= DFT_OT_T400MK, GP / DR / ZERO excluded).
 bool has_ig(const dft_ot_functional *otf) { return otf->model >= 32 && otf->model < 524288; }
+// ... and its table is on the device (allocated with the functional)
+bool ig_on(const dft_ot_functional *otf) { return has_ig(otf) && otf->device->igtab; }
 ig::Params ig_params(const dft_ot_functional *otf) {
   ig::Params p;
   p.tab = otf->device->igtab;
@@ -1135,7 +1137,7 @@ void potential_impl(dft_ot_functional *otf, cgrid *V, const cplx<T> *psi, double
   }
   // ideal Bose gas of the thermal models, with the whole potential (dft_ot_potential,
   // dft_ot_potential2: src/ot.c:402-407, 505-510)
-  if (LOC && parts.lj < 0 && has_ig(otf)) {
+  if (LOC && parts.lj < 0 && ig_on(otf)) {
     const ig::Params igp = ig_params(otf);
     LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, 0>), S<T>(d, VRE), (const T *)rho, (const T *)nullptr, igp, d->n);
   }
@@ -1515,7 +1517,7 @@ void fused_impl(dft_ot_functional *otf, const cplx<T> *psi, double wf_mass, T *r
     LAUNCH_PW("ot_stage_kc_terms_bf_div", 5, (k_stageC<T, false, true>), ci, co, pw);
   }
   // ---- ideal Bose gas of the thermal models: pointwise, into V_re
-  if (has_ig(otf)) {
+  if (ig_on(otf)) {
     const ig::Params igp = ig_params(otf);
     LAUNCH_PW("ot_ideal_gas", 3, (k_ideal_gas<T, 0>), (T *)d->slot[VRE], (const T *)rho, (const T *)nullptr, igp, n);
   }
@@ -1646,7 +1648,7 @@ void energy_impl(dft_ot_functional *otf, T *eps, const cplx<T> *psi, double wf_m
     else if (KC) EB(false, true, false);
     else if (BF) EB(false, false, true);
 #undef EB
-    if (LOC && has_ig(otf)) {  // src/ot-energy.c:101-104
+    if (LOC && ig_on(otf)) {  // src/ot-energy.c:101-104
       const ig::Params igp = ig_params(otf);
       LAUNCH_PW("ot_ideal_gas_energy", 3, (k_ideal_gas<T, 1>), eps, (const T *)rho, (const T *)nullptr, igp, d->n);
     }
@@ -1727,7 +1729,7 @@ template <typename T> void gradient_impl(dft_ot_functional *otf, T *grad, const 
   }
   LAUNCH_PW("ot_grad_final", KC ? 6 : 3, (k_grad_final<T>), grad, S<T>(d, VRE), S<T>(d, RB),
             KC ? S<T>(d, WX) : (T *)nullptr, S<T>(d, WY), S<T>(d, WZ), pw);
-  if (has_ig(otf)) {
+  if (ig_on(otf)) {
     const ig::Params igp = ig_params(otf);
     LAUNCH_PW("ot_ideal_gas_grad", 3, (k_ideal_gas<T, 2>), grad, rho, (const T *)nullptr, igp, d->n);
   }
@@ -1991,7 +1993,7 @@ void vjp_ot(dft_ot_functional *otf, const T *vbar, const T *rho, cplx<T> *psib, 
     T *out[1] = {w.r[ACONV]};
     adj_conv<T>(d, w, 1, in, 1, &c, out);
   }
-  if (has_ig(otf)) {
+  if (ig_on(otf)) {
     const ig::Params igp = ig_params(otf);
     LAUNCH_PW("ot_adjoint_ideal_gas", 3, (k_ideal_gas<T, 3>), w.r[ARP], rho, vbar, igp, d->n);
   }
