@@ -263,3 +263,28 @@ def test_models_accepted_and_refused():
         f = L.Functional(m, w, 2, 8)
         f.free()
     w.free()
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PNAME.get)
+def test_switch_on_512_point_lines(oracle, prec):
+    """The switch's z-pass kernels on 512-point lines (two wavefronts per pair,
+    block-barrier exchanges): KC|HD|BF potential and energy with the widened switch."""
+    shape = (8, 8, 512)
+    o = oracle.OracleOT(11, shape, STEP, 2, 8)
+    _set_switch(SWITCH["widened"], o.p)
+    psi = inputs.smooth_modulated(shape, STEP, o.p.rho0, 0.45, seed=29)
+    w = L.Wave(shape, STEP, prec)
+    w.set(psi)
+    f = L.Functional(11, w, 2, 8)
+    _set_switch(SWITCH["widened"], f.p)
+    pot, eps = L.CField(shape, STEP, prec), L.RField(shape, STEP, prec)
+    f.potential(pot)
+    f.energy_density(eps)
+    Vo, eo = o.potential(psi), o.energy_density(psi)
+    V = pot.get()
+    _log("512 lines prec %d: Re V %.2e, Im V %.2e, eps %.2e" % (prec, relerr(V.real, Vo.real),
+                                                               relerr(V.imag, Vo.imag), relerr(eps.get(), eo)))
+    assert relerr(V.real, Vo.real) < TOL[prec]
+    assert relerr(V.imag, Vo.imag) < TOL[prec] * 10
+    assert relerr(eps.get(), eo) < TOL[prec]
+    pot.free(), eps.free(), f.free(), w.free()
