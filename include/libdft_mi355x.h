@@ -339,6 +339,21 @@ void dft_ot_energy_density(dft_ot_functional *otf, rgrid *energy_density, wf *gw
 void dft_ot_energy_density_kc(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density);
 void dft_ot_energy_density_bf(dft_ot_functional *otf, rgrid *energy_density, wf *gwf, rgrid *density);
 
+/* Ideal gas functions of the thermal models (host FP64), src/common.c:253
+   (thermal wavelength cubed), 320-374 (g_{1/2}, g_{3/2}, g_{5/2}: 256-term
+   series), 403-437 (z with g_{3/2}(z) = val, golden section to 1e-6),
+   489-530 (Bose gas energy per volume and its density derivative; params is
+   the dft_ot_functional), 452-480 (classical ideal gas). */
+REAL dft_common_lwl3(REAL mass, REAL temp);
+REAL dft_common_fit_g12(REAL z);
+REAL dft_common_fit_g32(REAL z);
+REAL dft_common_fit_g52(REAL z);
+REAL dft_common_fit_z(REAL val);
+REAL dft_common_bose_idealgas_energy(REAL rhop, void *params);
+REAL dft_common_bose_idealgas_dEdRho(REAL rhop, void *params);
+REAL dft_common_classical_idealgas_energy(REAL rhop, void *params);
+REAL dft_common_classical_idealgas_dEdRho(REAL rhop, void *params);
+
 /* Bulk closed forms, src/helium-ot-bulk.c (host FP64). */
 REAL dft_ot_bulk_energy(dft_ot_functional *otf, REAL rho);
 REAL dft_ot_bulk_dEdRho(dft_ot_functional *otf, REAL rho);

@@ -162,6 +162,11 @@ for _n in ("density", "chempot", "chempot2"):
     _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf"])
 for _n in ("dispersion", "istatic"):
     _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf", "p", "d"])
+_SIGS["dft_common_lwl3"] = ("d", ["d", "d"])
+for _n in ("g12", "g32", "g52", "z"):
+    _SIGS["dft_common_fit_" + _n] = ("d", ["d"])
+for _n in ("bose_idealgas_energy", "bose_idealgas_dEdRho", "classical_idealgas_energy", "classical_idealgas_dEdRho"):
+    _SIGS["dft_common_" + _n] = ("d", ["d", "p"])
 
 _T = {"i": C.c_int, "i64": C.c_int64, "d": C.c_double, "p": C.c_void_p, "s": C.c_char_p, "c": C.c_char, "b": C.c_byte,
       "z": Complex, "rg": _RG, "cg": C.POINTER(CGrid), "wf": C.POINTER(WF), "otf": C.POINTER(OTF)}

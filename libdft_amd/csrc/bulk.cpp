@@ -9,21 +9,52 @@
 #include "ideal_gas.hpp"
 
 namespace {
-dft::ig::Params ig_params(const dft_ot_functional *otf) {
+const double *ig_table() {
   static const struct Tab {
     double t[3 * dft::ig::kTerms];
     Tab() { dft::ig::fill_table(t); }
   } tab;
+  return tab.t;
+}
+dft::ig::Params ig_params(const dft_ot_functional *otf) {
+  const double *tab = ig_table();
   dft::ig::Params p;
-  p.tab = tab.t;
+  p.tab = tab;
   p.ckt = otf->c4 * (otf->temp / GRID_AUTOK);
   p.l3 = dft::ig::lambda3(otf->mass, otf->temp, GRID_AUTOK);
-  p.g1 = dft::ig::series(1.0, tab.t + dft::ig::kTerms);
+  p.g1 = dft::ig::series(1.0, tab + dft::ig::kTerms);
   return p;
 }
 }  // namespace
 
 extern "C" {
+
+// The ideal gas functions of src/common.c:245-530 (the thermal models' gas term).
+REAL dft_common_lwl3(REAL mass, REAL temp) { return dft::ig::lambda3(mass, temp, GRID_AUTOK); }
+REAL dft_common_fit_g12(REAL z) { return dft::ig::series(z, ig_table()); }
+REAL dft_common_fit_g32(REAL z) { return dft::ig::series(z, ig_table() + dft::ig::kTerms); }
+REAL dft_common_fit_g52(REAL z) { return dft::ig::series(z, ig_table() + 2 * dft::ig::kTerms); }
+REAL dft_common_fit_z(REAL val) {
+  const double *p32 = ig_table() + dft::ig::kTerms;
+  return dft::ig::fugacity(val, p32, dft::ig::series(1.0, p32));
+}
+REAL dft_common_bose_idealgas_energy(REAL rhop, void *params) {
+  return dft::ig::energy(rhop, ig_params((const dft_ot_functional *)params));
+}
+REAL dft_common_bose_idealgas_dEdRho(REAL rhop, void *params) {
+  return dft::ig::potential(rhop, ig_params((const dft_ot_functional *)params));
+}
+// classical ideal gas, src/common.c:452-480 (density floor 1e-14 inside the logarithm)
+REAL dft_common_classical_idealgas_energy(REAL rhop, void *params) {
+  const dft_ot_functional *otf = (const dft_ot_functional *)params;
+  const double l3 = dft::ig::lambda3(otf->mass, otf->temp, GRID_AUTOK);
+  return -rhop * (otf->temp / GRID_AUTOK) * (1.0 - std::log(rhop * l3 + 1e-14));
+}
+REAL dft_common_classical_idealgas_dEdRho(REAL rhop, void *params) {
+  const dft_ot_functional *otf = (const dft_ot_functional *)params;
+  const double l3 = dft::ig::lambda3(otf->mass, otf->temp, GRID_AUTOK);
+  return (otf->temp / GRID_AUTOK) * std::log(rhop * l3 + 1e-14);
+}
 
 // src/helium-ot-bulk.c:21-33
 REAL dft_ot_bulk_energy(dft_ot_functional *otf, REAL rho) {

@@ -139,3 +139,34 @@ def test_bulk_relations_product():
     assert abs(c * c - r0 * d2 / otf.mass) < 1e-6 * c * c
     # the sound speed of the OT functional at zero pressure, ~237 m/s (the fit's target)
     assert 200.0 < c * L.AUTOMPS < 260.0
+
+
+def test_ideal_gas_exports(oracle):
+    """The src/common.c ideal gas functions the library exports: the 256-term
+    series at z = 1 (sum of k^-s), the golden-section inverse of g_{3/2} to its
+    1e-6 bracket, the Bose gas energy and potential against the oracle's for a
+    thermal row, lambda^3 against the SI thermal wavelength of 4He, and the
+    classical gas as kT (ln(rho lambda^3)) and -rho kT (1 - ln(rho lambda^3))."""
+    lib = L.lib()
+    k = np.arange(1, 257, dtype=float)
+    for name, s in (("g12", 0.5), ("g32", 1.5), ("g52", 2.5)):
+        assert abs(getattr(lib, "dft_common_fit_" + name)(1.0) - np.sum(k ** -s)) < 1e-12 * np.sum(k ** -s)
+        assert abs(getattr(lib, "dft_common_fit_" + name)(0.3) - np.sum(0.3 ** k / k ** s)) < 1e-15
+    for z in (0.05, 0.5, 0.95):
+        assert abs(lib.dft_common_fit_z(lib.dft_common_fit_g32(z)) - z) < 1e-6
+    assert lib.dft_common_fit_z(3.0) == 1.0
+    otf = L.host_params(256 | 9)
+    o = oracle.OracleOT(256 | 9, (8, 8, 8), 1.0, 1, 1)
+    po = oracle.lib()
+    for rho_ang in (1e-4, 2e-3, 0.0218):
+        r = rho_ang * L.AUTOANG ** 3
+        assert lib.dft_common_bose_idealgas_energy(r, C.byref(otf)) == po.orc_ig_energy(o._h, r)
+        assert lib.dft_common_bose_idealgas_dEdRho(r, C.byref(otf)) == po.orc_ig_dEdRho(o._h, r)
+    h, kb, amu, bohr = 6.62607015e-34, 1.380649e-23, 1.66053906660e-27, 0.529177210903e-10
+    lam = h / np.sqrt(2 * np.pi * otf.mass * L.AUTOAMU * amu * kb * otf.temp) / bohr
+    l3 = lib.dft_common_lwl3(otf.mass, otf.temp)
+    assert abs(l3 / lam ** 3 - 1) < 1e-6
+    r, kt = 1e-3 * L.AUTOANG ** 3, otf.temp / L.AUTOK
+    assert abs(lib.dft_common_classical_idealgas_dEdRho(r, C.byref(otf)) - kt * np.log(r * l3)) < 1e-9 * kt
+    e = lib.dft_common_classical_idealgas_energy(r, C.byref(otf))
+    assert abs(e + r * kt * (1 - np.log(r * l3))) < 1e-9 * r * kt
