@@ -419,6 +419,9 @@ def main():
     ap.add_argument("--counters", nargs="+", metavar="COUNTER_CSV",
                     help="print per-kernel counter ratios from rocprofv3 --pmc counter_collection.csv files and exit")
     args = ap.parse_args()
+    if args.steps < 1 or args.profile_steps < 1 or args.warmup < 0:
+        ap.error("--steps and --profile-steps take at least 1 (the roofline needs the per-launch events), "
+                 "--warmup at least 0")
     if args.counters:
         print(summarize_counters(args.counters))
         return
