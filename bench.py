@@ -328,12 +328,13 @@ def size_sweep(steps=10, warmup=3):
     return out
 
 
-def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2, model=9):
+def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2, model=9, fd_prec=None):
     """The 512^3 released-cavity workload (h = 0.4 A, 17 A cavity with a 0.5 A wall, 2 fs
     steps; full OT): bubble radius (missing volume within a
     25.5 A window, as dft_ot_adjoint_cavity defines it) after `steps` steps and its
     derivative with respect to the initial cavity radius from the checkpointed adjoint sweep
-    (ceil(steps/every) + every stored states), against central differences of two forward runs."""
+    (ceil(steps/every) + every stored states), against central differences of two forward runs
+    (in the precision fd_prec, default the adjoint's)."""
     n, step = 512, 0.4 / L.AUTOANG
     rho0 = 0.0218360 * L.AUTOANG ** 3
     R0, wd, dt = 17.0 / L.AUTOANG, 0.5 / L.AUTOANG, 2.0 / L.AUTOFS
@@ -365,6 +366,10 @@ def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2, model
         vb = step ** 3 * np.sum(win * (1.0 - rho / rho0))
         return (3.0 * vb / (4.0 * np.pi)) ** (1.0 / 3.0)
 
+    if fd_prec is not None and fd_prec != prec:
+        f.free(), w.free()
+        w = L.Wave((n, n, n), step, fd_prec)
+        f = L.Functional(model, w)
     log("512^3 cavity: forward runs for the finite difference")
     t0 = time.perf_counter()
     d = rel_fd * R0
@@ -376,7 +381,8 @@ def cavity512_adjoint(steps=2000, every=45, prec=L.GRID_FP32, rel_fd=1e-2, model
             else "f64", "steps": steps, "every": every, "stored_states": states,
             "stored_GB": states * n ** 3 * (8 if prec == L.GRID_FP32 else 16) / 1e9,
             "R_b_A": J * L.AUTOANG, "dRb_dR0_adjoint": g, "dRb_dR0_fd": fd, "rel_err": abs(g / fd - 1),
-            "fd_step_A": d * L.AUTOANG, "adjoint_s": t_adj, "forward_s": t_fwd}
+            "fd_step_A": d * L.AUTOANG, "fd_dtype": "f64" if (fd_prec or prec) == L.GRID_FP64 else "f32",
+            "adjoint_s": t_adj, "forward_s": t_fwd}
 
 
 def accuracy(n, step, psi0, dt, k):
@@ -412,6 +418,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the per-size steps/s table")
     ap.add_argument("--cavity512-adjoint", action="store_true",
                     help="run the 512^3 released-cavity adjoint (2000 steps) against finite differences and exit")
+    ap.add_argument("--cavity-fd-f64", action="store_true",
+                    help="with --cavity512-adjoint: central differences from FP64 forward runs with a 0.1%% step")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the HBM traffic table from two rocprofv3 --pmc counter_collection.csv files and exit")
     ap.add_argument("--kernel-stats", metavar="STATS_CSV",
@@ -432,7 +440,10 @@ def main():
         print(summarize_kernel_stats(args.kernel_stats))
         return
     if args.cavity512_adjoint:
-        print(json.dumps(cavity512_adjoint()))
+        if args.cavity_fd_f64:
+            print(json.dumps(cavity512_adjoint(rel_fd=1e-3, fd_prec=L.GRID_FP64)))
+        else:
+            print(json.dumps(cavity512_adjoint()))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
