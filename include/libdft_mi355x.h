@@ -383,6 +383,31 @@ REAL dft_common_spherical_avg_k(void *arg, REAL kx, REAL ky, REAL kz);
 REAL dft_common_gaussian(void *arg, REAL x, REAL y, REAL z);
 REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z);
 
+/* Map functions for initial states and test potentials (host, for rgrid_map /
+   grid_wf_map): the complex gaussian (src/common.c:235-242), a plane-wave
+   density modulation (1214-1224, parameters as src/dft.h:55-60), the
+   Feynman-Onsager vortex-line potentials 1/(2 M r^2) with the core cut at
+   r = 0.05 bohr (1240-1290, param: the mass), vortex lines of one and two
+   quanta along x, y, z (src/initial.c:31-150) and a sharp bubble of radius
+   *prm (154-161). */
+typedef struct {
+  REAL kx, ky, kz; /* wave vector */
+  REAL a;          /* relative amplitude */
+  REAL rho;        /* background density */
+} dft_plane_wave;
+REAL_COMPLEX dft_common_cgaussian(void *arg, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_common_planewave(void *arg, REAL x, REAL y, REAL z);
+REAL dft_common_vortex_x(void *param, REAL x, REAL y, REAL z);
+REAL dft_common_vortex_y(void *param, REAL x, REAL y, REAL z);
+REAL dft_common_vortex_z(void *param, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_vortex_x_n1(void *na, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_vortex_y_n1(void *na, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_vortex_z_n1(void *na, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_vortex_x_n2(void *na, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_vortex_y_n2(void *na, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_vortex_z_n2(void *na, REAL x, REAL y, REAL z);
+REAL_COMPLEX dft_initial_bubble(void *prm, REAL x, REAL y, REAL z);
+
 /* ---- extensions ---- */
 /* One split-operator step with V = V_OT[psi] + V_ext - mu (vext may be NULL);
    equal to the reference single-step driver sequence cgrid_zero /

@@ -163,6 +163,12 @@ for _n in ("density", "chempot", "chempot2"):
 for _n in ("dispersion", "istatic"):
     _SIGS["dft_ot_bulk_" + _n] = ("d", ["otf", "p", "d"])
 _SIGS["dft_common_lwl3"] = ("d", ["d", "d"])
+for _n in ("dft_common_cgaussian", "dft_common_planewave", "dft_initial_bubble") + tuple(
+        "dft_initial_vortex_%s_n%d" % (a, q) for a in "xyz" for q in (1, 2)):
+    _SIGS[_n] = ("z", ["p", "d", "d", "d"])
+for _a in "xyz":
+    _SIGS["dft_common_vortex_" + _a] = ("d", ["p", "d", "d", "d"])
+_SIGS["dft_common_gaussian"] = ("d", ["p", "d", "d", "d"])
 for _n in ("g12", "g32", "g52", "z"):
     _SIGS["dft_common_fit_" + _n] = ("d", ["d"])
 for _n in ("bose_idealgas_energy", "bose_idealgas_dEdRho", "classical_idealgas_energy", "classical_idealgas_dEdRho"):

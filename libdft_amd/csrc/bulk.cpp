@@ -192,6 +192,61 @@ REAL dft_common_gaussian(void *arg, REAL x, REAL y, REAL z) {
   return norm * std::exp(-(x * x + y * y + z * z) * iw * iw);
 }
 
+// ---- map functions for initial states and test potentials
+namespace {
+REAL_COMPLEX cval(double re, double im) {
+  REAL_COMPLEX v;
+  __real__ v = re;
+  __imag__ v = im;
+  return v;
+}
+constexpr double kCore = 0.05;  // vortex core cut (bohr), src/common.c:1238
+}  // namespace
+
+// src/common.c:235-242 (the gaussian of dft_common_gaussian as a complex value)
+REAL_COMPLEX dft_common_cgaussian(void *arg, REAL x, REAL y, REAL z) { return cval(dft_common_gaussian(arg, x, y, z), 0.0); }
+
+// src/common.c:1214-1224: sqrt(rho) (1 + a cos(k.r))
+REAL_COMPLEX dft_common_planewave(void *arg, REAL x, REAL y, REAL z) {
+  const dft_plane_wave *p = (const dft_plane_wave *)arg;
+  const double psi = std::sqrt(p->rho);
+  return cval(psi + p->a * psi * std::cos(p->kx * x + p->ky * y + p->kz * z), 0.0);
+}
+
+// src/common.c:1240-1290: 1 / (2 M r_perp^2), r_perp at least the core cut
+static double vortex_pot(const void *param, double rp2) {
+  if (rp2 < kCore * kCore) rp2 = kCore * kCore;
+  return 1.0 / (2.0 * *(const double *)param * rp2);
+}
+REAL dft_common_vortex_x(void *param, REAL, REAL y, REAL z) { return vortex_pot(param, y * y + z * z); }
+REAL dft_common_vortex_y(void *param, REAL x, REAL, REAL z) { return vortex_pot(param, x * x + z * z); }
+REAL dft_common_vortex_z(void *param, REAL x, REAL y, REAL) { return vortex_pot(param, x * x + y * y); }
+
+// src/initial.c:31-89: (u + i v) / |(u, v)|, 0 inside the core
+static REAL_COMPLEX vortex1(double u, double v) {
+  const double d = std::sqrt(u * u + v * v);
+  return d < kCore ? cval(0.0, 0.0) : cval(u / d, v / d);
+}
+REAL_COMPLEX dft_initial_vortex_x_n1(void *, REAL, REAL y, REAL z) { return vortex1(y, z); }
+REAL_COMPLEX dft_initial_vortex_y_n1(void *, REAL x, REAL, REAL z) { return vortex1(x, z); }
+REAL_COMPLEX dft_initial_vortex_z_n1(void *, REAL x, REAL y, REAL) { return vortex1(x, y); }
+
+// src/initial.c:91-150: (u + i v)^2 / (u^2 + v^2), 0 where the core test holds;
+// the reference tests sqrt(x^2 + y^2) for every axis, and so does this
+static REAL_COMPLEX vortex2(double u, double v, double dx, double dy) {
+  if (std::sqrt(dx * dx + dy * dy) < kCore) return cval(0.0, 0.0);
+  const double u2 = u * u, v2 = v * v;
+  return cval((u2 - v2) / (u2 + v2), 2 * u * v / (u2 + v2));
+}
+REAL_COMPLEX dft_initial_vortex_x_n2(void *, REAL x, REAL y, REAL z) { return vortex2(y, z, x, y); }
+REAL_COMPLEX dft_initial_vortex_y_n2(void *, REAL x, REAL y, REAL z) { return vortex2(x, z, x, y); }
+REAL_COMPLEX dft_initial_vortex_z_n2(void *, REAL x, REAL y, REAL) { return vortex2(x, y, x, y); }
+
+// src/initial.c:154-161: 0 inside the radius *prm, 1 outside
+REAL_COMPLEX dft_initial_bubble(void *prm, REAL x, REAL y, REAL z) {
+  return cval(std::sqrt(x * x + y * y + z * z) < *(const double *)prm ? 0.0 : 1.0, 0.0);
+}
+
 // src/ot-private.h:11-22
 REAL dft_ot_backflow_pot(void *arg, REAL x, REAL y, REAL z) {
   const dft_ot_bf *p = (const dft_ot_bf *)arg;

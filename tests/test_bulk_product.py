@@ -170,3 +170,29 @@ def test_ideal_gas_exports(oracle):
     assert abs(lib.dft_common_classical_idealgas_dEdRho(r, C.byref(otf)) - kt * np.log(r * l3)) < 1e-9 * kt
     e = lib.dft_common_classical_idealgas_energy(r, C.byref(otf))
     assert abs(e + r * kt * (1 - np.log(r * l3))) < 1e-9 * r * kt
+
+
+def test_map_functions():
+    """The initial-state and test-potential map functions against their closed
+    forms (src/common.c:235-242, 1214-1290; src/initial.c:31-161)."""
+    lib = L.lib()
+    z = lambda c: complex(c.re, c.im)
+    iw = C.c_double(0.7)
+    g = lib.dft_common_gaussian(C.byref(iw), 0.3, -0.2, 1.1)
+    assert z(lib.dft_common_cgaussian(C.byref(iw), 0.3, -0.2, 1.1)) == complex(g, 0.0)
+    assert abs(g - (0.7 / np.sqrt(np.pi)) ** 3 * np.exp(-0.49 * (0.09 + 0.04 + 1.21))) < 1e-15
+    class PlaneWave(C.Structure):  # src/dft.h:55-60
+        _fields_ = [(n, C.c_double) for n in ("kx", "ky", "kz", "a", "rho")]
+    pw = PlaneWave(0.5, -0.3, 0.2, 0.1, 0.0218)
+    want = np.sqrt(0.0218) * (1 + 0.1 * np.cos(0.5 * 1.0 - 0.3 * 2.0 + 0.2 * 3.0))
+    assert abs(z(lib.dft_common_planewave(C.byref(pw), 1.0, 2.0, 3.0)) - want) < 1e-15
+    m = C.c_double(7296.3)
+    assert abs(lib.dft_common_vortex_z(C.byref(m), 3.0, 4.0, 9.0) - 1 / (2 * 7296.3 * 25.0)) < 1e-18
+    assert lib.dft_common_vortex_x(C.byref(m), 9.0, 0.0, 0.0) == 1 / (2 * 7296.3 * 0.05 ** 2)  # core cut
+    assert abs(z(lib.dft_initial_vortex_z_n1(None, 3.0, 4.0, 1.0)) - (3 + 4j) / 5) < 1e-15
+    assert abs(z(lib.dft_initial_vortex_x_n1(None, 1.0, 3.0, 4.0)) - (3 + 4j) / 5) < 1e-15
+    assert z(lib.dft_initial_vortex_y_n1(None, 0.01, 5.0, 0.02)) == 0
+    assert abs(z(lib.dft_initial_vortex_z_n2(None, 3.0, 4.0, 1.0)) - ((3 + 4j) / 5) ** 2) < 1e-15
+    assert abs(z(lib.dft_initial_vortex_y_n2(None, 3.0, 1.0, 4.0)) - ((3 + 4j) / 5) ** 2) < 1e-15
+    R = C.c_double(2.0)
+    assert z(lib.dft_initial_bubble(C.byref(R), 1.0, 1.0, 1.0)) == 0 and z(lib.dft_initial_bubble(C.byref(R), 2.0, 1.0, 1.0)) == 1
