@@ -20,6 +20,7 @@ LIBDIR = os.path.join(ROOT, "libdft_amd")
 DRIVERS = [
     ["imag_time/imag_time.c"],
     ["dynamics/dynamics.c"],
+    ["dispersion/dispersion.c"],
     ["dispersion/dispersion2.c"],
     ["dispersion/dispersion3.c"],
     ["dispersion/static-resp.c"],
