@@ -526,7 +526,7 @@ def main():
         out["sizes"] = size_sweep()
 
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, one step)")
+        log("cpu baseline (oracle, 1 warm-up + 3 timed steps)")
         out["cpu_baseline"], e_oracle = cpu_baseline(n, step, psi0, dt)
         # one FP32 step from the same state, kernels sampled once per cell as in the oracle run
         w = L.Wave((n, n, n), step, L.GRID_FP32)
