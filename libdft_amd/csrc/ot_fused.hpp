@@ -103,13 +103,17 @@ template <typename T> __device__ __forceinline__ T velocity_factor(const PW<T> &
   return pw.invM / (r + pw.eps_v);
 }
 template <typename T, int D, class Q>
-__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f) {
-  const cplx<T> s = q.template at<cplx<T>>(psi, w, i);
+__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f,
+                                        cplx<T> s) {
   const cplx<T> a = q.template nb<cplx<T>, D, 1>(psi, w, i), b = q.template nb<cplx<T>, D, -1>(psi, w, i);
   const T dx = (a.x - b.x) * pw.inv2h, dy = (a.y - b.y) * pw.inv2h;
   T v = (s.x * dy - s.y * dx) * f;
   if (pw.clamp_v) v = clampv(v, pw.vmax);
   return v;
+}
+template <typename T, int D, class Q>
+__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f) {
+  return velocity_d<T, D>(psi, q, w, i, pw, f, q.template at<cplx<T>>(psi, w, i));
 }
 
 template <typename T, int D, class Q>
@@ -176,16 +180,29 @@ template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { retur
 #ifndef DFT_ZC_MINB
 #define DFT_ZC_MINB 0
 #endif
+// FP32 zA with backflow keeps psi of its points in registers for the three
+// velocity axes and q4 instead of reading psi and rho again (zA 158 -> 136 us at
+// 256^3, 5 waves per SIMD either way)
+#ifndef DFT_ZA_KEEP
+#define DFT_ZA_KEEP 1
+#endif
 // HB: the backflow density switch of HD / HD2 (q_d = rho~ v_d, q4 = rho~ |v|^2
 // and a sixth output, the transform of rho~ = G rho)
 template <typename T, int N, bool BF, bool HB>
 __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
                                         const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
-  C v[E];
+  // with backflow, psi of the points is kept for the three axes (KEEP: FP32 up to
+  // N = 256, where it costs no occupancy) or read again
+  constexpr bool KEEP = BF && DFT_ZA_KEEP && sizeof(T) == 4 && N <= 256;
+  C v[E], sp[2][KEEP ? E : 1];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const C sa = q.template at<C>(psi, 0, i), sb = q.template at<C>(psi, 1, i);
+    if constexpr (KEEP) {
+      sp[0][i] = sa;
+      sp[1][i] = sb;
+    }
     v[i] = cmk<C>(sa.x * sa.x + sa.y * sa.y, sb.x * sb.x + sb.y * sb.y);
   }
   if (act) {
@@ -220,10 +237,12 @@ __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, 
         T o[2];
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
-          const C s = qq.template at<C>(psi, w, i);
+          C s;
+          if constexpr (KEEP) s = sp[w][i];
+          else s = qq.template at<C>(psi, w, i);
           T r = s.x * s.x + s.y * s.y;
           if constexpr (HB) r = rt[w][i];
-          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i]);
+          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i], s);
           v2[w][i] += vd * vd;
           o[w] = vd * r;
         }
@@ -237,6 +256,9 @@ __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, 
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       if constexpr (HB) v[i] = cmk<C>(v2[0][i] * rt[0][i], v2[1][i] * rt[1][i]);
+      else if constexpr (KEEP)
+        v[i] = cmk<C>(v2[0][i] * (sp[0][i].x * sp[0][i].x + sp[0][i].y * sp[0][i].y),
+                      v2[1][i] * (sp[1][i].x * sp[1][i].x + sp[1][i].y * sp[1][i].y));
       else v[i] = cmk<C>(v2[0][i] * q.template at<T>(rho, 0, i), v2[1][i] * q.template at<T>(rho, 1, i));
     }
     r2c_store<T, N>(v, X, s_tw, t, outs.p[4], p, P, act);
