@@ -197,7 +197,7 @@ def summarize_traffic(fetch_csv, write_csv):
     return "\n".join(lines)
 
 
-def summarize_kernel_stats(stats_csv, top=16):
+def summarize_kernel_stats(stats_csv, top=40):
     """Markdown table of a rocprofv3 --kernel-trace --stats kernel_stats.csv."""
     import csv
     rows = list(csv.DictReader(open(stats_csv)))
