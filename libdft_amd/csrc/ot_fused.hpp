@@ -102,14 +102,18 @@ template <typename T, int N, int TPL> struct PairPos {
 template <typename T> __device__ __forceinline__ T velocity_factor(const PW<T> &pw, T r) {
   return pw.invM / (r + pw.eps_v);
 }
-template <typename T, int D, class Q>
-__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f,
-                                        cplx<T> s) {
-  const cplx<T> a = q.template nb<cplx<T>, D, 1>(psi, w, i), b = q.template nb<cplx<T>, D, -1>(psi, w, i);
+// the same from psi at the point (s) and at its two neighbours (a: +d, b: -d)
+template <typename T>
+__device__ __forceinline__ T velocity_sab(const PW<T> &pw, T f, cplx<T> s, cplx<T> a, cplx<T> b) {
   const T dx = (a.x - b.x) * pw.inv2h, dy = (a.y - b.y) * pw.inv2h;
   T v = (s.x * dy - s.y * dx) * f;
   if (pw.clamp_v) v = clampv(v, pw.vmax);
   return v;
+}
+template <typename T, int D, class Q>
+__device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f,
+                                        cplx<T> s) {
+  return velocity_sab(pw, f, s, q.template nb<cplx<T>, D, 1>(psi, w, i), q.template nb<cplx<T>, D, -1>(psi, w, i));
 }
 template <typename T, int D, class Q>
 __device__ __forceinline__ T velocity_d(const void *psi, const Q &q, int w, int i, const PW<T> &pw, T f) {
@@ -186,6 +190,18 @@ template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { retur
 #ifndef DFT_ZA_KEEP
 #define DFT_ZA_KEEP 1
 #endif
+// ... and takes the z neighbours of those points from the line's other lanes
+// (zA 135 -> 122 us at 256^3, bit-identical; at 4 waves per SIMD, where it
+// needs no spills)
+#ifndef DFT_ZA_ZSHFL
+#define DFT_ZA_ZSHFL 1
+#endif
+#ifndef DFT_ZA_MINB_SH
+#define DFT_ZA_MINB_SH 4
+#endif
+template <typename T, int N> constexpr int za_minb() {
+  return sizeof(T) != 4 ? 0 : (DFT_ZA_KEEP && DFT_ZA_ZSHFL && N <= 256 ? DFT_ZA_MINB_SH : DFT_ZA_MINB);
+}
 // HB: the backflow density switch of HD / HD2 (q_d = rho~ v_d, q4 = rho~ |v|^2
 // and a sixth output, the transform of rho~ = G rho)
 template <typename T, int N, bool BF, bool HB>
@@ -232,6 +248,28 @@ __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, 
     auto qd = [&](auto dtag) {
       constexpr int D = decltype(dtag)::value;
       const auto qq = q.here();
+      // z neighbours of the kept points from the line's other lanes: point
+      // t + TPL i has z + 1 at lane t + 1 (index i), or at lane 0 (index i + 1)
+      // for t = TPL - 1; z - 1 likewise (wrapping around the line)
+      constexpr bool ZSH = KEEP && D == 2 && DFT_ZA_ZSHFL && TPL <= 64;
+      C zp[2][ZSH ? E : 1], zm[2][ZSH ? E : 1];
+      if constexpr (ZSH) {
+        const int up = (t + 1) & (TPL - 1), dn = (t + TPL - 1) & (TPL - 1);
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          C su[E], sd[E];
+#pragma unroll
+          for (int i = 0; i < E; ++i) {
+            su[i] = cmk<C>(__shfl(sp[w][i].x, up, TPL), __shfl(sp[w][i].y, up, TPL));
+            sd[i] = cmk<C>(__shfl(sp[w][i].x, dn, TPL), __shfl(sp[w][i].y, dn, TPL));
+          }
+#pragma unroll
+          for (int i = 0; i < E; ++i) {
+            zp[w][i] = t == TPL - 1 ? su[(i + 1) % E] : su[i];
+            zm[w][i] = t == 0 ? sd[(i + E - 1) % E] : sd[i];
+          }
+        }
+      }
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         T o[2];
@@ -242,7 +280,9 @@ __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, 
           else s = qq.template at<C>(psi, w, i);
           T r = s.x * s.x + s.y * s.y;
           if constexpr (HB) r = rt[w][i];
-          const T vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i], s);
+          T vd;
+          if constexpr (ZSH) vd = velocity_sab(pw, fv[w][i], s, zp[w][i], zm[w][i]);
+          else vd = velocity_d<T, D>(psi, qq, w, i, pw, fv[w][i], s);
           v2[w][i] += vd * vd;
           o[w] = vd * r;
         }
@@ -272,7 +312,7 @@ __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, 
   }
 }
 template <typename T, int N, bool BF>
-__global__ void __launch_bounds__((PairBS<T, N>), sizeof(T) == 4 ? DFT_ZA_MINB : 0)
+__global__ void __launch_bounds__((PairBS<T, N>), (za_minb<T, N>()))
     k_zA(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
          const cplx<T> *__restrict__ g_tw) {
   zA_body<T, N, BF, false>(psi, rho, outs, npairs, P, pw, g_tw);
