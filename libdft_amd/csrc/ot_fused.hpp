@@ -199,6 +199,11 @@ template <typename C> __device__ __forceinline__ auto &part(C &c, int w) { retur
 #ifndef DFT_ZA_MINB_SH
 #define DFT_ZA_MINB_SH 4
 #endif
+// the same for FP64 (132 -> 192 VGPRs, 3 -> 2 waves per SIMD; zA 257 -> 243 us
+// at 256^3)
+#ifndef DFT_ZA_KEEP64
+#define DFT_ZA_KEEP64 1
+#endif
 template <typename T, int N> constexpr int za_minb() {
   return sizeof(T) != 4 ? 0 : (DFT_ZA_KEEP && DFT_ZA_ZSHFL && N <= 256 ? DFT_ZA_MINB_SH : DFT_ZA_MINB);
 }
@@ -208,9 +213,9 @@ template <typename T, int N, bool BF, bool HB>
 __device__ __forceinline__ void zA_body(const cplx<T> *psi, T *rho, Ptrs8 outs, int64_t npairs, int P, PW<T> pw,
                                         const cplx<T> *__restrict__ g_tw) {
   FUSED_PAIR_SETUP();
-  // with backflow, psi of the points is kept for the three axes (KEEP: FP32 up to
-  // N = 256, where it costs no occupancy) or read again
-  constexpr bool KEEP = BF && DFT_ZA_KEEP && sizeof(T) == 4 && N <= 256;
+  // with backflow, psi of the points is kept for the three axes (KEEP: up to
+  // N = 256; at 512 FP32 it would spill) or read again
+  constexpr bool KEEP = BF && DFT_ZA_KEEP && (sizeof(T) == 4 || DFT_ZA_KEEP64) && N <= 256;
   C v[E], sp[2][KEEP ? E : 1];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
